@@ -1,0 +1,813 @@
+// capi.hip -- the C ABI (include/gexs_hip.h) over the gfx950 kernels.
+//
+// BaseManager semantics (Managers/BaseManager.h:22-35, as GpuManager implements
+// them in Managers/Accel/GPU/GPUManager.cpp):
+//   transfer_lines  uploads one chunk, sorts it by nu on the device, resets the
+//                   execute counter (GPUManager.cpp:149)
+//   execute         asynchronous; result grid = number of executes since the
+//                   last transfer (one grid per (T,P), GPUManager.cpp:186-201)
+//   transfer_results waits and copies one grid out (GPUManager.cpp:280-287)
+// Unlike GpuManager the line data are never modified in place (the reference
+// rescales gamma in place per (T,P) on concurrent streams,
+// cross_kernal_v2.cu:136-146); per-(T,P) values go to separate records.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gexs_hip.h"
+#include "gexs_internal.h"
+#include "kernels.h"
+#include "serve.h"
+
+namespace {
+
+struct DevArr {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevArr() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t b) {
+        if (b <= bytes && p) return hipSuccess;
+        release();
+        size_t want = std::max<size_t>(b, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = want;
+        return hipSuccess;
+    }
+    template <typename T>
+    T* as() const { return (T*)p; }
+};
+
+struct PosKey {
+    double cutoff = NAN, start = NAN, dfreq = NAN;
+    int win_lo = -1, win_hi = -1;
+    int64_t n = -1;
+    uint64_t gen = 0;
+    bool operator==(const PosKey& o) const {
+        return cutoff == o.cutoff && start == o.start && dfreq == o.dfreq && win_lo == o.win_lo &&
+               win_hi == o.win_hi && n == o.n && gen == o.gen;
+    }
+};
+
+}  // namespace
+
+struct gexs_ctx {
+    int device = 0;
+    int profile = GEXS_VOIGT;
+    hipStream_t stream = nullptr;
+    // one (start, kernel begin, kernel end) event triple per timed execute
+    // since the last gexs_timing(reset); the last one feeds gexs_last_stats
+    std::vector<hipEvent_t> evpool;
+    int tcount = 0;
+    hipEvent_t ev_start = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    std::string err;
+
+    gexs_constants k{};
+    bool have_k = false;
+    int npoints = 0;
+    int num_cs = 0;
+    double start = 0.0;
+    bool have_grid = false;
+    std::vector<double*> results;
+    int64_t capacity = 0;
+    int tasks = 0;
+
+    // current lines, sorted by nu (FP64 SoA + int32 g)
+    int64_t n = 0;
+    bool has_gamma = false;
+    uint64_t gen = 0;
+    DevArr nu, E, A, gamma, nexp, g;
+    // unsorted upload staging and sort scratch
+    DevArr r_nu, r_E, r_A, r_gamma, r_nexp, r_g, keys, perm;
+    gxk::SortBuffers sb;
+
+    // resident states (for the device partition function)
+    int64_t n_states = 0;
+    DevArr st_E, st_g, q_T, q_part, q_Q;
+
+    // derived per grid / per (T,P)
+    DevArr pos, rec, lf64, items, items_tmp, cost_key, item_idx, jobs, tp, scale, neval, counter;
+    PosKey pos_key;
+    PosKey items_key;
+    int items_span = 0, items_order = -1, items_njobs = 0, n_items = 0;
+
+    int ppl = 4;
+    int order = 1;
+    int persistent_blocks = 0;
+    gexs_stats last{};
+    bool last_neval_pending = false;
+
+    // batch graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
+    DevArr batch_out;
+
+    std::unique_ptr<gxs::Server> server;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    int hipfail(hipError_t e, const char* where) {
+        err = std::string(where) + ": " + hipGetErrorString(e);
+        return GEXS_EHIP;
+    }
+};
+
+#define HIPCHK(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e__ = (call);                            \
+        if (e__ != hipSuccess) return (ctx)->hipfail(e__, #call); \
+    } while (0)
+
+namespace {
+
+double effective_cutoff(const gexs_constants& k, double P) {
+    if (k.cutoff > 0.0) return k.cutoff;
+    return std::min(25.0 * P, 100.0);  // OpenMPManager.cpp:386
+}
+
+TPParams make_tp(const gexs_ctx* c, double T, double P, double Q, double start, double dfreq,
+                 int win_lo, int win_hi) {
+    TPParams p{};
+    p.T = T;
+    p.P = P;
+    p.Q = Q;
+    p.mass = c->k.mean_mass;
+    p.t_ref = c->k.ref_temp;
+    p.p_scale = c->k.ref_press;
+    p.cutoff = effective_cutoff(c->k, P);
+    p.start = start;
+    p.dfreq = dfreq;
+    p.gauss_hwhm = c->k.gauss_hwhm;
+    p.profile = c->profile;
+    p.width_conv = c->k.width_conv;
+    p.pressure_mode = c->k.pressure_mode;
+    p.win_lo = win_lo;
+    p.win_hi = win_hi;
+    return p;
+}
+
+constexpr int kMaxTimed = 2048;
+
+// selects the event triple of the next timed execute
+int next_events(gexs_ctx* c) {
+    int t = c->tcount < kMaxTimed ? c->tcount : kMaxTimed - 1;
+    while ((int)c->evpool.size() < 3 * (t + 1)) {
+        hipEvent_t e;
+        hipError_t err = hipEventCreate(&e);
+        if (err != hipSuccess) return c->hipfail(err, "hipEventCreate");
+        c->evpool.push_back(e);
+    }
+    c->ev_start = c->evpool[3 * t];
+    c->ev_k0 = c->evpool[3 * t + 1];
+    c->ev_k1 = c->evpool[3 * t + 2];
+    if (c->tcount < kMaxTimed) c->tcount++;
+    return GEXS_OK;
+}
+
+int set_device(gexs_ctx* c) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return c->hipfail(e, "hipSetDevice");
+    return GEXS_OK;
+}
+
+// sort the staged (r_*) lines by nu into the current arrays
+int sort_and_commit(gexs_ctx* c, int64_t n, bool has_gamma) {
+    hipStream_t s = c->stream;
+    HIPCHK(c, c->keys.ensure((size_t)n * 8));
+    HIPCHK(c, c->perm.ensure((size_t)n * 4));
+    HIPCHK(c, gxk::make_nu_keys(s, c->r_nu.as<double>(), n, c->keys.as<uint64_t>(), c->perm.as<uint32_t>()));
+    HIPCHK(c, gxk::radix_sort_u64(s, c->keys.as<uint64_t>(), c->perm.as<uint32_t>(), n, c->sb, 0, 64));
+    HIPCHK(c, c->nu.ensure((size_t)n * 8));
+    HIPCHK(c, c->E.ensure((size_t)n * 8));
+    HIPCHK(c, c->A.ensure((size_t)n * 8));
+    HIPCHK(c, c->g.ensure((size_t)n * 4));
+    const uint32_t* pm = c->perm.as<uint32_t>();
+    HIPCHK(c, gxk::gather_f64(s, c->r_nu.as<double>(), pm, n, c->nu.as<double>()));
+    HIPCHK(c, gxk::gather_f64(s, c->r_E.as<double>(), pm, n, c->E.as<double>()));
+    HIPCHK(c, gxk::gather_f64(s, c->r_A.as<double>(), pm, n, c->A.as<double>()));
+    HIPCHK(c, gxk::gather_i32(s, c->r_g.as<int32_t>(), pm, n, c->g.as<int32_t>()));
+    if (has_gamma) {
+        HIPCHK(c, c->gamma.ensure((size_t)n * 8));
+        HIPCHK(c, c->nexp.ensure((size_t)n * 8));
+        HIPCHK(c, gxk::gather_f64(s, c->r_gamma.as<double>(), pm, n, c->gamma.as<double>()));
+        HIPCHK(c, gxk::gather_f64(s, c->r_nexp.as<double>(), pm, n, c->nexp.as<double>()));
+    }
+    c->n = n;
+    c->has_gamma = has_gamma;
+    c->gen++;
+    c->tasks = 0;
+    return GEXS_OK;
+}
+
+// window + nearest point of every current line for one grid window
+int ensure_pos(gexs_ctx* c, double cutoff, double start, double dfreq, int win_lo, int win_hi) {
+    PosKey key;
+    key.cutoff = cutoff;
+    key.start = start;
+    key.dfreq = dfreq;
+    key.win_lo = win_lo;
+    key.win_hi = win_hi;
+    key.n = c->n;
+    key.gen = c->gen;
+    if (key == c->pos_key) return GEXS_OK;
+    HIPCHK(c, c->pos.ensure((size_t)std::max<int64_t>(c->n, 1) * sizeof(LinePos)));
+    HIPCHK(c, gxk::line_pos(c->stream, c->nu.as<double>(), (int)c->n, cutoff, start, dfreq, win_lo,
+                            win_hi, c->pos.as<LinePos>()));
+    c->pos_key = key;
+    c->items_key = PosKey();  // tiles depend on positions
+    return GEXS_OK;
+}
+
+// work items for njobs jobs that share one grid window; sorted heaviest first
+int ensure_items(gexs_ctx* c, int njobs) {
+    const int span = GX_WAVE * c->ppl;
+    const PosKey key = c->pos_key;
+    if (key == c->items_key && c->items_span == span && c->items_order == c->order &&
+        c->items_njobs == njobs)
+        return GEXS_OK;
+    const int win_lo = key.win_lo, win_hi = key.win_hi;
+    const int per = win_hi > win_lo ? (win_hi - win_lo + span - 1) / span : 0;
+    const int total = per * njobs;
+    HIPCHK(c, c->items.ensure((size_t)std::max(total, 1) * sizeof(WorkItem)));
+    HIPCHK(c, c->items_tmp.ensure((size_t)std::max(total, 1) * sizeof(WorkItem)));
+    HIPCHK(c, c->cost_key.ensure((size_t)std::max(total, 1) * 4));
+    HIPCHK(c, c->item_idx.ensure((size_t)std::max(total, 1) * 4));
+    const bool sorted = c->order == 1 && total > 1;
+    WorkItem* dst = sorted ? c->items_tmp.as<WorkItem>() : c->items.as<WorkItem>();
+    for (int j = 0; j < njobs; ++j) {
+        int nt = 0;
+        HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), (int)c->n, win_lo, win_hi, span, j, dst,
+                             sorted ? c->cost_key.as<uint32_t>() : nullptr,
+                             sorted ? c->item_idx.as<uint32_t>() : nullptr, j * per, &nt));
+    }
+    if (sorted) {
+        HIPCHK(c, gxk::radix_sort_u32(c->stream, c->cost_key.as<uint32_t>(), c->item_idx.as<uint32_t>(),
+                                      total, c->sb, 0, 32));
+        HIPCHK(c, gxk::permute_items(c->stream, c->items_tmp.as<WorkItem>(), c->item_idx.as<uint32_t>(),
+                                     total, c->items.as<WorkItem>()));
+    }
+    c->items_key = key;
+    c->items_span = span;
+    c->items_order = c->order;
+    c->items_njobs = njobs;
+    c->n_items = total;
+    return GEXS_OK;
+}
+
+int upload_tp(gexs_ctx* c, const std::vector<TPParams>& tps) {
+    HIPCHK(c, c->tp.ensure(tps.size() * sizeof(TPParams)));
+    HIPCHK(c, hipMemcpyAsync(c->tp.p, tps.data(), tps.size() * sizeof(TPParams), hipMemcpyHostToDevice,
+                             c->stream));
+    return GEXS_OK;
+}
+
+// The FP32C profile pass for njobs (T,P) over the current lines and one
+// window; outs[j] is the grid of job j (points indexed from 0).
+int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<double*>& outs,
+              bool timed) {
+    const int njobs = (int)tps.size();
+    const int n = (int)c->n;
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = upload_tp(c, tps)) != GEXS_OK) return rc;
+    HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRec)));
+    HIPCHK(c, c->scale.ensure((size_t)njobs * 8));
+    HIPCHK(c, c->neval.ensure((size_t)njobs * 8));
+    HIPCHK(c, c->counter.ensure(256));
+    HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 1, 0ull));
+    const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
+    const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(), n,
+                            c->pos.as<LinePos>(), c->tp.as<TPParams>(), njobs,
+                            c->scale.as<unsigned long long>()));
+    HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            gam, nex, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(), njobs,
+                            c->scale.as<unsigned long long>(), c->rec.as<VRec>(), n,
+                            c->neval.as<unsigned long long>()));
+    std::vector<Job> jobs((size_t)njobs);
+    for (int j = 0; j < njobs; ++j) {
+        jobs[j].pos = c->pos.as<LinePos>();
+        jobs[j].rec = c->rec.as<VRec>() + (size_t)j * n;
+        jobs[j].out = outs[j];
+        jobs[j].n = n;
+        jobs[j].profile = c->profile;
+        jobs[j].scale_slot = j;
+        jobs[j].pad = 0;
+    }
+    HIPCHK(c, c->jobs.ensure(jobs.size() * sizeof(Job)));
+    HIPCHK(c, hipMemcpyAsync(c->jobs.p, jobs.data(), jobs.size() * sizeof(Job), hipMemcpyHostToDevice, s));
+    if ((rc = ensure_items(c, njobs)) != GEXS_OK) return rc;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->items.as<WorkItem>(), c->n_items,
+                                 c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
+                                 c->counter.as<unsigned int>(), c->persistent_blocks));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+    c->last.n_tiles = c->n_items;
+    return GEXS_OK;
+}
+
+int run_fp64(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
+    const int n = (int)c->n;
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = upload_tp(c, std::vector<TPParams>{tp})) != GEXS_OK) return rc;
+    HIPCHK(c, c->lf64.ensure((size_t)std::max(n, 1) * sizeof(LineF64)));
+    HIPCHK(c, c->neval.ensure(8));
+    HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), 1, 0ull));
+    const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
+    const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    HIPCHK(c, gxk::line_f64(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            gam, nex, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(),
+                            c->lf64.as<LineF64>(), c->neval.as<unsigned long long>()));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
+    HIPCHK(c, gxk::fp64_points(s, c->profile, c->nu.as<double>(), c->pos.as<LinePos>(), c->lf64.as<LineF64>(), n,
+                               tp.start, tp.dfreq, tp.win_lo, tp.win_hi, out));
+    if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+    c->last.n_tiles = 0;
+    return GEXS_OK;
+}
+
+bool valid_profile(int p) {
+    return p == GEXS_GAUSSIAN || p == GEXS_DOPPLER || p == GEXS_VOIGT || p == GEXS_VOIGT_QUAD;
+}
+
+}  // namespace
+
+// =====================================================================  C ABI
+extern "C" {
+
+int gexs_create(int device, int profile, gexs_ctx** out) {
+    if (!out) return GEXS_EINVAL;
+    *out = nullptr;
+    if (!valid_profile(profile)) return GEXS_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GEXS_EHIP;
+    if (device < 0 || device >= ndev) return GEXS_EINVAL;
+    gexs_ctx* c = new (std::nothrow) gexs_ctx();
+    if (!c) return GEXS_ENOMEM;
+    c->device = device;
+    c->profile = profile;
+    if (set_device(c) != GEXS_OK || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        next_events(c) != GEXS_OK) {
+        delete c;
+        return GEXS_EHIP;
+    }
+    c->tcount = 0;
+    // defaults of the reference (Input.cpp:2-15, BaseProfile.cpp:49-54, defines.h)
+    c->k.half_width = 100.0;
+    c->k.temperature = 296.0;
+    c->k.partition = 1.0;
+    c->k.dfreq = 0.01;
+    c->k.mean_mass = 18.0;
+    c->k.pressure = 1.0;
+    c->k.ref_temp = 296.0;
+    c->k.ref_press = 1.0;
+    c->k.cutoff = 0.0;
+    c->k.gauss_hwhm = 0.1;
+    c->k.width_conv = 0;
+    c->k.pressure_mode = 1;
+    c->k.precision = GEXS_FP32C;
+    *out = c;
+    return GEXS_OK;
+}
+
+int gexs_destroy(gexs_ctx* c) {
+    if (!c) return GEXS_EINVAL;
+    (void)hipSetDevice(c->device);
+    c->server.reset();
+    (void)hipStreamSynchronize(c->stream);
+    for (double* p : c->results) (void)hipFree(p);
+    c->results.clear();
+    gxk::sort_buffers_free(c->sb);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GEXS_OK;
+}
+
+int gexs_init_vectors(gexs_ctx* c, int npoints, int num_cs, int64_t mem_budget,
+                      int64_t* line_capacity) {
+    if (!c || npoints <= 0 || num_cs <= 0) return GEXS_EINVAL;
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (double* p : c->results) (void)hipFree(p);
+    c->results.assign((size_t)num_cs, nullptr);
+    for (int i = 0; i < num_cs; ++i) {
+        hipError_t e = hipMalloc(&c->results[i], (size_t)npoints * 8);
+        if (e != hipSuccess) return c->fail(GEXS_ENOMEM, "result grids do not fit");
+        HIPCHK(c, gxk::fill_f64(c->stream, c->results[i], npoints, 0.0));
+    }
+    c->npoints = npoints;
+    c->num_cs = num_cs;
+    // line capacity: resident bytes per line = sorted SoA (40) + staging (40)
+    // + sort keys (12) + positions (16) + one FP32 record (32) + FP64 record (24)
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+    int64_t budget = mem_budget > 0 ? mem_budget : (int64_t)(0.9 * (double)free_b);
+    int64_t per_line = 40 + 40 + 12 + 12 + 16 + 32 + 24;
+    int64_t cap = budget / per_line;
+    cap = std::min<int64_t>(cap, (int64_t)0x7FFFFFFF);  // GetNtrans() is held in an int (BaseProfile.h:38)
+    if (cap <= 0) return c->fail(GEXS_ENOMEM, "no room for lines");
+    c->capacity = cap;
+    if (line_capacity) *line_capacity = cap;
+    return GEXS_OK;
+}
+
+int gexs_init_constants(gexs_ctx* c, const gexs_constants* k) {
+    if (!c || !k) return GEXS_EINVAL;
+    if (!(k->dfreq > 0.0) || !(k->mean_mass > 0.0) || !(k->ref_temp > 0.0))
+        return c->fail(GEXS_EINVAL, "dfreq, mean_mass and ref_temp must be positive");
+    if (k->precision != GEXS_FP32C && k->precision != GEXS_FP64)
+        return c->fail(GEXS_EINVAL, "unknown precision");
+    if ((c->profile == GEXS_GAUSSIAN) && !(k->gauss_hwhm > 0.0))
+        return c->fail(GEXS_EINVAL, "GAUSSIAN needs gauss_hwhm > 0");
+    c->k = *k;
+    c->have_k = true;
+    return GEXS_OK;
+}
+
+int gexs_transfer_freq(gexs_ctx* c, const double* freq, int n) {
+    if (!c || !freq || n <= 0) return GEXS_EINVAL;
+    if (n != c->npoints) return c->fail(GEXS_ESTATE, "transfer_freq: n differs from init_vectors npoints");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    c->start = freq[0];
+    c->have_grid = true;
+    for (double* p : c->results) HIPCHK(c, gxk::fill_f64(c->stream, p, n, 0.0));
+    return GEXS_OK;
+}
+
+static int transfer_common(gexs_ctx* c, int64_t n, const double* E, const double* nu,
+                           const double* A, const int32_t* g, const double* gamma,
+                           const double* nexp, hipMemcpyKind kind) {
+    if (!c || n < 0 || (n > 0 && (!E || !nu || !A || !g))) return GEXS_EINVAL;
+    if (n > 0x7FFFFFFF) return c->fail(GEXS_EINVAL, "more than 2^31-1 lines in one chunk");
+    if (c->capacity > 0 && n > c->capacity) return c->fail(GEXS_ENOMEM, "chunk exceeds line capacity");
+    const bool needs_gamma = (c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD);
+    const bool has_gamma = gamma != nullptr && nexp != nullptr;
+    if (needs_gamma && !has_gamma)
+        return c->fail(GEXS_EINVAL, "Voigt profiles need gamma and n per line");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipStreamSynchronize(s));  // TransferVectors waits for earlier work
+    size_t nb = (size_t)std::max<int64_t>(n, 1);
+    HIPCHK(c, c->r_nu.ensure(nb * 8));
+    HIPCHK(c, c->r_E.ensure(nb * 8));
+    HIPCHK(c, c->r_A.ensure(nb * 8));
+    HIPCHK(c, c->r_g.ensure(nb * 4));
+    if (n > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->r_nu.p, nu, (size_t)n * 8, kind, s));
+        HIPCHK(c, hipMemcpyAsync(c->r_E.p, E, (size_t)n * 8, kind, s));
+        HIPCHK(c, hipMemcpyAsync(c->r_A.p, A, (size_t)n * 8, kind, s));
+        HIPCHK(c, hipMemcpyAsync(c->r_g.p, g, (size_t)n * 4, kind, s));
+    }
+    if (has_gamma) {
+        HIPCHK(c, c->r_gamma.ensure(nb * 8));
+        HIPCHK(c, c->r_nexp.ensure(nb * 8));
+        if (n > 0) {
+            HIPCHK(c, hipMemcpyAsync(c->r_gamma.p, gamma, (size_t)n * 8, kind, s));
+            HIPCHK(c, hipMemcpyAsync(c->r_nexp.p, nexp, (size_t)n * 8, kind, s));
+        }
+    }
+    if ((rc = sort_and_commit(c, n, has_gamma)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(s));  // caller's buffers are free on return
+    return GEXS_OK;
+}
+
+int gexs_transfer_lines(gexs_ctx* c, int64_t n, const double* E, const double* nu, const double* A,
+                        const int32_t* g, const double* gamma, const double* nexp) {
+    return transfer_common(c, n, E, nu, A, g, gamma, nexp, hipMemcpyHostToDevice);
+}
+
+int gexs_transfer_lines_device(gexs_ctx* c, int64_t n, const double* E, const double* nu,
+                               const double* A, const int32_t* g, const double* gamma,
+                               const double* nexp) {
+    return transfer_common(c, n, E, nu, A, g, gamma, nexp, hipMemcpyDeviceToDevice);
+}
+
+int gexs_execute(gexs_ctx* c, int npoints, int64_t nlines, int start_idx, double T, double P,
+                 double Q) {
+    if (!c) return GEXS_EINVAL;
+    if (!c->have_k || !c->have_grid) return c->fail(GEXS_ESTATE, "execute before init_constants/transfer_freq");
+    if (c->tasks >= (int)c->results.size())
+        return c->fail(GEXS_ESTATE, "more executes than result grids since the last transfer");
+    if (nlines != c->n) return c->fail(GEXS_EINVAL, "nlines must equal the transferred chunk size");
+    if (start_idx < 0 || npoints < 0 || (int64_t)start_idx + npoints > c->npoints)
+        return c->fail(GEXS_EINVAL, "window outside the grid");
+    if (!(T > 0.0) || !(Q > 0.0)) return c->fail(GEXS_EINVAL, "T and Q must be positive");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    double* out = c->results[(size_t)c->tasks];
+    c->tasks++;
+    c->last = gexs_stats{};
+    c->last.n_lines = c->n;
+    if (c->n == 0 || npoints == 0) return GEXS_OK;
+    const double dfreq = c->k.dfreq;
+    const int win_lo = start_idx, win_hi = start_idx + npoints;
+    TPParams tp = make_tp(c, T, P, Q, c->start, dfreq, win_lo, win_hi);
+    if ((rc = ensure_pos(c, tp.cutoff, c->start, dfreq, win_lo, win_hi)) != GEXS_OK) return rc;
+    if ((rc = next_events(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
+    if (c->k.precision == GEXS_FP64)
+        rc = run_fp64(c, tp, out, true);
+    else
+        rc = run_fp32c(c, std::vector<TPParams>{tp}, std::vector<double*>{out}, true);
+    c->last_neval_pending = true;
+    return rc;
+}
+
+int gexs_transfer_results(gexs_ctx* c, double* out, int n, int cs_index, int accumulate) {
+    if (!c || !out || n <= 0) return GEXS_EINVAL;
+    if (cs_index < 0 || cs_index >= (int)c->results.size()) return c->fail(GEXS_EINVAL, "bad cs_index");
+    if (n > c->npoints) return c->fail(GEXS_EINVAL, "n exceeds the grid");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!accumulate) {
+        HIPCHK(c, hipMemcpy(out, c->results[(size_t)cs_index], (size_t)n * 8, hipMemcpyDeviceToHost));
+        return GEXS_OK;
+    }
+    std::vector<double> tmp((size_t)n);
+    HIPCHK(c, hipMemcpy(tmp.data(), c->results[(size_t)cs_index], (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) out[i] += tmp[(size_t)i];
+    return GEXS_OK;
+}
+
+int gexs_ready(gexs_ctx* c) {
+    if (!c) return GEXS_EINVAL;
+    return hipStreamQuery(c->stream) == hipSuccess ? 1 : 0;
+}
+
+int gexs_sync(gexs_ctx* c) {
+    if (!c) return GEXS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GEXS_OK;
+}
+
+const char* gexs_last_error(const gexs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gexs_last_stats(gexs_ctx* c, gexs_stats* out) {
+    if (!c || !out) return GEXS_EINVAL;
+    if (c->last_neval_pending) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        unsigned long long ne = 0;
+        if (c->neval.p) HIPCHK(c, hipMemcpy(&ne, c->neval.p, 8, hipMemcpyDeviceToHost));
+        c->last.n_eval = (int64_t)ne;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) == hipSuccess) c->last.kernel_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev_start, c->ev_k1) == hipSuccess) c->last.total_ms = ms;
+        c->last_neval_pending = false;
+    }
+    *out = c->last;
+    return GEXS_OK;
+}
+
+int gexs_timing(gexs_ctx* c, int reset, double* kernel_ms, double* total_ms, int* count) {
+    if (!c) return GEXS_EINVAL;
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double ks = 0.0, ts = 0.0;
+    const int n = c->tcount;
+    if (!reset) {
+        for (int t = 0; t < n; ++t) {
+            float a = 0.f, b = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&a, c->evpool[3 * t + 1], c->evpool[3 * t + 2]));
+            HIPCHK(c, hipEventElapsedTime(&b, c->evpool[3 * t], c->evpool[3 * t + 2]));
+            ks += a;
+            ts += b;
+        }
+    } else {
+        c->tcount = 0;
+    }
+    if (kernel_ms) *kernel_ms = ks;
+    if (total_ms) *total_ms = ts;
+    if (count) *count = reset ? 0 : n;
+    return GEXS_OK;
+}
+
+int gexs_line_windows(gexs_ctx* c, int64_t n, double* nu_sorted, int32_t* ib, int32_t* ie) {
+    if (!c) return GEXS_EINVAL;
+    if (n != c->n) return c->fail(GEXS_EINVAL, "n differs from the number of current lines");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n == 0) return GEXS_OK;
+    if (nu_sorted) HIPCHK(c, hipMemcpy(nu_sorted, c->nu.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (ib || ie) {
+        if (!(c->pos_key.n == c->n && c->pos_key.gen == c->gen))
+            return c->fail(GEXS_ESTATE, "no windows for the current lines yet (execute first)");
+        std::vector<LinePos> lp((size_t)n);
+        HIPCHK(c, hipMemcpy(lp.data(), c->pos.p, (size_t)n * sizeof(LinePos), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) {
+            if (ib) ib[i] = lp[(size_t)i].ib;
+            if (ie) ie[i] = lp[(size_t)i].ie;
+        }
+    }
+    return GEXS_OK;
+}
+
+int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order) {
+    if (!c) return GEXS_EINVAL;
+    if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4 && points_per_lane != 8)
+        return c->fail(GEXS_EINVAL, "points_per_lane must be 1, 2, 4 or 8");
+    if (order != 0 && order != 1) return c->fail(GEXS_EINVAL, "order must be 0 or 1");
+    c->ppl = points_per_lane;
+    c->order = order;
+    return GEXS_OK;
+}
+
+// ------------------------------------------------------------ resident list
+int gexs_load_list(gexs_ctx* c, const gexs_states* st, const gexs_transitions* tr) {
+    if (!c || !st || !tr || st->n <= 0 || tr->n < 0 || !st->E || !st->g) return GEXS_EINVAL;
+    if (tr->n > 0 && (!tr->upper || !tr->lower || !tr->A)) return GEXS_EINVAL;
+    if (tr->n > 0x7FFFFFFF) return c->fail(GEXS_EINVAL, "more than 2^31-1 transitions");
+    if (st->n > 0xFFFFFFFFll) return c->fail(GEXS_EINVAL, "more than 2^32 states");
+    const bool has_gamma = tr->gamma != nullptr && tr->nexp != nullptr;
+    if ((c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD) && !has_gamma)
+        return c->fail(GEXS_EINVAL, "Voigt profiles need gamma and n per transition");
+    // indices are checked on the host before any device gather uses them
+    for (int64_t i = 0; i < tr->n; ++i)
+        if ((int64_t)tr->upper[i] >= st->n || (int64_t)tr->lower[i] >= st->n)
+            return c->fail(GEXS_EINVAL, "transition index outside the state table");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipStreamSynchronize(s));
+    const int64_t n = tr->n;
+    const size_t nb = (size_t)std::max<int64_t>(n, 1);
+    HIPCHK(c, c->st_E.ensure((size_t)st->n * 8));
+    HIPCHK(c, c->st_g.ensure((size_t)st->n * 4));
+    HIPCHK(c, hipMemcpyAsync(c->st_E.p, st->E, (size_t)st->n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->st_g.p, st->g, (size_t)st->n * 4, hipMemcpyHostToDevice, s));
+    c->n_states = st->n;
+    // stage transitions: indices in the key/perm buffers, FP32 A/gamma/n widened
+    DevArr up, lo, f32;
+    HIPCHK(c, up.ensure(nb * 4));
+    HIPCHK(c, lo.ensure(nb * 4));
+    HIPCHK(c, f32.ensure(nb * 4));
+    HIPCHK(c, c->r_nu.ensure(nb * 8));
+    HIPCHK(c, c->r_E.ensure(nb * 8));
+    HIPCHK(c, c->r_A.ensure(nb * 8));
+    HIPCHK(c, c->r_g.ensure(nb * 4));
+    if (n > 0) {
+        HIPCHK(c, hipMemcpyAsync(up.p, tr->upper, (size_t)n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(lo.p, tr->lower, (size_t)n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(c, gxk::assemble_lines(s, c->st_E.as<double>(), c->st_g.as<int32_t>(), up.as<uint32_t>(),
+                                      lo.as<uint32_t>(), n, c->r_nu.as<double>(), c->r_E.as<double>(),
+                                      c->r_g.as<int32_t>()));
+        HIPCHK(c, hipMemcpyAsync(f32.p, tr->A, (size_t)n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, c->r_A.as<double>()));
+        if (has_gamma) {
+            HIPCHK(c, c->r_gamma.ensure(nb * 8));
+            HIPCHK(c, c->r_nexp.ensure(nb * 8));
+            HIPCHK(c, hipStreamSynchronize(s));
+            HIPCHK(c, hipMemcpyAsync(f32.p, tr->gamma, (size_t)n * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, c->r_gamma.as<double>()));
+            HIPCHK(c, hipStreamSynchronize(s));
+            HIPCHK(c, hipMemcpyAsync(f32.p, tr->nexp, (size_t)n * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, c->r_nexp.as<double>()));
+        }
+    }
+    if ((rc = sort_and_commit(c, n, has_gamma)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(s));
+    return GEXS_OK;
+}
+
+int gexs_partition(gexs_ctx* c, const double* T, int nT, double* Q_out) {
+    if (!c || !T || !Q_out || nT <= 0) return GEXS_EINVAL;
+    if (c->n_states <= 0) return c->fail(GEXS_ESTATE, "no resident state table (gexs_load_list)");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    const int nblk = 512;
+    HIPCHK(c, c->q_T.ensure((size_t)nT * 8));
+    HIPCHK(c, c->q_part.ensure((size_t)nT * nblk * 8));
+    HIPCHK(c, c->q_Q.ensure((size_t)nT * 8));
+    HIPCHK(c, hipMemcpyAsync(c->q_T.p, T, (size_t)nT * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, gxk::partition(c->stream, c->st_E.as<double>(), c->st_g.as<int32_t>(), c->n_states,
+                             c->q_T.as<double>(), nT, c->q_part.as<double>(), nblk, c->q_Q.as<double>()));
+    HIPCHK(c, hipMemcpyAsync(Q_out, c->q_Q.p, (size_t)nT * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GEXS_OK;
+}
+
+int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid, double* out) {
+    if (!c || !tp || ntp <= 0 || !grid || !out || grid->npoints <= 0 || !(grid->end > grid->start))
+        return GEXS_EINVAL;
+    if (!c->have_k) return c->fail(GEXS_ESTATE, "run_batch before init_constants");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    const double dfreq = (grid->end - grid->start) / (double)grid->npoints;  // BaseProfile.cpp:10
+    std::vector<double> Qs((size_t)ntp);
+    std::vector<double> needT;
+    for (int i = 0; i < ntp; ++i) {
+        if (!(tp[i].T > 0.0)) return c->fail(GEXS_EINVAL, "T must be positive");
+        Qs[i] = tp[i].Q;
+        if (!(tp[i].Q > 0.0)) needT.push_back(tp[i].T);
+    }
+    if (!needT.empty()) {
+        std::vector<double> q(needT.size());
+        if ((rc = gexs_partition(c, needT.data(), (int)needT.size(), q.data())) != GEXS_OK) return rc;
+        size_t w = 0;
+        for (int i = 0; i < ntp; ++i)
+            if (!(tp[i].Q > 0.0)) Qs[i] = q[w++];
+    }
+    // one cutoff for the whole batch (positions and tiles are shared)
+    const double cutoff = effective_cutoff(c->k, tp[0].P);
+    for (int i = 1; i < ntp; ++i)
+        if (effective_cutoff(c->k, tp[i].P) != cutoff)
+            return c->fail(GEXS_EINVAL, "a batch needs one cutoff (set gexs_constants.cutoff > 0)");
+    if ((rc = ensure_pos(c, cutoff, grid->start, dfreq, 0, grid->npoints)) != GEXS_OK) return rc;
+    const size_t np = (size_t)grid->npoints;
+    HIPCHK(c, c->batch_out.ensure(np * (size_t)ntp * 8));
+    std::vector<TPParams> tps((size_t)ntp);
+    std::vector<double*> outs((size_t)ntp);
+    for (int i = 0; i < ntp; ++i) {
+        tps[i] = make_tp(c, tp[i].T, tp[i].P, Qs[i], grid->start, dfreq, 0, grid->npoints);
+        outs[i] = c->batch_out.as<double>() + (size_t)i * np;
+    }
+    HIPCHK(c, gxk::fill_f64(c->stream, c->batch_out.as<double>(), (int64_t)np * ntp, 0.0));
+    if ((rc = next_events(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
+    c->last = gexs_stats{};
+    c->last.n_lines = c->n;
+    if (c->k.precision == GEXS_FP64) {
+        for (int i = 0; i < ntp; ++i)
+            if ((rc = run_fp64(c, tps[i], outs[i], i == 0)) != GEXS_OK) return rc;
+        HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
+    } else {
+        if ((rc = run_fp32c(c, tps, outs, true)) != GEXS_OK) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // N_eval summed over the batch
+    std::vector<unsigned long long> ne((size_t)(c->k.precision == GEXS_FP64 ? 1 : ntp), 0ull);
+    HIPCHK(c, hipMemcpy(ne.data(), c->neval.p, ne.size() * 8, hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (auto v : ne) tot += (int64_t)v;
+    c->last.n_eval = tot;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) == hipSuccess) c->last.kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev_start, c->ev_k1) == hipSuccess) c->last.total_ms = ms;
+    c->last_neval_pending = false;
+    return GEXS_OK;
+}
+
+// --------------------------------------------------------------- serving
+int gexs_serve_start(gexs_ctx* c, int max_inflight, int64_t mem_budget) {
+    (void)max_inflight;
+    (void)mem_budget;
+    if (!c) return GEXS_EINVAL;
+    return c->fail(GEXS_ESTATE, "serving is not available in this build");
+}
+int gexs_submit(gexs_ctx* c, const gexs_request*, uint64_t*) {
+    if (!c) return GEXS_EINVAL;
+    return c->fail(GEXS_ESTATE, "serving not started");
+}
+int gexs_poll(gexs_ctx* c, uint64_t, gexs_result*) {
+    if (!c) return GEXS_EINVAL;
+    return c->fail(GEXS_ESTATE, "serving not started");
+}
+int gexs_wait(gexs_ctx* c, uint64_t, gexs_result*) {
+    if (!c) return GEXS_EINVAL;
+    return c->fail(GEXS_ESTATE, "serving not started");
+}
+int gexs_release(gexs_ctx* c, uint64_t) {
+    if (!c) return GEXS_EINVAL;
+    return c->fail(GEXS_ESTATE, "serving not started");
+}
+int gexs_serve_stop(gexs_ctx* c) {
+    if (!c) return GEXS_EINVAL;
+    c->server.reset();
+    return GEXS_OK;
+}
+
+}  // extern "C"

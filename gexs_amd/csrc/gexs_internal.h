@@ -1,0 +1,89 @@
+// gexs_internal.h -- layouts shared by the gfx950 kernels and the host side.
+//
+// Data layout in HBM (one context):
+//   lines (sorted by nu, SoA, FP64 + int32 g)        40 B / line, resident
+//   LinePos  {ib, ie, c, cf}   per line and grid        16 B  (grid-dependent)
+//   VRec     8 words           per line and (T,P)       32 B  (FP32 profile constants)
+//   WorkItem {lo, hi, w0, w1, job}  per wave-tile       32 B
+//   result grids FP64                                    8 B / point
+#pragma once
+#include <stdint.h>
+
+// physical constants, common/defines.h:4-27 of the reference (their exact
+// values are part of the parity contract)
+#define GX_LN2 0.69314718056
+#define GX_SQRTLN2 0.832554611
+#define GX_ISQRTPI 0.564189584
+#define GX_NA 6.0221412927e23
+#define GX_CM_COEF 1.327209365e-12
+#define GX_PI 3.14159265359
+#define GX_PLANCK 6.6260693e-27
+#define GX_VELLGT 2.99792458e+10
+#define GX_BOLTZ 1.380658e-16
+
+// profile ids (common/Input.h:10-17)
+#define GX_PROF_GAUSSIAN 0
+#define GX_PROF_DOPPLER 1
+#define GX_PROF_VOIGT 2
+#define GX_PROF_VOIGT_QUAD 4
+
+// points of a wave-tile = 64 * points-per-lane
+#define GX_WAVE 64
+#define GX_WG 256
+
+// per line, per grid: window [ib, ie) clamped to the job window, nearest
+// grid point c = round((nu - start)/dfreq) (as int and exact float)
+struct __attribute__((aligned(16))) LinePos {
+    int32_t ib, ie, c;
+    float cf;
+};
+
+// per line, per (T,P): FP32 constants of the profile sum.
+// Voigt:       x_j = fma(c - j, A, B) (= (nu - f_j) / w), y = gammaL / w,
+//              region I:  K*Kg = (Kl*x^2 + Klb) / (x^2 (x^2 + 2y^2 - 1) + b2),
+//              Kl = Kg*0.5641896*y, Klb = Kl*(0.5 + y^2), b2 = (0.5 + y^2)^2,
+//              k15 = smallest |c - j| with |x| + y >= 15 for certain.
+//              Kg = S / w * scale.
+// Voigt-Quad:  A, B, y, Kg as Voigt.
+// bin-Gauss:   A, B as Voigt with 1/w = gG, y = H = gG*dfreq/2,
+//              Kg = S * 0.5/dfreq * scale.
+struct __attribute__((aligned(32))) VRec {
+    int32_t k15;
+    float A, B, y, Kl, Klb, b2, Kg;
+};
+
+struct __attribute__((aligned(32))) WorkItem {
+    int32_t lo, hi;  // line range [lo, hi) of the job's (sorted) lines
+    int32_t w0, w1;  // grid points [w0, w1)
+    int32_t job;
+    int32_t pad[3];
+};
+
+// one cross-section
+struct __attribute__((aligned(16))) Job {
+    const LinePos* pos;  // job's lines, [0, n)
+    const VRec* rec;
+    double* out;         // grid (index = grid point)
+    int32_t n;
+    int32_t profile;
+    int32_t scale_slot;  // index into the scale array (uint64 bit pattern of max amplitude)
+    int32_t pad;
+};
+
+// per-(T,P) physics of one job, FP64
+struct TPParams {
+    double T, P, Q;
+    double mass, t_ref, p_scale;
+    double cutoff;
+    double start, dfreq;
+    double gauss_hwhm;
+    int32_t profile, width_conv, pressure_mode, win_lo;
+    int32_t win_hi, pad0, pad1, pad2;
+};
+
+// plain-FP64 per (line, (T,P)) values of the baseline path
+struct LineF64 {
+    double Kg;    // S / w   (bin-Gauss: S)
+    double y;     // gammaL / w   (bin-Gauss: unused)
+    double inv_w; // 1 / w
+};

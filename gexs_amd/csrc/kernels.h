@@ -1,0 +1,76 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (kernels.hip, sort.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gexs_internal.h"
+
+namespace gxk {
+
+hipError_t line_pos(hipStream_t s, const double* nu, int n, double cutoff, double start,
+                    double dfreq, int win_lo, int win_hi, LinePos* pos);
+hipError_t line_max(hipStream_t s, const double* nu, const double* E, const double* A,
+                    const int* g, int n, const LinePos* pos, const TPParams* tp, int ntp,
+                    unsigned long long* scale_bits);
+hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
+                    const int* g, const double* gamma, const double* nexp, int n,
+                    const LinePos* pos, const TPParams* tp, int ntp,
+                    const unsigned long long* scale_bits, VRec* rec, int64_t rec_stride,
+                    unsigned long long* neval);
+hipError_t tiles(hipStream_t s, const LinePos* pos, int n, int win_lo, int win_hi, int span,
+                 int job, WorkItem* items, uint32_t* cost_key, uint32_t* item_idx,
+                 int item_base, int* ntiles_out);
+hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx, int n,
+                         WorkItem* out);
+hipError_t profile_tiles(hipStream_t s, int profile, int ppl, const WorkItem* items, int nitems,
+                         const Job* jobs, const unsigned long long* scale_bits,
+                         unsigned int* counter, int persistent_blocks);
+hipError_t line_f64(hipStream_t s, const double* nu, const double* E, const double* A,
+                    const int* g, const double* gamma, const double* nexp, int n,
+                    const LinePos* pos, const TPParams* tp, LineF64* out,
+                    unsigned long long* neval);
+hipError_t fp64_points(hipStream_t s, int profile, const double* nu, const LinePos* pos,
+                       const LineF64* lf, int n, double start, double dfreq, int win_lo,
+                       int win_hi, double* out);
+hipError_t partition(hipStream_t s, const double* E, const int* g, int64_t n, const double* dT,
+                     int nT, double* partial, int nblk, double* dQ);
+hipError_t fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v);
+hipError_t fill_f64(hipStream_t s, double* p, int64_t n, double v);
+hipError_t add_f64(hipStream_t s, double* dst, const double* src, int64_t n);
+
+// ---- sort.hip: stable LSD radix sort of (key, uint32 value) pairs.
+// Keys are uint64 (order-preserving image of a double) or uint32.
+struct SortBuffers {
+    void* keys_alt = nullptr;      // n keys
+    uint32_t* vals_alt = nullptr;  // n values
+    uint32_t* hist = nullptr;      // 256 * nblk
+    uint32_t* totals = nullptr;    // 256 (device)
+    uint32_t* totals_host = nullptr;  // 256 (pinned host)
+    int64_t cap = 0;
+    int key_bytes = 0;
+};
+size_t sort_scratch_bytes(int64_t n, int key_bytes);
+// Sorts keys/vals in place (results end in keys/vals).  Blocks on the stream
+// once per pass to skip passes whose digit is constant.
+hipError_t radix_sort_u64(hipStream_t s, uint64_t* keys, uint32_t* vals, int64_t n,
+                          SortBuffers& buf, int begin_bit, int end_bit);
+hipError_t radix_sort_u32(hipStream_t s, uint32_t* keys, uint32_t* vals, int64_t n,
+                          SortBuffers& buf, int begin_bit, int end_bit);
+hipError_t sort_buffers_alloc(SortBuffers& buf, int64_t n, int key_bytes);
+void sort_buffers_free(SortBuffers& buf);
+
+// nu (double) -> order-preserving uint64 key, and the identity permutation
+hipError_t make_nu_keys(hipStream_t s, const double* nu, int64_t n, uint64_t* keys,
+                        uint32_t* vals);
+// out[i] = in[perm[i]] for 8- and 4-byte elements
+hipError_t gather_f64(hipStream_t s, const double* in, const uint32_t* perm, int64_t n,
+                      double* out);
+hipError_t gather_i32(hipStream_t s, const int32_t* in, const uint32_t* perm, int64_t n,
+                      int32_t* out);
+// assemble lines of a resident list from states + transitions
+hipError_t assemble_lines(hipStream_t s, const double* sE, const int32_t* sg,
+                          const uint32_t* up, const uint32_t* lo, int64_t n, double* nu,
+                          double* E, int32_t* g);
+hipError_t widen_f32(hipStream_t s, const float* in, int64_t n, double* out);
+
+}  // namespace gxk

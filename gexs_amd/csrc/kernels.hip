@@ -1,0 +1,659 @@
+// kernels.hip -- gfx950 kernels of the line-by-line cross-section path.
+//
+// Per (T,P) the path is
+//   k_line_max      FP64 line amplitudes -> max (power-of-two scale of the FP32 sums)
+//   k_line_rec      FP64 S, gammaL, 1/w, window -> 32-B FP32 record per line, N_eval
+//   k_profile_tiles one wavefront per wave-tile of 64*PPL grid points; walks the
+//                   nu-sorted lines whose window meets the tile with scalar loads,
+//                   evaluates the profile in FP32 and keeps per-point Kahan sums in
+//                   registers; writes each point once (no atomics on the grid)
+// Grid-dependent, T,P-independent steps (once per chunk and grid):
+//   k_line_pos      window [ib, ie) and nearest point c per line (FP64, bit-exact
+//                   to OpenMPManager.cpp:399-400)
+//   k_tiles         binary-searched line range per wave-tile -> work items
+// Baseline:
+//   k_fp64_points   one work-item per grid point, binary-searched line range,
+//                   FP64 W4 and FP64 sum (the "plain FP64" GPU path)
+// Partition function:
+//   k_partition / k_partition_final  FP64 sum of g exp(-beta E) in fixed order
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "gexs_internal.h"
+#include "kernels.h"
+#include "profiles_device.h"
+
+namespace {
+
+// ----------------------------------------------------------- FP64 physics
+// Operation order follows Managers/CPU/OpenMPManager.cpp:385-404.
+__device__ __forceinline__ double dpw_coef(const TPParams& p) {
+    return sqrt(2.0 * GX_LN2 * GX_BOLTZ * p.T * GX_NA / (p.mass)) / GX_VELLGT;
+}
+
+__device__ __forceinline__ double line_strength(const TPParams& p, double nu, double E, double A,
+                                                int g) {
+#pragma clang fp contract(off)
+    double cmcoef = 1.0 / (8.0 * GX_PI * GX_VELLGT);
+    double beta = GX_PLANCK * GX_VELLGT / (GX_BOLTZ * p.T);
+    double v = nu == 0.0 ? 1e-6 : nu;
+    double s = cmcoef * A * g;
+    if (p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD) s = s * GX_ISQRTPI;
+    return s * exp(-beta * E) * (1.0 - exp(-beta * v)) / (v * v * p.Q);
+}
+
+__device__ __forceinline__ double inv_width(const TPParams& p, double nu) {
+    if (p.profile == GX_PROF_GAUSSIAN) return GX_SQRTLN2 / p.gauss_hwhm;
+    double v = nu == 0.0 ? 1e-6 : nu;
+    if (p.width_conv) return GX_SQRTLN2 / (v * dpw_coef(p));
+    return 1.0 / (v * dpw_coef(p));
+}
+
+__device__ __forceinline__ double gamma_l(const TPParams& p, double gamma, double nexp) {
+    double gl = gamma * pow(p.t_ref / p.T, nexp);
+    if (p.pressure_mode) gl = gl * (p.P * p.p_scale);
+    return gl;
+}
+
+// amplitude that multiplies the FP32 profile value of one line
+__device__ __forceinline__ double line_amp(const TPParams& p, double S, double iw) {
+    if (p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD) return S * iw;
+    return S * (0.5 / p.dfreq);
+}
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double v) {
+    // positive doubles order like their bit patterns
+    if (v > 0.0 && isfinite(v)) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- k_line_pos
+__global__ __launch_bounds__(256) void k_line_pos(const double* __restrict__ nu, int n,
+                                                  double cutoff, double start, double dfreq,
+                                                  int win_lo, int win_hi,
+                                                  LinePos* __restrict__ pos) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = nu[i] == 0.0 ? 1e-6 : nu[i];
+    // OpenMPManager.cpp:399-400: round(((nu - cut) - start)/dnu), clamped
+    double b = round((v - cutoff - start) / dfreq);
+    double e = round((v + cutoff - start) / dfreq);
+    double lo = (double)win_lo, hi = (double)win_hi;
+    double bb = (b < lo) ? lo : b;
+    double ee = (hi < e) ? hi : e;
+    double c = round((v - start) / dfreq);
+    // keep c within int range; far-away lines never reach the evaluation
+    c = fmin(fmax(c, -1.0e9), 1.0e9);
+    LinePos lp;
+    lp.ib = (int)bb;
+    lp.ie = (int)ee;
+    lp.c = (int)c;
+    lp.cf = (float)c;
+    pos[i] = lp;
+}
+
+// ------------------------------------------------------------- k_line_max
+__global__ __launch_bounds__(256) void k_line_max(const double* __restrict__ nu,
+                                                  const double* __restrict__ E,
+                                                  const double* __restrict__ A,
+                                                  const int* __restrict__ g, int n,
+                                                  const LinePos* __restrict__ pos,
+                                                  const TPParams* __restrict__ tp, int ntp,
+                                                  unsigned long long* __restrict__ scale_bits) {
+    int k = blockIdx.y;
+    const TPParams p = tp[k];
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const LinePos lp = pos[i];
+        if (lp.ie <= lp.ib) continue;  // only lines that reach the grid set the scale
+        double S = line_strength(p, nu[i], E[i], A[i], g[i]);
+        double amp = line_amp(p, S, inv_width(p, nu[i]));
+        if (isfinite(amp)) m = fmax(m, amp);
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(&scale_bits[k], m);
+}
+
+// power-of-two scale so that the largest amplitude maps into [0.5, 1)
+__device__ __forceinline__ double scale_of(unsigned long long bits) {
+    double m = __longlong_as_double((long long)bits);
+    if (!(m > 0.0)) return 1.0;
+    int ex;
+    frexp(m, &ex);
+    return ldexp(1.0, -ex);
+}
+
+// ------------------------------------------------------------- k_line_rec
+__global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
+                                                  const double* __restrict__ E,
+                                                  const double* __restrict__ A,
+                                                  const int* __restrict__ g,
+                                                  const double* __restrict__ gamma,
+                                                  const double* __restrict__ nexp, int n,
+                                                  const LinePos* __restrict__ pos,
+                                                  const TPParams* __restrict__ tp, int ntp,
+                                                  const unsigned long long* __restrict__ scale_bits,
+                                                  VRec* __restrict__ rec, int64_t rec_stride,
+                                                  unsigned long long* __restrict__ neval) {
+#pragma clang fp contract(off)
+    int k = blockIdx.y;
+    const TPParams p = tp[k];
+    const double scale = scale_of(scale_bits[k]);
+    long long ne = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double v = nu[i] == 0.0 ? 1e-6 : nu[i];
+        double S = line_strength(p, nu[i], E[i], A[i], g[i]);
+        double iw = inv_width(p, nu[i]);
+        LinePos lp = pos[i];
+        if (S != 0.0 && lp.ie > lp.ib) ne += lp.ie - lp.ib;
+        // offset of the line from its nearest grid point, FP64 (f_c = start + c*dfreq)
+        double fc = p.start + (double)lp.c * p.dfreq;
+        double delta = v - fc;
+        VRec r;
+        double Af = p.dfreq * iw;
+        double Bf = delta * iw;
+        r.A = (float)Af;
+        r.B = (float)Bf;
+        double amp = line_amp(p, S, iw) * scale;
+        if (!isfinite(amp)) amp = 0.0;
+        if (p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD) {
+            double y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
+            double b = 0.5 + y * y;
+            double kl = amp * 0.5641896 * y;
+            r.y = (float)y;
+            r.Kl = (float)kl;
+            r.Klb = (float)(kl * b);
+            r.b2 = (float)(b * b);
+            // |x| >= |k| A - |B|; region I needs |x| + y >= 15 (with margin)
+            double kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
+            r.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e9 ? 1000000000 : (int)ceil(kk));
+        } else {
+            r.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
+            r.Kl = 0.0f;
+            r.Klb = 0.0f;
+            r.b2 = 0.0f;
+            r.k15 = 0;
+        }
+        r.Kg = (float)amp;
+        rec[(int64_t)k * rec_stride + i] = r;
+    }
+    ne = wave_sum_ll(ne);
+    if (neval != nullptr && (threadIdx.x & 63) == 0 && ne != 0)
+        atomicAdd(&neval[k], (unsigned long long)ne);
+}
+
+// ---------------------------------------------------------------- k_tiles
+// One work item per wave-tile: lines whose window meets [w0, w1) form the
+// contiguous range [lo, hi) because ib and ie are non-decreasing in nu.
+__global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos, int n,
+                                               int win_lo, int win_hi, int span, int ntiles,
+                                               int job, WorkItem* __restrict__ items,
+                                               uint32_t* __restrict__ cost_key,
+                                               uint32_t* __restrict__ item_idx, int item_base) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int w0 = win_lo + t * span;
+    int w1 = min(w0 + span, win_hi);
+    // lo = first i with ie > w0
+    int a = 0, b = n;
+    while (a < b) {
+        int m = (a + b) >> 1;
+        if (pos[m].ie > w0) b = m; else a = m + 1;
+    }
+    int lo = a;
+    // hi = first i with ib >= w1
+    a = lo;
+    b = n;
+    while (a < b) {
+        int m = (a + b) >> 1;
+        if (pos[m].ib >= w1) b = m; else a = m + 1;
+    }
+    WorkItem it;
+    it.lo = lo;
+    it.hi = a;
+    it.w0 = w0;
+    it.w1 = w1;
+    it.job = job;
+    it.pad[0] = it.pad[1] = it.pad[2] = 0;
+    items[item_base + t] = it;
+    if (cost_key) {
+        uint32_t c = (uint32_t)max(a - lo, 0);
+        cost_key[item_base + t] = 0xFFFFFFFFu - c;  // ascending key = heaviest first
+        item_idx[item_base + t] = (uint32_t)(item_base + t);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_permute_items(const WorkItem* __restrict__ in,
+                                                       const uint32_t* __restrict__ idx, int n,
+                                                       WorkItem* __restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = in[idx[t]];
+}
+
+// --------------------------------------------------------- profile tiles
+namespace {
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Line records are read-only while a profile kernel runs; addressing them in
+// the constant address space lets the wave-uniform loads below issue as
+// scalar loads (s_load) into SGPRs, so per-line decisions are scalar branches.
+// (The host pass of this single-source file only parses device code.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) LinePos* ConstPos;
+typedef const __attribute__((address_space(4))) VRec* ConstRec;
+#else
+typedef const LinePos* ConstPos;
+typedef const VRec* ConstRec;
+#endif
+
+template <int PROF, int PPL>
+__device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
+                                         int lane) {
+    const ConstPos pos = (ConstPos)jb.pos;
+    const ConstRec rec = (ConstRec)jb.rec;
+    float sum[PPL], cmp[PPL], jf[PPL];
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        sum[q] = 0.0f;
+        cmp[q] = 0.0f;
+        jf[q] = (float)(it.w0 + lane + GX_WAVE * q);
+    }
+    const int w0 = it.w0, w1 = it.w1;
+    constexpr int CH = 32;  // lines per uncompensated partial sum
+    for (int i0 = it.lo; i0 < it.hi; i0 += CH) {
+        const int i1 = min(i0 + CH, it.hi);
+        float part[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) part[q] = 0.0f;
+        for (int i = i0; i < i1; ++i) {
+            const LinePos lp = pos[i];
+            const VRec r = rec[i];
+            const bool full = (lp.ib <= w0) && (lp.ie >= w1);
+            if (PROF == GX_PROF_VOIGT) {
+                const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
+                if (full && kmin >= r.k15) {
+                    // every point of the tile is in Humlicek region I
+                    const float e = fmaf(2.0f * r.y, r.y, -1.0f);
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) {
+                        float x = fmaf(lp.cf - jf[q], r.A, r.B);
+                        float x2 = x * x;
+                        float den = fmaf(x2, x2 + e, r.b2);
+                        float num = fmaf(x2, r.Kl, r.Klb);
+                        part[q] = fmaf(num, __builtin_amdgcn_rcpf(den), part[q]);
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                const int j = w0 + lane + GX_WAVE * q;
+                if (j >= lp.ib && j < lp.ie) {
+                    float x = fabsf(fmaf(lp.cf - jf[q], r.A, r.B));
+                    float k;
+                    if (PROF == GX_PROF_VOIGT) {
+                        k = gx::w4_f32(x, r.y);
+                    } else if (PROF == GX_PROF_VOIGT_QUAD) {
+                        float rr = x * x + r.y * r.y;
+                        k = rr < 100.0f ? (float)gx::voigt916_f64((double)x, (double)r.y)
+                                        : gx::gh11_f32(x, r.y);
+                    } else {
+                        k = gx::bin_erf_f32(x, r.y);
+                    }
+                    part[q] = fmaf(r.Kg, k, part[q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) kahan_add(sum[q], cmp[q], part[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        const int j = w0 + lane + GX_WAVE * q;
+        if (j < w1) jb.out[j] += ((double)sum[q] - (double)cmp[q]) * inv_scale;
+    }
+}
+
+}  // namespace
+
+template <int PROF, int PPL>
+__global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restrict__ items,
+                                                         int nitems, const Job* __restrict__ jobs,
+                                                         const unsigned long long* __restrict__ scale_bits,
+                                                         unsigned int* __restrict__ counter) {
+    const int lane = threadIdx.x & (GX_WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (counter == nullptr) {
+        const int idx = blockIdx.x * (GX_WG / GX_WAVE) + wv;
+        if (idx >= nitems) return;
+        const WorkItem it = items[idx];
+        const Job jb = jobs[it.job];
+        run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+        return;
+    }
+    // persistent: waves pull items from a counter until the list is drained
+    for (;;) {
+        unsigned int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1u);
+        const int idx = __builtin_amdgcn_readfirstlane((int)t);
+        if (idx >= nitems) return;
+        const WorkItem it = items[idx];
+        const Job jb = jobs[it.job];
+        run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+    }
+}
+
+// ------------------------------------------------------ plain FP64 baseline
+__global__ __launch_bounds__(256) void k_line_f64(const double* __restrict__ nu,
+                                                  const double* __restrict__ E,
+                                                  const double* __restrict__ A,
+                                                  const int* __restrict__ g,
+                                                  const double* __restrict__ gamma,
+                                                  const double* __restrict__ nexp, int n,
+                                                  const LinePos* __restrict__ pos,
+                                                  const TPParams* __restrict__ tp,
+                                                  LineF64* __restrict__ out,
+                                                  unsigned long long* __restrict__ neval) {
+    const TPParams p = tp[0];
+    long long ne = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double S = line_strength(p, nu[i], E[i], A[i], g[i]);
+        double iw = inv_width(p, nu[i]);
+        LinePos lp = pos[i];
+        if (S != 0.0 && lp.ie > lp.ib) ne += lp.ie - lp.ib;
+        LineF64 r;
+        if (p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD) {
+            r.Kg = S * iw;
+            r.y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
+        } else {
+            r.Kg = S;
+            r.y = 0.0;
+        }
+        r.inv_w = iw;
+        out[i] = r;
+    }
+    ne = wave_sum_ll(ne);
+    if (neval != nullptr && (threadIdx.x & 63) == 0 && ne != 0)
+        atomicAdd(neval, (unsigned long long)ne);
+}
+
+template <int PROF>
+__global__ __launch_bounds__(256) void k_fp64_points(const double* __restrict__ nu,
+                                                     const LinePos* __restrict__ pos,
+                                                     const LineF64* __restrict__ lf, int n,
+                                                     double start, double dfreq, int win_lo,
+                                                     int win_hi, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int j = win_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= win_hi) return;
+    // lines with ib <= j < ie: [first ie > j, first ib > j)
+    int a = 0, b = n;
+    while (a < b) {
+        int m = (a + b) >> 1;
+        if (pos[m].ie > j) b = m; else a = m + 1;
+    }
+    const int lo = a;
+    a = lo;
+    b = n;
+    while (a < b) {
+        int m = (a + b) >> 1;
+        if (pos[m].ib > j) b = m; else a = m + 1;
+    }
+    const int hi = a;
+    const double f = start + (double)j * dfreq;
+    const double x0 = dfreq * 0.5;
+    double acc = 0.0;
+    for (int i = lo; i < hi; ++i) {
+        const LineF64 r = lf[i];
+        double v = nu[i] == 0.0 ? 1e-6 : nu[i];
+        double d = fabs(v - f);
+        if (PROF == GX_PROF_VOIGT) {
+            acc += r.Kg * gx::w4_f64(d * r.inv_w, r.y);
+        } else if (PROF == GX_PROF_VOIGT_QUAD) {
+            double x = d * r.inv_w;
+            double k = (x * x + r.y * r.y < 100.0) ? gx::voigt916_f64(x, r.y) : gx::gh11_f64(x, r.y);
+            acc += r.Kg * k;
+        } else {
+            double xp = r.inv_w * (d + x0), xm = r.inv_w * (d - x0);
+            acc += r.Kg * (erf(xp) - erf(xm));
+        }
+    }
+    if (PROF == GX_PROF_VOIGT || PROF == GX_PROF_VOIGT_QUAD)
+        out[j] += acc;
+    else
+        out[j] += acc * (0.5 / dfreq);
+}
+
+// ------------------------------------------------------- partition function
+// Q(T_k) = sum_i g_i exp(-beta_k E_i).  Block b sums states [b*chunk, (b+1)*chunk)
+// with a fixed per-thread stride and a fixed LDS tree, then one block per T
+// adds the partials in block order: the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_partition(const double* __restrict__ E,
+                                                   const int* __restrict__ g, int64_t n,
+                                                   int64_t chunk, const double* __restrict__ T,
+                                                   double* __restrict__ partial, int nblk) {
+    __shared__ double red[256];
+    const int k = blockIdx.y;
+    const double beta = GX_PLANCK * GX_VELLGT / (GX_BOLTZ * T[k]);
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+    double s = 0.0;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) s += g[i] * exp(-beta * E[i]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)k * nblk + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_partition_final(const double* __restrict__ partial,
+                                                         int nblk, double* __restrict__ Q) {
+    __shared__ double red[256];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(int64_t)k * nblk + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Q[k] = red[0];
+}
+
+// ------------------------------------------------------------- small kernels
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_add_f64(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] += src[i];
+}
+
+// ---------------------------------------------------------- host launchers
+namespace gxk {
+
+static inline int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+hipError_t line_pos(hipStream_t s, const double* nu, int n, double cutoff, double start,
+                    double dfreq, int win_lo, int win_hi, LinePos* pos) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_line_pos, dim3(blocks_for(n, 256)), dim3(256), 0, s, nu, n, cutoff,
+                       start, dfreq, win_lo, win_hi, pos);
+    return hipGetLastError();
+}
+
+hipError_t line_max(hipStream_t s, const double* nu, const double* E, const double* A,
+                    const int* g, int n, const LinePos* pos, const TPParams* tp, int ntp,
+                    unsigned long long* scale_bits) {
+    if (n <= 0 || ntp <= 0) return hipSuccess;
+    int gx = std::min(blocks_for(n, 256), 1024);
+    hipLaunchKernelGGL(k_line_max, dim3(gx, ntp), dim3(256), 0, s, nu, E, A, g, n, pos, tp, ntp,
+                       scale_bits);
+    return hipGetLastError();
+}
+
+hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
+                    const int* g, const double* gamma, const double* nexp, int n,
+                    const LinePos* pos, const TPParams* tp, int ntp,
+                    const unsigned long long* scale_bits, VRec* rec, int64_t rec_stride,
+                    unsigned long long* neval) {
+    if (n <= 0 || ntp <= 0) return hipSuccess;
+    int gx = std::min(blocks_for(n, 256), 2048);
+    hipLaunchKernelGGL(k_line_rec, dim3(gx, ntp), dim3(256), 0, s, nu, E, A, g, gamma, nexp, n,
+                       pos, tp, ntp, scale_bits, rec, rec_stride, neval);
+    return hipGetLastError();
+}
+
+hipError_t tiles(hipStream_t s, const LinePos* pos, int n, int win_lo, int win_hi, int span,
+                 int job, WorkItem* items, uint32_t* cost_key, uint32_t* item_idx,
+                 int item_base, int* ntiles_out) {
+    int ntiles = win_hi > win_lo ? (win_hi - win_lo + span - 1) / span : 0;
+    *ntiles_out = ntiles;
+    if (ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, pos, n, win_lo,
+                       win_hi, span, ntiles, job, items, cost_key, item_idx, item_base);
+    return hipGetLastError();
+}
+
+hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx, int n,
+                         WorkItem* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_permute_items, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, idx, n,
+                       out);
+    return hipGetLastError();
+}
+
+template <int PROF, int PPL>
+static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitems,
+                                 const Job* jobs, const unsigned long long* scale_bits,
+                                 unsigned int* counter, int persistent_blocks) {
+    if (nitems <= 0) return hipSuccess;
+    const int wpb = GX_WG / GX_WAVE;
+    int nb = persistent_blocks > 0 ? std::min(persistent_blocks, blocks_for(nitems, wpb))
+                                   : blocks_for(nitems, wpb);
+    hipLaunchKernelGGL((k_profile_tiles<PROF, PPL>), dim3(nb), dim3(GX_WG), 0, s, items, nitems,
+                       jobs, scale_bits, persistent_blocks > 0 ? counter : nullptr);
+    return hipGetLastError();
+}
+
+template <int PROF>
+static hipError_t launch_prof(hipStream_t s, int ppl, const WorkItem* items, int nitems,
+                              const Job* jobs, const unsigned long long* scale_bits,
+                              unsigned int* counter, int persistent_blocks) {
+    switch (ppl) {
+        case 1: return launch_tiles_t<PROF, 1>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        case 2: return launch_tiles_t<PROF, 2>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        case 4: return launch_tiles_t<PROF, 4>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        case 8: return launch_tiles_t<PROF, 8>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t profile_tiles(hipStream_t s, int profile, int ppl, const WorkItem* items, int nitems,
+                         const Job* jobs, const unsigned long long* scale_bits,
+                         unsigned int* counter, int persistent_blocks) {
+    switch (profile) {
+        case GX_PROF_VOIGT:
+            return launch_prof<GX_PROF_VOIGT>(s, ppl, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        case GX_PROF_VOIGT_QUAD:
+            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        case GX_PROF_DOPPLER:
+        case GX_PROF_GAUSSIAN:
+            return launch_prof<GX_PROF_DOPPLER>(s, ppl, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+hipError_t line_f64(hipStream_t s, const double* nu, const double* E, const double* A,
+                    const int* g, const double* gamma, const double* nexp, int n,
+                    const LinePos* pos, const TPParams* tp, LineF64* out,
+                    unsigned long long* neval) {
+    if (n <= 0) return hipSuccess;
+    int gx = std::min(blocks_for(n, 256), 2048);
+    hipLaunchKernelGGL(k_line_f64, dim3(gx), dim3(256), 0, s, nu, E, A, g, gamma, nexp, n, pos,
+                       tp, out, neval);
+    return hipGetLastError();
+}
+
+hipError_t fp64_points(hipStream_t s, int profile, const double* nu, const LinePos* pos,
+                       const LineF64* lf, int n, double start, double dfreq, int win_lo,
+                       int win_hi, double* out) {
+    int np = win_hi - win_lo;
+    if (np <= 0) return hipSuccess;
+    dim3 grid(blocks_for(np, 256)), blk(256);
+    switch (profile) {
+        case GX_PROF_VOIGT:
+            hipLaunchKernelGGL(k_fp64_points<GX_PROF_VOIGT>, grid, blk, 0, s, nu, pos, lf, n, start, dfreq, win_lo, win_hi, out);
+            break;
+        case GX_PROF_VOIGT_QUAD:
+            hipLaunchKernelGGL(k_fp64_points<GX_PROF_VOIGT_QUAD>, grid, blk, 0, s, nu, pos, lf, n, start, dfreq, win_lo, win_hi, out);
+            break;
+        default:
+            hipLaunchKernelGGL(k_fp64_points<GX_PROF_DOPPLER>, grid, blk, 0, s, nu, pos, lf, n, start, dfreq, win_lo, win_hi, out);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t partition(hipStream_t s, const double* E, const int* g, int64_t n, const double* dT,
+                     int nT, double* partial, int nblk, double* dQ) {
+    if (nT <= 0) return hipSuccess;
+    int64_t chunk = (n + nblk - 1) / nblk;
+    if (chunk < 1) chunk = 1;
+    hipLaunchKernelGGL(k_partition, dim3(nblk, nT), dim3(256), 0, s, E, g, n, chunk, dT, partial,
+                       nblk);
+    hipLaunchKernelGGL(k_partition_final, dim3(nT), dim3(256), 0, s, partial, nblk, dQ);
+    return hipGetLastError();
+}
+
+hipError_t fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t fill_f64(hipStream_t s, double* p, int64_t n, double v) {
+    if (n <= 0) return hipSuccess;
+    int nb = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_f64, dim3(nb), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t add_f64(hipStream_t s, double* dst, const double* src, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    int nb = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_add_f64, dim3(nb), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+}  // namespace gxk

@@ -1,0 +1,214 @@
+// profiles_device.h -- profile functions evaluated on gfx950.
+//
+// Humlicek W4 (JQSRT 27:437, 1982): the four-region rational/exponential
+// approximation of K(x,y) = Re w(x+iy) used by the reference in
+// compute/CPU/profiles.cpp:11-44 (FP64) and compute/CUDA/cross_kernal_v2.cu:52-84
+// (FP32).  Here complex arithmetic is spelled out on (re, im) pairs and only
+// the real part of the last quotient is formed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gx {
+
+template <typename R>
+struct cplx {
+    R r, i;
+};
+
+template <typename R>
+__device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> cadd(R a, cplx<R> b) {
+    return {a + b.r, b.i};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> csubr(R a, cplx<R> b) {  // a - b
+    return {a - b.r, -b.i};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> cscale(cplx<R> a, R s) {
+    return {a.r * s, a.i * s};
+}
+// a + t*p and a - u*p as Horner steps
+template <typename R>
+__device__ __forceinline__ cplx<R> hadd(R a, cplx<R> t, cplx<R> p) {
+    return cadd(a, cmul(t, p));
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> hsub(R a, cplx<R> u, cplx<R> p) {
+    return csubr(a, cmul(u, p));
+}
+
+// Region III:  (16.4955 + t(20.20933 + t(11.96482 + t(3.778987 + 0.5642236 t))))
+//            / (16.4955 + t(38.82363 + t(39.27121 + t(21.69274 + t(6.699398 + t)))))
+template <typename R>
+__device__ __forceinline__ void w4_region3(cplx<R> t, cplx<R>& num, cplx<R>& den) {
+    cplx<R> p = cadd(R(3.778987), cscale(t, R(.5642236)));
+    p = hadd(R(11.96482), t, p);
+    p = hadd(R(20.20933), t, p);
+    num = hadd(R(16.4955), t, p);
+    cplx<R> q = cadd(R(6.699398), t);
+    q = hadd(R(21.69274), t, q);
+    q = hadd(R(39.27121), t, q);
+    q = hadd(R(38.82363), t, q);
+    den = hadd(R(16.4955), t, q);
+}
+
+// Region IV rational part: t (36183.31 - u(3321.9905 - u(1540.787 - u(219.0313
+//   - u(35.76683 - u(1.320522 - 0.56419 u)))))) / (32066.6 - u(24322.84 - u(9022.228
+//   - u(2186.181 - u(364.2191 - u(61.57037 - u(1.841439 - u)))))))
+template <typename R>
+__device__ __forceinline__ void w4_region4(cplx<R> t, cplx<R> u, cplx<R>& num, cplx<R>& den) {
+    cplx<R> p = csubr(R(1.320522), cscale(u, R(.56419)));
+    p = hsub(R(35.76683), u, p);
+    p = hsub(R(219.0313), u, p);
+    p = hsub(R(1540.787), u, p);
+    p = hsub(R(3321.9905), u, p);
+    p = hsub(R(36183.31), u, p);
+    num = cmul(t, p);
+    cplx<R> q = csubr(R(1.841439), u);
+    q = hsub(R(61.57037), u, q);
+    q = hsub(R(364.2191), u, q);
+    q = hsub(R(2186.181), u, q);
+    q = hsub(R(9022.228), u, q);
+    q = hsub(R(24322.84), u, q);
+    den = hsub(R(32066.6), u, q);
+}
+
+// Re(a / b)
+template <typename R>
+__device__ __forceinline__ R re_div(cplx<R> a, cplx<R> b) {
+    return (a.r * b.r + a.i * b.i) / (b.r * b.r + b.i * b.i);
+}
+__device__ __forceinline__ float re_div_f(cplx<float> a, cplx<float> b) {
+    return (a.r * b.r + a.i * b.i) * __builtin_amdgcn_rcpf(b.r * b.r + b.i * b.i);
+}
+
+// Region I only: c*y*(0.5+y^2+x^2) / ((0.5+y^2-x^2)^2 + 4x^2y^2)
+template <typename R>
+__device__ __forceinline__ R w4_region1(R x, R y) {
+    R x2 = x * x, b = R(0.5) + y * y;
+    R d = b - x2;
+    return R(0.5641896) * y * (b + x2) / (d * d + R(4) * x2 * y * y);
+}
+
+// full W4 in FP64 (regions chosen on S = |x| + y exactly as the reference)
+__device__ inline double w4_f64(double x, double y) {
+    typedef cplx<double> C;
+    C t = {y, -x};
+    double s = fabs(x) + y;
+    if (s >= 15.0) return w4_region1(fabs(x), y);
+    C u = cmul(t, t);
+    if (s >= 5.5) {
+        C num = cmul(t, cadd(1.410474, cscale(u, .5641896)));
+        C den = cadd(.75, cmul(u, cadd(3.0, u)));
+        return re_div(num, den);
+    }
+    C num, den;
+    if (y >= 0.195 * fabs(x) - 0.176) {
+        w4_region3(t, num, den);
+        return re_div(num, den);
+    }
+    w4_region4(t, u, num, den);
+    // Re exp(u) = e^{u.r} cos(u.i)
+    return exp(u.r) * cos(u.i) - re_div(num, den);
+}
+
+// W4 regions II-IV in FP32 (region I has its own fast path in the kernel)
+__device__ inline float w4_f32(float x, float y) {
+    typedef cplx<float> C;
+    C t = {y, -x};
+    float s = x + y;  // x >= 0 here
+    if (s >= 15.0f) return w4_region1(x, y);
+    C u = cmul(t, t);
+    if (s >= 5.5f) {
+        C num = cmul(t, cadd(1.410474f, cscale(u, .5641896f)));
+        C den = cadd(.75f, cmul(u, cadd(3.0f, u)));
+        return re_div_f(num, den);
+    }
+    C num, den;
+    if (y >= 0.195f * x - 0.176f) {
+        w4_region3(t, num, den);
+        return re_div_f(num, den);
+    }
+    w4_region4(t, u, num, den);
+    return expf(u.r) * cosf(u.i) - re_div_f(num, den);
+}
+
+// Zaghloul & Ali (2011) ACM TOMS 38(2) Algorithm 916, series for Re w, with
+// a = 1/2: the n-sums use e^{-(an)^2} = e^{-n^2/4} and e^{-(an +- x)^2} =
+// e^{-x^2} e^{-n^2/4} e^{-+ n x}, so only three exponentials per point.
+__device__ inline double voigt916_f64(double x, double y) {
+    x = fabs(x);
+    double ex2 = exp(-x * x);
+    if (x == 0.0) return erfcx(y);
+    if (y == 0.0) return ex2;
+    double yy = y * y;
+    double q = exp(-x), qi = exp(x);  // e^{-x}, e^{x}
+    double qn = 1.0, qin = 1.0;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int nmax = (int)ceil(2.0 * x) + 16;
+    for (int n = 1; n <= nmax; ++n) {
+        double an = 0.5 * n;
+        double cn = exp(-an * an);
+        double f = cn / (an * an + yy);
+        qn *= q;
+        qin *= qi;
+        s1 += f;
+        s2 += f * qn;
+        s3 += f * qin;
+    }
+    s1 *= ex2;
+    s2 *= ex2;
+    s3 *= ex2;
+    double xy = x * y;
+    double c2 = cos(2.0 * xy), sn = sin(xy);
+    const double a2pi = 1.0 / 3.14159265358979323846;  // 2a/pi at a = 1/2
+    double k = ex2 * erfcx(y) * c2;
+    k += a2pi * sn * sn * ex2 / y;
+    k += a2pi * y * (-c2 * s1 + 0.5 * (s2 + s3));
+    return k;
+}
+
+// 11-point Gauss-Hermite quadrature of K (A&S 25.4.46, table 25.10)
+__device__ __forceinline__ float gh11_f32(float x, float y) {
+    const float t2[5] = {0.4313988071478515f, 1.7597536984236963f, 4.104465362828315f,
+                         7.746703779542555f, 13.457678352057581f};
+    const float w[5] = {0.42935975235612495f, 0.11722787516770851f, 0.011911395444911507f,
+                        0.00034681946632334544f, 1.4395603937142596e-06f};
+    float r = x * x + y * y;
+    float acc = 0.6547592869145916f / r;
+    float fx2 = 4.0f * x * x;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        float qq = r + t2[k];
+        acc += w[k] * 2.0f * qq / (qq * qq - fx2 * t2[k]);
+    }
+    return acc * y * 0.3183098861837907f;
+}
+
+__device__ __forceinline__ double gh11_f64(double x, double y) {
+    const double t2[5] = {0.4313988071478515, 1.7597536984236963, 4.104465362828315,
+                          7.746703779542555, 13.457678352057581};
+    const double w[5] = {0.42935975235612495, 0.11722787516770851, 0.011911395444911507,
+                         0.00034681946632334544, 1.4395603937142596e-06};
+    double r = x * x + y * y;
+    double acc = 0.6547592869145916 / r;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        double qq = r + t2[k];
+        acc += w[k] * 2.0 * qq / (qq * qq - 4.0 * x * x * t2[k]);
+    }
+    return acc * y / 3.14159265358979323846;
+}
+
+// bin average of a unit Gaussian over [u - h, u + h] (times 2):
+// erf(u + h) - erf(u - h) for u >= 0, written without cancellation.
+__device__ __forceinline__ float bin_erf_f32(float u, float h) {
+    float a = u + h, b = u - h;
+    return (b >= 0.0f) ? (erfcf(b) - erfcf(a)) : (erff(a) + erff(-b));
+}
+
+}  // namespace gx

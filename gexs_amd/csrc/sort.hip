@@ -1,0 +1,278 @@
+// sort.hip -- stable LSD radix sort on gfx950, plus gather/assembly helpers.
+//
+// One block = 256 threads x 16 rounds = 4096 keys.  Per 8-bit pass:
+//   k_rs_hist     per-block digit histogram (LDS atomics on counts only)
+//   k_rs_scan     per digit, exclusive scan of the block counts (fixed order)
+//   k_rs_scatter  stable scatter: within a round, a lane's rank among the
+//                 lanes of its wavefront holding the same digit comes from
+//                 eight 64-bit ballots; waves and rounds are ordered through
+//                 LDS counters, so equal keys keep their input order.
+// The output order is a pure function of the input (no timing dependence).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+
+#define RS_THREADS 256
+#define RS_ROUNDS 16
+#define RS_TILE (RS_THREADS * RS_ROUNDS)
+
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K* __restrict__ keys, int64_t n,
+                                                        int shift, uint32_t* __restrict__ hist,
+                                                        int nblk) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        int64_t i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t* __restrict__ hist, int nblk,
+                                                 uint32_t* __restrict__ totals) {
+    __shared__ uint32_t part[256];
+    const int d = blockIdx.x;
+    uint32_t* h = hist + (int64_t)d * nblk;
+    const int seg = (nblk + 255) / 256;
+    const int b0 = threadIdx.x * seg;
+    const int b1 = min(b0 + seg, nblk);
+    uint32_t s = 0;
+    for (int b = b0; b < b1; ++b) s += h[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 256 partial sums
+    for (int o = 1; o < 256; o <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0u;
+    for (int b = b0; b < b1; ++b) {
+        uint32_t c = h[b];
+        h[b] = run;
+        run += c;
+    }
+    if (threadIdx.x == 255) totals[d] = part[255];
+}
+
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           K* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout, int64_t n,
+                                                           int shift,
+                                                           const uint32_t* __restrict__ hist,
+                                                           int nblk,
+                                                           const uint32_t* __restrict__ totals) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    // digit base = exclusive scan of totals + this block's prefix within the digit
+    base[tid] = totals[tid];
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        uint32_t v = tid >= o ? base[tid - o] : 0u;
+        __syncthreads();
+        base[tid] += v;
+        __syncthreads();
+    }
+    uint32_t excl = (tid > 0 ? base[tid - 1] : 0u) + hist[(int64_t)tid * nblk + blockIdx.x];
+    __syncthreads();
+    base[tid] = excl;
+    run[tid] = 0;
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const int64_t blk0 = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = blk0 + r * RS_THREADS + tid;
+        const bool valid = i < n;
+        K key = valid ? kin[i] : (K)0;
+        uint32_t val = valid ? vin[i] : 0u;
+        const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const int rank = __popcll(m & lt);
+        const int cnt = __popcll(m);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        if (valid && rank == cnt - 1) wcnt[wave][d] = (uint32_t)cnt;
+        __syncthreads();
+        if (valid) {
+            uint32_t p = base[d] + run[d] + (uint32_t)rank;
+            for (int w = 0; w < wave; ++w) p += wcnt[w][d];
+            kout[p] = key;
+            vout[p] = val;
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        __syncthreads();
+    }
+}
+
+__global__ void k_nu_keys(const double* __restrict__ nu, int64_t n, uint64_t* __restrict__ keys,
+                          uint32_t* __restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t u = (uint64_t)__double_as_longlong(nu[i]);
+    u = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+    keys[i] = u;
+    vals[i] = (uint32_t)i;
+}
+
+template <typename T>
+__global__ void k_gather(const T* __restrict__ in, const uint32_t* __restrict__ perm, int64_t n,
+                         T* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[perm[i]];
+}
+
+__global__ void k_assemble(const double* __restrict__ sE, const int32_t* __restrict__ sg,
+                           const uint32_t* __restrict__ up, const uint32_t* __restrict__ lo,
+                           int64_t n, double* __restrict__ nu, double* __restrict__ E,
+                           int32_t* __restrict__ g) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = up[i], l = lo[i];
+        const double eu = sE[u], el = sE[l];
+        nu[i] = eu - el;  // ExomolStateReader.cpp:160
+        E[i] = el;        // :161
+        g[i] = sg[u];     // :162
+    }
+}
+
+__global__ void k_widen(const float* __restrict__ in, int64_t n, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+namespace gxk {
+
+static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
+
+size_t sort_scratch_bytes(int64_t n, int key_bytes) {
+    int64_t nblk = (n + RS_TILE - 1) / RS_TILE;
+    return (size_t)n * key_bytes + (size_t)n * 4 + (size_t)nblk * 256 * 4 + 1024;
+}
+
+hipError_t sort_buffers_alloc(SortBuffers& buf, int64_t n, int key_bytes) {
+    if (buf.cap >= n && buf.key_bytes >= key_bytes && buf.keys_alt) return hipSuccess;
+    sort_buffers_free(buf);
+    int64_t nblk = (n + RS_TILE - 1) / RS_TILE;
+    hipError_t e;
+    if ((e = hipMalloc(&buf.keys_alt, (size_t)std::max<int64_t>(n, 1) * key_bytes)) != hipSuccess) return e;
+    if ((e = hipMalloc(&buf.vals_alt, (size_t)std::max<int64_t>(n, 1) * 4)) != hipSuccess) return e;
+    if ((e = hipMalloc(&buf.hist, (size_t)std::max<int64_t>(nblk, 1) * 256 * 4)) != hipSuccess) return e;
+    if ((e = hipMalloc(&buf.totals, 256 * 4)) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&buf.totals_host, 256 * 4, hipHostMallocDefault)) != hipSuccess) return e;
+    buf.cap = n;
+    buf.key_bytes = key_bytes;
+    return hipSuccess;
+}
+
+void sort_buffers_free(SortBuffers& buf) {
+    if (buf.keys_alt) (void)hipFree(buf.keys_alt);
+    if (buf.vals_alt) (void)hipFree(buf.vals_alt);
+    if (buf.hist) (void)hipFree(buf.hist);
+    if (buf.totals) (void)hipFree(buf.totals);
+    if (buf.totals_host) (void)hipHostFree(buf.totals_host);
+    buf = SortBuffers();
+}
+
+template <typename K>
+static hipError_t radix_sort(hipStream_t s, K* keys, uint32_t* vals, int64_t n, SortBuffers& buf,
+                             int begin_bit, int end_bit) {
+    if (n <= 1) return hipSuccess;
+    hipError_t e = sort_buffers_alloc(buf, n, (int)sizeof(K));
+    if (e != hipSuccess) return e;
+    const int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
+    K* kin = keys;
+    uint32_t* vin = vals;
+    K* kout = (K*)buf.keys_alt;
+    uint32_t* vout = buf.vals_alt;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL((k_rs_hist<K>), dim3(nblk), dim3(RS_THREADS), 0, s, kin, n, shift,
+                           buf.hist, nblk);
+        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, s, buf.hist, nblk, buf.totals);
+        if ((e = hipMemcpyAsync(buf.totals_host, buf.totals, 256 * 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        bool constant = false;
+        for (int d = 0; d < 256; ++d)
+            if ((int64_t)buf.totals_host[d] == n) constant = true;
+        if (constant) continue;  // every key has the same digit: order unchanged
+        hipLaunchKernelGGL((k_rs_scatter<K>), dim3(nblk), dim3(RS_THREADS), 0, s, kin, vin, kout,
+                           vout, n, shift, buf.hist, nblk, buf.totals);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (kin != keys) {
+        if ((e = hipMemcpyAsync(keys, kin, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(vals, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t radix_sort_u64(hipStream_t s, uint64_t* keys, uint32_t* vals, int64_t n,
+                          SortBuffers& buf, int begin_bit, int end_bit) {
+    return radix_sort<uint64_t>(s, keys, vals, n, buf, begin_bit, end_bit);
+}
+
+hipError_t radix_sort_u32(hipStream_t s, uint32_t* keys, uint32_t* vals, int64_t n,
+                          SortBuffers& buf, int begin_bit, int end_bit) {
+    return radix_sort<uint32_t>(s, keys, vals, n, buf, begin_bit, end_bit);
+}
+
+hipError_t make_nu_keys(hipStream_t s, const double* nu, int64_t n, uint64_t* keys,
+                        uint32_t* vals) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nu_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nu, n, keys,
+                       vals);
+    return hipGetLastError();
+}
+
+hipError_t gather_f64(hipStream_t s, const double* in, const uint32_t* perm, int64_t n,
+                      double* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, s, in, perm, n, out);
+    return hipGetLastError();
+}
+
+hipError_t gather_i32(hipStream_t s, const int32_t* in, const uint32_t* perm, int64_t n,
+                      int32_t* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(n)), dim3(256), 0, s, in, perm, n, out);
+    return hipGetLastError();
+}
+
+hipError_t assemble_lines(hipStream_t s, const double* sE, const int32_t* sg,
+                          const uint32_t* up, const uint32_t* lo, int64_t n, double* nu,
+                          double* E, int32_t* g) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_assemble, dim3(grid_for(n)), dim3(256), 0, s, sE, sg, up, lo, n, nu, E, g);
+    return hipGetLastError();
+}
+
+hipError_t widen_f32(hipStream_t s, const float* in, int64_t n, double* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_widen, dim3(grid_for(n)), dim3(256), 0, s, in, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace gxk

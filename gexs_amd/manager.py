@@ -1,0 +1,178 @@
+"""HipManager: the BaseManager plugin API over the gexs HIP C ABI.
+
+Method names, arguments and call order follow Managers/BaseManager.h:22-35 of
+the reference (InitializeVectors, InitializeConstants, TransferFreq,
+TransferVectors, ExecuteCrossSection, TransferResults, ReadyForWork,
+GetNtrans, Cleanup), so a driver written against BaseManager (as
+Profiles/BaseProfile.cpp:60-112,281 is) runs unchanged.  Arrays are numpy;
+every call goes through libgexs_hip.so.  Errors raise GexsError (the reference
+calls exit(-1) instead, compute/CUDA/cuda_utils.cu:8-17).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Constants, GexsError, Stats
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class HipManager:
+    def __init__(self, profile="voigt", device=0, precision="fp32c"):
+        L = _lib.load()
+        self._L = L
+        self.profile = _lib.PROFILE_IDS[profile] if isinstance(profile, str) else int(profile)
+        h = C.c_void_p()
+        rc = L.gexs_create(int(device), self.profile, C.byref(h))
+        if rc != 0:
+            raise GexsError(rc, "gexs_create failed (is a GPU visible?)")
+        self._h = h
+        self._precision = _lib.FP64 if precision == "fp64" else _lib.FP32C
+        self.N_trans = 0
+        self.npoints = 0
+        self._keep = []
+
+    # ------------------------------------------------------------ helpers
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self._L.gexs_last_error(self._h).decode()
+            raise GexsError(rc, msg)
+        return rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.gexs_destroy(self._h)
+            self._h = None
+
+    Cleanup = close
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----------------------------------------------------- BaseManager API
+    def InitializeVectors(self, Npoints, num_cs=1, mem_budget=0):
+        cap = C.c_int64()
+        self._chk(self._L.gexs_init_vectors(self._h, int(Npoints), int(num_cs), int(mem_budget),
+                                            C.byref(cap)))
+        self.N_trans = cap.value
+        self.npoints = int(Npoints)
+
+    def GetNtrans(self):
+        return self.N_trans
+
+    def InitializeConstants(self, half_width, temperature, partition, dfreq, meanmass,
+                            pressure=10.0, ref_temp=296.0, ref_press=1.0, cutoff=0.0,
+                            width_conv=0, pressure_mode=1, gauss_hwhm=None):
+        k = Constants(float(half_width), float(temperature), float(partition), float(dfreq),
+                      float(meanmass), float(pressure), float(ref_temp), float(ref_press),
+                      float(cutoff),
+                      float(gauss_hwhm if gauss_hwhm is not None else half_width),
+                      int(width_conv), int(pressure_mode), int(self._precision), 0)
+        self._chk(self._L.gexs_init_constants(self._h, C.byref(k)))
+
+    def TransferFreq(self, freq, intens=None, N=None):
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        n = int(N if N is not None else freq.shape[0])
+        self._chk(self._L.gexs_transfer_freq(self._h, _ptr(freq), n))
+
+    def TransferVectors(self, Nener, energies, nu, aif, gns, gamma=None, nexp=None):
+        E = np.ascontiguousarray(energies, dtype=np.float64)
+        v = np.ascontiguousarray(nu, dtype=np.float64)
+        A = np.ascontiguousarray(aif, dtype=np.float64)
+        g = np.ascontiguousarray(gns, dtype=np.int32)
+        ga = np.ascontiguousarray(gamma, dtype=np.float64) if gamma is not None else None
+        nx = np.ascontiguousarray(nexp, dtype=np.float64) if nexp is not None else None
+        self._chk(self._L.gexs_transfer_lines(self._h, int(Nener), _ptr(E), _ptr(v), _ptr(A),
+                                              _ptr(g), _ptr(ga), _ptr(nx)))
+        self._n_lines = int(Nener)
+
+    def TransferVectorsDevice(self, Nener, E_ptr, nu_ptr, A_ptr, g_ptr, gamma_ptr=None,
+                              nexp_ptr=None):
+        """TransferVectors from device pointers (ints) already in HBM."""
+        vp = lambda p: C.c_void_p(p) if p else None  # noqa: E731
+        self._chk(self._L.gexs_transfer_lines_device(self._h, int(Nener), vp(E_ptr), vp(nu_ptr),
+                                                     vp(A_ptr), vp(g_ptr), vp(gamma_ptr),
+                                                     vp(nexp_ptr)))
+        self._n_lines = int(Nener)
+
+    def ExecuteCrossSection(self, N, N_ener, start_idx, temperature, pressure, partition):
+        self._chk(self._L.gexs_execute(self._h, int(N), int(N_ener), int(start_idx),
+                                       float(temperature), float(pressure), float(partition)))
+
+    def TransferResults(self, freq, intens, N, I=0, accumulate=True):
+        out = intens
+        assert out.dtype == np.float64 and out.flags["C_CONTIGUOUS"]
+        self._chk(self._L.gexs_transfer_results(self._h, _ptr(out), int(N), int(I),
+                                                1 if accumulate else 0))
+        return out
+
+    def ReadyForWork(self):
+        return self._L.gexs_ready(self._h) == 1
+
+    # ------------------------------------------------------------ extras
+    def sync(self):
+        self._chk(self._L.gexs_sync(self._h))
+
+    def stats(self):
+        s = Stats()
+        self._chk(self._L.gexs_last_stats(self._h, C.byref(s)))
+        return dict(n_eval=s.n_eval, n_lines=s.n_lines, n_tiles=s.n_tiles,
+                    kernel_ms=s.kernel_ms, total_ms=s.total_ms)
+
+    def timing(self, reset=False):
+        """(sum of profile-kernel ms, sum of whole-pass ms, count) since the last reset."""
+        k, t, n = C.c_double(), C.c_double(), C.c_int()
+        self._chk(self._L.gexs_timing(self._h, 1 if reset else 0, C.byref(k), C.byref(t),
+                                      C.byref(n)))
+        return k.value, t.value, n.value
+
+    def line_windows(self, n):
+        """(sorted nu, ib, ie) of the current lines as the device holds them."""
+        nu = np.empty(n, np.float64)
+        ib = np.empty(n, np.int32)
+        ie = np.empty(n, np.int32)
+        self._chk(self._L.gexs_line_windows(self._h, int(n), _ptr(nu), _ptr(ib), _ptr(ie)))
+        return nu, ib, ie
+
+    def set_tuning(self, points_per_lane=4, order=1):
+        self._chk(self._L.gexs_set_tuning(self._h, int(points_per_lane), int(order)))
+
+    # ------------------------------------------------- resident list / batch
+    def load_list(self, lst):
+        """Upload a synth.LineList (states + transitions) and sort it on the device."""
+        E = np.ascontiguousarray(lst.E, dtype=np.float64)
+        g = np.ascontiguousarray(lst.g, dtype=np.int32)
+        up = np.ascontiguousarray(lst.upper, dtype=np.uint32)
+        lo = np.ascontiguousarray(lst.lower, dtype=np.uint32)
+        A = np.ascontiguousarray(lst.A, dtype=np.float32)
+        ga = np.ascontiguousarray(lst.gamma, dtype=np.float32)
+        nx = np.ascontiguousarray(lst.nexp, dtype=np.float32)
+        st = _lib.States(E.shape[0], E.ctypes.data, g.ctypes.data)
+        tr = _lib.Transitions(up.shape[0], up.ctypes.data, lo.ctypes.data, A.ctypes.data,
+                              ga.ctypes.data, nx.ctypes.data)
+        self._chk(self._L.gexs_load_list(self._h, C.byref(st), C.byref(tr)))
+
+    def partition(self, temps):
+        T = np.ascontiguousarray(temps, dtype=np.float64)
+        Q = np.empty_like(T)
+        self._chk(self._L.gexs_partition(self._h, _ptr(T), T.shape[0], _ptr(Q)))
+        return Q
+
+    def run_batch(self, tps, start, end, npoints, out=None):
+        """Cross-sections at [(T, P[, Q]), ...] over grid [start, end) with npoints."""
+        arr = (_lib.TP * len(tps))()
+        for i, t in enumerate(tps):
+            arr[i] = _lib.TP(float(t[0]), float(t[1]), float(t[2]) if len(t) > 2 else 0.0)
+        grid = _lib.Grid(float(start), float(end), int(npoints), 0)
+        if out is None:
+            out = np.empty((len(tps), int(npoints)), dtype=np.float64)
+        self._chk(self._L.gexs_run_batch(self._h, arr, len(tps), C.byref(grid), _ptr(out)))
+        return out

@@ -1,0 +1,216 @@
+/*
+ * gexs_hip.h -- C ABI of the MI355X (gfx950) line-by-line cross-section library.
+ *
+ * This is the boundary a HipManager (the BaseManager drop-in beside
+ * Managers/Accel/GPU/GpuManager) calls.  Plain pointers and sizes only.
+ * Citations are to files of the GEXS reference tree.
+ *
+ * Contract
+ *   - Every function returns 0 on success and a negative GEXS_E* code on
+ *     failure; gexs_last_error() gives the message.  Nothing calls exit().
+ *   - One gexs_ctx per device and per caller thread (externally
+ *     synchronised).  A ctx owns its HIP stream, device memory and graphs.
+ *   - Host buffers passed in are copied before the call returns (or, for the
+ *     asynchronous gexs_execute, are not touched at all); none is retained.
+ *   - Grid: f[j] = start + j*dfreq, j in [0, npoints)  (Profiles/BaseProfile.cpp:10,19).
+ *   - Result grids are FP64, like BaseManager::TransferResults (BaseManager.h:31-32).
+ */
+#ifndef GEXS_HIP_H
+#define GEXS_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* profile ids: the values of ProfileType, common/Input.h:10-17 */
+enum {
+    GEXS_GAUSSIAN = 0,
+    GEXS_DOPPLER = 1,
+    GEXS_VOIGT = 2,
+    GEXS_VOIGT_QUAD = 4
+};
+
+/* arithmetic of the profile sum */
+enum {
+    GEXS_FP32C = 0, /* FP32 evaluation, per-point compensated (Kahan) sums, FP64 fold */
+    GEXS_FP64 = 1   /* plain FP64: one work-item per grid point, FP64 W4 and sums */
+};
+
+enum {
+    GEXS_OK = 0,
+    GEXS_EINVAL = -1,   /* bad argument */
+    GEXS_ENOMEM = -2,   /* device allocation failed or over the ctx budget */
+    GEXS_EHIP = -3,     /* HIP runtime error */
+    GEXS_ESTATE = -4,   /* call out of order (e.g. execute before transfer) */
+    GEXS_EBUSY = -5,    /* serving queue full */
+    GEXS_ENOTFOUND = -6 /* unknown request id */
+};
+
+typedef struct gexs_ctx gexs_ctx;
+
+/* Arguments of BaseManager::InitializeConstants (BaseManager.h:25) plus the
+ * conventions the reference leaves implicit.  T, P and Q here are defaults;
+ * gexs_execute passes its own. */
+typedef struct gexs_constants {
+    double half_width;    /* line-selection half width (VoigtProfile.h:15 = 100) */
+    double temperature;   /* K */
+    double partition;     /* Q(T) */
+    double dfreq;         /* grid step, cm^-1 */
+    double mean_mass;     /* amu */
+    double pressure;      /* in the unit p_scale converts to gamma's unit */
+    double ref_temp;      /* T_ref of gamma_ref (BaseProfile.cpp:49,53) */
+    double ref_press;     /* pressure unit scale (BaseProfile.cpp:50,54) */
+    double cutoff;        /* window half-width cm^-1; <= 0 selects min(25 P, 100)
+                             (OpenMPManager.cpp:386) */
+    double gauss_hwhm;    /* GAUSSIAN fixed HWHM (HALF-WIDTH keyword, Input.cpp:168-169) */
+    int32_t width_conv;   /* 0: 1/w = 1/(nu*dpwcoef) as OpenMPManager.cpp:404;
+                             1: sqrt(ln2)/(nu*dpwcoef) as cross_kernal_v2.cu:441 */
+    int32_t pressure_mode;/* 0: gammaL = gamma*(Tref/T)^n (OpenMPManager.cpp:398);
+                             1: times P*ref_press (cross_kernal_v2.cu:142) */
+    int32_t precision;    /* GEXS_FP32C or GEXS_FP64 */
+    int32_t reserved;
+} gexs_constants;
+
+/* ---------------------------------------------------------------- manager
+ * One-to-one with the BaseManager virtuals (Managers/BaseManager.h:22-35). */
+
+/* GpuManager::GpuManager (GPUManager.cpp:5-18); profile = GEXS_* profile id */
+int gexs_create(int device, int profile, gexs_ctx** out);
+/* GpuManager::Cleanup + delete */
+int gexs_destroy(gexs_ctx* ctx);
+/* InitializeVectors(int Npoints, int num_cs) (BaseManager.h:23): grid plus
+ * num_cs result grids; *line_capacity receives GetNtrans() (BaseManager.h:35),
+ * capped at 2^31-1 and at mem_budget (0 = 90% of free device memory). */
+int gexs_init_vectors(gexs_ctx* ctx, int npoints, int num_cs, int64_t mem_budget,
+                      int64_t* line_capacity);
+/* InitializeConstants (BaseManager.h:25) */
+int gexs_init_constants(gexs_ctx* ctx, const gexs_constants* k);
+/* TransferFreq(double* freq, double* intens, int N) (BaseManager.h:27):
+ * takes the grid (start = freq[0]) and zeroes every result grid. */
+int gexs_transfer_freq(gexs_ctx* ctx, const double* freq, int n);
+/* TransferVectors (BaseManager.h:26): one chunk of lines, host FP64 SoA.
+ * gamma / nexp may be NULL for profiles without pressure broadening.
+ * Waits for earlier work, uploads, then sorts the chunk by nu on the device.
+ * Resets the execute counter (GPUManager.cpp:149). */
+int gexs_transfer_lines(gexs_ctx* ctx, int64_t n, const double* E, const double* nu,
+                        const double* A, const int32_t* g, const double* gamma,
+                        const double* nexp);
+/* Same, from device pointers already resident in HBM (no host copy). */
+int gexs_transfer_lines_device(gexs_ctx* ctx, int64_t n, const double* d_E, const double* d_nu,
+                               const double* d_A, const int32_t* d_g, const double* d_gamma,
+                               const double* d_nexp);
+/* ExecuteCrossSection(N, N_ener, start_idx, T, P, Q) (BaseManager.h:30):
+ * asynchronous; accumulates the chunk's lines over grid points
+ * [start_idx, start_idx+npoints) into result grid number k, where k counts
+ * the executes since the last transfer_lines. */
+int gexs_execute(gexs_ctx* ctx, int npoints, int64_t nlines, int start_idx, double T,
+                 double P, double Q);
+/* TransferResults(freq, out, N, I) (BaseManager.h:32): waits, then copies
+ * result grid I to out (accumulate != 0 adds into out). */
+int gexs_transfer_results(gexs_ctx* ctx, double* out, int n, int cs_index, int accumulate);
+/* ReadyForWork (BaseManager.h:34): 1 when no work is pending, else 0 */
+int gexs_ready(gexs_ctx* ctx);
+/* wait for all work of the ctx */
+int gexs_sync(gexs_ctx* ctx);
+const char* gexs_last_error(const gexs_ctx* ctx);
+
+/* Counters of the last gexs_execute: N_eval (sum over lines with S != 0 of
+ * the window length), lines, and the device time of its profile kernel and
+ * of the whole execute in ms (HIP events on the ctx stream). */
+typedef struct gexs_stats {
+    int64_t n_eval;
+    int64_t n_lines;
+    int64_t n_tiles;
+    double kernel_ms;
+    double total_ms;
+} gexs_stats;
+int gexs_last_stats(gexs_ctx* ctx, gexs_stats* out);
+
+/* Sums of the HIP-event durations (ms) of the profile kernel and of the whole
+ * pass over every execute / run_batch since the last reset, and their count
+ * (at most 2048 are kept).  Waits for the ctx stream.  reset != 0 clears. */
+int gexs_timing(gexs_ctx* ctx, int reset, double* kernel_ms, double* total_ms, int* count);
+
+/* The current lines in device order (ascending nu) with the window [ib, ie)
+ * each received in the last execute / run_batch (OpenMPManager.cpp:399-400).
+ * Any pointer may be NULL; n must equal the number of current lines. */
+int gexs_line_windows(gexs_ctx* ctx, int64_t n, double* nu_sorted, int32_t* ib, int32_t* ie);
+
+/* Tuning of the FP32C profile kernel: points per lane (1, 2, 4 or 8; a
+ * wave covers 64*ppl grid points) and the dispatch order of the tiles
+ * (0 = grid order, 1 = heaviest first). */
+int gexs_set_tuning(gexs_ctx* ctx, int points_per_lane, int order);
+
+/* ------------------------------------------------------ resident line list
+ * ExoMol-style list held in HBM: a state table and transitions indexing it
+ * (Readers/Exomol/exomol_objects.h:3-14, ExomolStateReader.cpp:158-162). */
+typedef struct gexs_states {
+    int64_t n;
+    const double* E;     /* cm^-1 */
+    const int32_t* g;    /* total degeneracy */
+} gexs_states;
+
+typedef struct gexs_transitions {
+    int64_t n;
+    const uint32_t* upper;  /* 0-based state index */
+    const uint32_t* lower;
+    const float* A;         /* Einstein A, s^-1 */
+    const float* gamma;     /* Lorentz gamma_ref (NULL: Doppler-only list) */
+    const float* nexp;      /* temperature exponent */
+} gexs_transitions;
+
+/* Upload, assemble lines (nu = E_u - E_l, E'' = E_l, g' = g_u), keep lines
+ * with nu > 0 and sort them by nu.  Replaces any previous list. */
+int gexs_load_list(gexs_ctx* ctx, const gexs_states* st, const gexs_transitions* tr);
+/* Q(T) on the device from the resident states, FP64, fixed reduction order
+ * (exomol_functions.cpp:50-73). */
+int gexs_partition(gexs_ctx* ctx, const double* T, int nT, double* Q_out);
+
+typedef struct gexs_grid {
+    double start;
+    double end;        /* dfreq = (end - start) / npoints */
+    int32_t npoints;
+    int32_t reserved;
+} gexs_grid;
+
+typedef struct gexs_tp {
+    double T;
+    double P;
+    double Q;          /* <= 0: computed on the device from the resident states */
+} gexs_tp;
+
+/* Cross-sections of the resident list at ntp (T,P) points over one grid, in
+ * one launch sequence.  out (host or device pointer) receives ntp*npoints
+ * doubles, row-major by (T,P). */
+int gexs_run_batch(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* grid,
+                   double* out);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous batching over the resident list: requests are admitted by HBM
+ * footprint as they arrive and run while others are on the device. */
+typedef struct gexs_request {
+    double T, P;
+    double start, end;   /* range, cm^-1 */
+    double dfreq;        /* resolution */
+} gexs_request;
+
+typedef struct gexs_result {
+    int32_t status;      /* 0 done, 1 pending, < 0 error */
+    int32_t npoints;
+    const double* sigma; /* valid until gexs_release(id) */
+    double latency_ms;   /* submit -> completion */
+    double device_ms;
+} gexs_result;
+
+int gexs_serve_start(gexs_ctx* ctx, int max_inflight, int64_t mem_budget);
+int gexs_submit(gexs_ctx* ctx, const gexs_request* req, uint64_t* id);
+int gexs_poll(gexs_ctx* ctx, uint64_t id, gexs_result* out);
+int gexs_wait(gexs_ctx* ctx, uint64_t id, gexs_result* out);
+int gexs_release(gexs_ctx* ctx, uint64_t id);
+int gexs_serve_stop(gexs_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,284 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bars (per profile and precision):
+  window indices ib/ie and the nu-sorted line order      bit-exact
+  N_eval                                                  exact
+  FP32C  max|sigma - oracle| / max(oracle)                <= 1e-4
+         |sum sigma - sum oracle| / sum oracle            <= 1e-6
+  FP64   max|sigma - oracle| / max(oracle)                <= 1e-10
+Sizes are the workloads' own grids with line lists the oracle finishes in
+seconds; the full-size lists are covered by sampled points in
+test_gpu_fullsize.py.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gexs_amd import synth
+from gexs_amd.manager import HipManager
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+P_SCALE = synth.P_SCALE_BAR
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def run_manager(profile, lines, T, P, Q, start, end, N, cutoff, precision="fp32c", mass=18.0153,
+                width_conv=0, pressure_mode=1, hwhm=0.1, ppl=4, order=1, start_idx=0, nwin=None):
+    nu, E, A, g, gam, nex = lines
+    dnu = (end - start) / N
+    m = HipManager(profile, precision=precision)
+    m.set_tuning(ppl, order)
+    m.InitializeVectors(N, 1)
+    m.InitializeConstants(100.0, T, Q, dnu, mass, P, 296.0, P_SCALE, cutoff=cutoff,
+                          width_conv=width_conv, pressure_mode=pressure_mode, gauss_hwhm=hwhm)
+    freq = start + np.arange(N, dtype=np.float64) * dnu
+    m.TransferFreq(freq)
+    voigt = profile in ("voigt", "voigt_quad")
+    m.TransferVectors(nu.shape[0], E, nu, A, g, gam if voigt else None, nex if voigt else None)
+    m.ExecuteCrossSection(N if nwin is None else nwin, nu.shape[0], start_idx, T, P, Q)
+    out = np.zeros(N)
+    m.TransferResults(freq, out, N, 0)
+    st = m.stats()
+    return out, st, m
+
+
+def run_oracle(O, profile, lines, T, P, Q, start, end, N, cutoff, mass=18.0153, width_conv=0,
+               pressure_mode=1, hwhm=0.1, start_idx=0, nwin=None):
+    nu, E, A, g, gam, nex = lines
+    dnu = (end - start) / N
+    p = O.params(profile, T=T, P=P, Q=Q, mass=mass, cutoff=cutoff, width_conv=width_conv,
+                 pressure_mode=pressure_mode, p_scale=P_SCALE, hwhm=hwhm)
+    out, ne = O.cross_section(p, start, dnu, N, nu, E, A, g, gam, nex, nthreads=NTHREADS,
+                              start_idx=start_idx, nwin=nwin)
+    return out, ne, p
+
+
+def check(out, ref, tol, band_tol=None, what=""):
+    peak = np.max(np.abs(ref))
+    assert peak > 0
+    err = np.max(np.abs(out - ref)) / peak
+    log(f"{what}: peak-relative error {err:.3e}")
+    assert err <= tol, f"{what}: {err}"
+    if band_tol is not None:
+        b = abs(out.sum() - ref.sum()) / ref.sum()
+        log(f"{what}: band-strength error {b:.3e}")
+        assert b <= band_tol, f"{what}: band {b}"
+
+
+@pytest.fixture(scope="module")
+def c1_small():
+    L = synth.workload_list("voigt_1e6", scale=0.02)
+    return L, L.lines()
+
+
+@pytest.mark.parametrize("ppl", [1, 4, 8])
+def test_voigt_fp32c_vs_oracle(O, c1_small, ppl):
+    L, lines = c1_small
+    T, P = 296.0, 1.0
+    Q = O.partition(L.E, L.g, T)
+    log(f"voigt fp32c ppl={ppl} lines={lines[0].shape[0]}")
+    out, st, _ = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0, ppl=ppl)
+    ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0)
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-4, 1e-6, f"voigt fp32c ppl={ppl}")
+
+
+def test_voigt_fp64_vs_oracle(O, c1_small):
+    L, lines = c1_small
+    T, P = 296.0, 1.0
+    Q = O.partition(L.E, L.g, T)
+    out, st, _ = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0,
+                             precision="fp64")
+    ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0)
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-10, 1e-10, "voigt fp64")
+
+
+def test_windows_and_sort_bit_exact(O, c1_small):
+    L, lines = c1_small
+    nu = lines[0]
+    T, P = 296.0, 1.0
+    Q = O.partition(L.E, L.g, T)
+    _, _, m = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0)
+    nus, ib, ie = m.line_windows(nu.shape[0])
+    assert np.array_equal(nus, np.sort(nu, kind="stable"))
+    rib, rie = O.windows(nus, 25.0, 0.0, 10000.0 / 1_000_000, 0, 1_000_000)
+    assert np.array_equal(ib, rib) and np.array_equal(ie, rie)
+
+
+def test_windows_round_half_boundaries(O):
+    # positions whose ((nu - cut) - start)/dnu is exactly k + 0.5 in FP64
+    start, dnu, N, cut = 0.0, 0.5, 4000, 1.0
+    k = np.arange(10, 3000, 7, dtype=np.float64)
+    nu = (k + 0.5) * dnu + cut  # exact binary fractions
+    n = nu.shape[0]
+    lines = (nu, np.zeros(n), np.ones(n), np.ones(n, np.int32), np.full(n, 0.05), np.full(n, 0.5))
+    _, _, m = run_manager("voigt", lines, 296.0, 1.0, 1.0, start, start + N * dnu, N, cut)
+    nus, ib, ie = m.line_windows(n)
+    rib, rie = O.windows(nus, cut, start, dnu, 0, N)
+    assert np.array_equal(ib, rib) and np.array_equal(ie, rie)
+    assert np.all(ib == (k + 1).astype(np.int32))  # half away from zero
+
+
+def test_doppler_full_config_vs_oracle(O):
+    L = synth.workload_list("doppler_2e5")
+    lines = L.lines()
+    T = 1000.0
+    Q = O.partition(L.E, L.g, T)
+    ref, ne, _ = run_oracle(O, "doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0)
+    for prec, tol, btol in (("fp32c", 1e-4, 1e-6), ("fp64", 1e-10, 1e-10)):
+        out, st, _ = run_manager("doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0,
+                                 precision=prec)
+        assert st["n_eval"] == ne
+        check(out, ref, tol, btol, f"doppler {prec}")
+
+
+def test_gaussian_vs_oracle(O, c1_small):
+    L, lines = c1_small
+    T, hwhm = 296.0, 0.05
+    Q = O.partition(L.E, L.g, T)
+    ref, ne, _ = run_oracle(O, "gaussian", lines, T, 1.0, Q, 0.0, 10000.0, 200_000, 0.5,
+                            hwhm=hwhm)
+    for prec, tol, btol in (("fp32c", 1e-4, 1e-6), ("fp64", 1e-10, 1e-10)):
+        out, st, _ = run_manager("gaussian", lines, T, 1.0, Q, 0.0, 10000.0, 200_000, 0.5,
+                                 precision=prec, hwhm=hwhm)
+        assert st["n_eval"] == ne
+        check(out, ref, tol, btol, f"gaussian {prec}")
+
+
+def test_voigt_quad_vs_oracle(O):
+    L = synth.workload_list("voigt_1e6", scale=0.002)
+    lines = L.lines()
+    T, P = 1000.0, 0.05
+    Q = O.partition(L.E, L.g, T)
+    ref, ne, _ = run_oracle(O, "voigt_quad", lines, T, P, Q, 0.0, 10000.0, 500_000, 10.0)
+    for prec, tol, btol in (("fp32c", 1e-4, 1e-6), ("fp64", 1e-10, 1e-10)):
+        out, st, _ = run_manager("voigt_quad", lines, T, P, Q, 0.0, 10000.0, 500_000, 10.0,
+                                 precision=prec)
+        assert st["n_eval"] == ne
+        check(out, ref, tol, btol, f"voigt-quad {prec}")
+
+
+@pytest.mark.parametrize("T,P", [(2000.0, 1e-4), (200.0, 1e-5), (3000.0, 100.0), (296.0, 0.0)])
+def test_voigt_pressure_regimes(O, T, P):
+    # low P: Humlicek regions III/IV at line centres; high P: region I everywhere;
+    # P = 0: the Doppler limit (y = 0)
+    L = synth.workload_list("voigt_tp_table_1e7", scale=0.001)
+    lines = L.lines()
+    Q = O.partition(L.E, L.g, T)
+    out, st, _ = run_manager("voigt", lines, T, P, Q, 500.0, 15000.0, 1_450_000, 25.0, mass=28.0)
+    ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 500.0, 15000.0, 1_450_000, 25.0,
+                            mass=28.0)
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-4, 1e-6, f"voigt T={T} P={P}")
+
+
+def test_reference_cpu_conventions(O, c1_small):
+    # the reference CPU path: no pressure in gammaL, cutoff min(25P, 100),
+    # 1/w = 1/(nu*dpwcoef); and the textbook width convention
+    L, lines = c1_small
+    T, P = 500.0, 0.5
+    Q = O.partition(L.E, L.g, T)
+    for wc in (0, 1):
+        out, st, _ = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 500_000, 0.0,
+                                 pressure_mode=0, width_conv=wc)
+        ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 0.0, 10000.0, 500_000,
+                                min(25 * P, 100.0), pressure_mode=0, width_conv=wc)
+        assert st["n_eval"] == ne
+        check(out, ref, 1e-4, 1e-6, f"reference conventions width_conv={wc}")
+
+
+def test_window_subset_and_chunk_accumulation(O, c1_small):
+    L, (nu, E, A, g, gam, nex) = c1_small
+    T, P, N = 296.0, 1.0, 300_000
+    Q = O.partition(L.E, L.g, T)
+    dnu = 10000.0 / N
+    m = HipManager("voigt")
+    m.InitializeVectors(N, 2)
+    m.InitializeConstants(100.0, T, Q, dnu, 18.0153, P, 296.0, P_SCALE, cutoff=25.0)
+    freq = np.arange(N) * dnu
+    m.TransferFreq(freq)
+    half = nu.shape[0] // 2
+    # two chunks into result grid 0 (a chunk restarts the execute counter), a
+    # window subset into grid 1 of the second chunk
+    m.TransferVectors(half, E[:half], nu[:half], A[:half], g[:half], gam[:half], nex[:half])
+    m.ExecuteCrossSection(N, half, 0, T, P, Q)
+    m.TransferVectors(nu.shape[0] - half, E[half:], nu[half:], A[half:], g[half:], gam[half:],
+                      nex[half:])
+    m.ExecuteCrossSection(N, nu.shape[0] - half, 0, T, P, Q)
+    m.ExecuteCrossSection(50_000, nu.shape[0] - half, 100_000, T, P, Q)
+    out0 = np.zeros(N)
+    out1 = np.zeros(N)
+    m.TransferResults(freq, out0, N, 0)
+    m.TransferResults(freq, out1, N, 1)
+    ref, _, _ = run_oracle(O, "voigt", (nu, E, A, g, gam, nex), T, P, Q, 0.0, 10000.0, N, 25.0)
+    check(out0, ref, 1e-4, 1e-6, "two chunks")
+    sub = (nu[half:], E[half:], A[half:], g[half:], gam[half:], nex[half:])
+    ref1, _, _ = run_oracle(O, "voigt", sub, T, P, Q, 0.0, 10000.0, N, 25.0, start_idx=100_000,
+                            nwin=50_000)
+    assert np.all(out1[:100_000] == 0) and np.all(out1[150_000:] == 0)
+    check(out1, ref1, 1e-4, 1e-6, "window subset")
+
+
+def test_edge_cases(O):
+    T, P, N = 296.0, 1.0, 1000
+    start, end = 100.0, 110.0
+    nu = np.array([0.0, 99.0, 100.0, 104.999, 105.0, 105.0, 109.99, 111.0, 500.0])
+    n = nu.shape[0]
+    lines = (nu, np.full(n, 100.0), np.full(n, 1.0), np.full(n, 3, np.int32), np.full(n, 0.07),
+             np.full(n, 0.6))
+    out, st, _ = run_manager("voigt", lines, T, P, 50.0, start, end, N, 0.5, ppl=1)
+    ref, ne, _ = run_oracle(O, "voigt", lines, T, P, 50.0, start, end, N, 0.5)
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-4, 1e-6, "edge lines")
+    # a grid smaller than one wave, and an empty chunk
+    out, st, _ = run_manager("voigt", lines, T, P, 50.0, 104.9, 105.1, 20, 0.5, ppl=1)
+    ref, ne, _ = run_oracle(O, "voigt", lines, T, P, 50.0, 104.9, 105.1, 20, 0.5)
+    check(out, ref, 1e-4, 1e-6, "tiny grid")
+    empty = tuple(a[:0] for a in lines)
+    out, st, _ = run_manager("voigt", empty, T, P, 50.0, start, end, N, 0.5)
+    assert np.all(out == 0) and st["n_eval"] == 0
+
+
+def test_deterministic_repeat(c1_small, O):
+    L, lines = c1_small
+    Q = O.partition(L.E, L.g, 296.0)
+    a, _, _ = run_manager("voigt", lines, 296.0, 1.0, Q, 0.0, 10000.0, 1_000_000, 25.0)
+    b, _, _ = run_manager("voigt", lines, 296.0, 1.0, Q, 0.0, 10000.0, 1_000_000, 25.0, order=0)
+    assert np.array_equal(a, b)
+
+
+def test_device_partition(O):
+    L = synth.workload_list("voigt_tp_table_1e7", scale=0.01)
+    m = HipManager("voigt")
+    m.load_list(L)
+    Ts = np.array([200.0, 296.0, 1000.0, 3000.0])
+    Q = m.partition(Ts)
+    for t, q in zip(Ts, Q):
+        assert abs(q - O.partition(L.E, L.g, t)) / q < 1e-13
+
+
+def test_batch_equals_manager(O):
+    L = synth.workload_list("voigt_tp_table_1e7", scale=0.002)
+    nu, E, A, g, gam, nex = L.lines()
+    A32 = L.A.astype(np.float64)  # the resident list stores FP32 A, gamma, n
+    lines = (nu, E, A32, g, L.gamma.astype(np.float64), L.nexp.astype(np.float64))
+    tps = [(300.0, 1.0), (1500.0, 0.01), (2500.0, 30.0)]
+    N, start, end = 145_000, 500.0, 15000.0
+    m = HipManager("voigt")
+    m.InitializeConstants(100.0, 296.0, 1.0, (end - start) / N, 28.0, 1.0, 296.0, P_SCALE,
+                          cutoff=25.0)
+    m.load_list(L)
+    Q = m.partition([t for t, _ in tps])
+    batch = m.run_batch([(t, p, q) for (t, p), q in zip(tps, Q)], start, end, N)
+    for k, (t, p) in enumerate(tps):
+        out, _, _ = run_manager("voigt", lines, t, p, Q[k], start, end, N, 25.0, mass=28.0)
+        assert np.array_equal(batch[k], out), k
+        ref, _, _ = run_oracle(O, "voigt", lines, t, p, Q[k], start, end, N, 25.0, mass=28.0)
+        check(batch[k], ref, 1e-4, 1e-6, f"batch T={t} P={p}")

@@ -1,0 +1,240 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each check is chosen so that a plausible slip in the oracle (a dropped factor,
+a wrong sign or index, banker's instead of half-away rounding, a swapped
+Gauss-Hermite weight) fails: closed forms, library routines
+(scipy.special.wofz / erfcx / erf, numpy hermgauss), brute force, invariants,
+and -- where oracle/_ref is built -- the reference's own CPU code.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erf, erfcx, wofz
+
+from gexs_amd import synth
+
+
+def lines_small(scale=0.002, name="voigt_1e6"):
+    L = synth.workload_list(name, scale=scale)
+    return L, L.lines()
+
+
+# ------------------------------------------------------------------ Q(T)
+def test_partition_two_level_closed_form(O):
+    E = np.array([0.0, 1000.0])
+    g = np.array([3, 7], dtype=np.int32)
+    for T in (100.0, 296.0, 3000.0):
+        c2 = 6.6260693e-27 * 2.99792458e10 / (1.380658e-16)
+        q = 3 + 7 * math.exp(-c2 * 1000.0 / T)
+        assert O.partition(E, g, T) == pytest.approx(q, rel=1e-15)
+
+
+def test_partition_mpmath_and_invariants(O):
+    import mpmath as mp
+    mp.mp.dps = 40
+    E, g, _ = synth.make_states(7, 3000, 4000.0, 20000.0)
+    c2 = mp.mpf("6.6260693e-27") * mp.mpf("2.99792458e10") / mp.mpf("1.380658e-16")
+    prev = 0.0
+    for T in (200.0, 296.0, 1000.0, 3000.0):
+        exact = mp.fsum(int(gi) * mp.e ** (-c2 * mp.mpf(float(e)) / T) for e, gi in zip(E, g))
+        q = O.partition(E, g, T)
+        assert abs(q - float(exact)) / float(exact) < 1e-13
+        assert q >= g[E == 0.0].sum()  # ground states count fully
+        assert q > prev  # monotone in T
+        prev = q
+
+
+def test_partition_matches_reference(O, R):
+    E, g, _ = synth.make_states(11, 5000, 4000.0, 20000.0)
+    for T in (296.0, 1000.0):
+        assert O.partition(E, g, T) == R.gxr_partition(E, g, E.shape[0], T)
+
+
+# ----------------------------------------------------- selection, windows
+def test_select_filter_both_sides(O):
+    # keep iff start-hw <= nu <= end+hw and nu != 0 (BaseProfile.cpp:152-157, no break)
+    assert O.lib().gxo_select(5.0, 0.0, 10.0, 1.0) == 1
+    assert O.lib().gxo_select(-1.0, 0.0, 10.0, 1.0) == 1
+    assert O.lib().gxo_select(-1.0000001, 0.0, 10.0, 1.0) == 0
+    assert O.lib().gxo_select(11.0, 0.0, 10.0, 1.0) == 1
+    assert O.lib().gxo_select(11.0000001, 0.0, 10.0, 1.0) == 0
+    assert O.lib().gxo_select(0.0, -5.0, 10.0, 1.0) == 0
+
+
+def test_windows_round_half_away_and_clamp(O):
+    # ((nu - cut) - start)/dnu lands exactly on k + 0.5 (all values binary-exact)
+    ib, ie = O.windows(np.array([3.25]), 1.0, 0.0, 0.5, 0, 100)
+    assert (ib[0], ie[0]) == (5, 9)        # 4.5 -> 5, 8.5 -> 9 (numpy's round gives 4, 8)
+    ib, ie = O.windows(np.array([0.25]), 1.5, 0.0, 0.5, -100, 100)
+    assert (ib[0], ie[0]) == (-3, 4)       # -2.5 -> -3, 3.5 -> 4
+    ib, ie = O.windows(np.array([0.25]), 1.5, 0.0, 0.5, 0, 3)
+    assert (ib[0], ie[0]) == (0, 3)        # clamped at lo and hi
+    ib, ie = O.windows(np.array([0.0]), 1.0, 0.0, 0.5, 0, 100)
+    assert (ib[0], ie[0]) == (0, 2)        # nu == 0 -> 1e-6 (OpenMPManager.cpp:392)
+
+
+def test_windows_brute_force(O):
+    rng = np.random.default_rng(3)
+    start, dnu, N, cut = 10.0, 0.037, 500, 0.4
+    nu = rng.uniform(start - 2, start + N * dnu + 2, 2000)
+    ib, ie = O.windows(nu, cut, start, dnu, 0, N)
+    for v, b, e in zip(nu, ib, ie):
+        lo = math.floor((v - cut - start) / dnu + 0.5)  # half-away == floor(x+.5) for x not at -k-.5
+        hi = math.floor((v + cut - start) / dnu + 0.5)
+        assert b == max(lo, 0) and e == min(hi, N)
+
+
+# ------------------------------------------------------------ Voigt K
+def test_humlic_vs_wofz_and_special_cases(O):
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(0, 20, 3000), rng.uniform(0, 6, 3000)])
+    ys = np.concatenate([10 ** rng.uniform(-5, 2, 3000), 10 ** rng.uniform(-5, 0.7, 3000)])
+    h = np.array([O.humlic(x, y) for x, y in zip(xs, ys)])
+    w = wofz(xs + 1j * ys).real
+    # Humlicek (1982) W4 relative accuracy bound 1e-4
+    assert np.max(np.abs(h - w) / w) < 1e-4
+    for y in (0.01, 0.5, 3.0, 30.0):
+        assert O.humlic(0.0, y) == pytest.approx(erfcx(y), rel=1e-4)
+    for x in (0.3, 1.0, 2.0):
+        assert O.humlic(x, 1e-7) == pytest.approx(math.exp(-x * x), rel=1e-4, abs=1e-6)
+    # region I (|x|+y >= 15) in closed form: 0.5641896 y (b + x^2)/((b - x^2)^2 + 4x^2y^2), b = 1/2 + y^2
+    x, y = 100.0, 2.0
+    b = 0.5 + y * y
+    assert O.humlic(x, y) == pytest.approx(0.5641896 * y * (b + x * x) / ((b - x * x) ** 2 + 4 * x * x * y * y), rel=1e-14)
+    # symmetric in x
+    assert O.humlic(-2.3, 0.4) == O.humlic(2.3, 0.4)
+
+
+def test_humlic_matches_reference_bit_for_bit(O, R):
+    rng = np.random.default_rng(6)
+    for x, y in zip(rng.uniform(0, 20, 5000), 10 ** rng.uniform(-5, 2, 5000)):
+        assert O.humlic(x, y) == R.gxr_humlic(x, y)
+
+
+def test_voigt916_and_gh11(O):
+    rng = np.random.default_rng(7)
+    xs, ys = rng.uniform(0, 12, 4000), 10 ** rng.uniform(-4, 1.2, 4000)
+    w = wofz(xs + 1j * ys).real
+    v = np.array([O.voigt916(x, y) for x, y in zip(xs, ys)])
+    core = xs ** 2 + ys ** 2 < 100
+    assert np.max(np.abs(v[core] - w[core]) / w[core]) < 1e-9
+    q = np.array([O.gh11(x, y) for x, y in zip(xs, ys)])
+    assert np.max(np.abs(q[~core] - w[~core]) / w[~core]) < 1e-6
+    vq = np.array([O.voigt_quad(x, y) for x, y in zip(xs, ys)])
+    assert np.max(np.abs(vq - w) / w) < 1e-6
+    assert O.voigt916(0.0, 2.0) == pytest.approx(erfcx(2.0), rel=1e-14)
+    assert O.voigt916(1.5, 0.0) == pytest.approx(math.exp(-2.25), rel=1e-14)
+
+
+def test_gh11_nodes_are_hermgauss():
+    t, w = np.polynomial.hermite.hermgauss(11)
+    src = open(__import__("os").path.join(__import__("os").path.dirname(__file__), "..", "oracle",
+                                          "gexs_oracle.c")).read()
+    for tk, wk in zip(t[6:], w[6:]):
+        assert repr(float(tk * tk))[:12] in src and repr(float(wk))[:12] in src
+    assert repr(float(w[5]))[:12] in src
+    assert abs(w.sum() - math.sqrt(math.pi)) < 1e-14
+
+
+# --------------------------------------------------- line sums, invariants
+def test_voigt_matches_reference_compute_voigt_tp(O, R):
+    L, (nu, E, A, g, gam, nex) = lines_small(0.002)
+    T, P, N = 296.0, 1.0, 200000
+    start, dnu = 0.0, 10000.0 / N
+    Q = O.partition(L.E, L.g, T)
+    ref = O.ref_voigt_tp(start, dnu, N, T, P, Q, 18.0153, 296.0, 1 / 1.01325, nu, E, A, g, gam,
+                         nex, nthreads=3)
+    # reference CPU conventions: gammaL without pressure, cutoff min(25P, 100)
+    p = O.params("voigt", T=T, P=P, Q=Q, cutoff=min(25 * P, 100.0), pressure_mode=0)
+    out, _ = O.cross_section(p, start, dnu, N, nu, E, A, g, gam, nex, nthreads=3)
+    assert np.array_equal(out, ref)
+
+
+def test_doppler_matches_reference_compute_doppler(O, R):
+    L, (nu, E, A, g, _, _) = lines_small(0.002, "doppler_2e5")
+    T, N = 1000.0, 100000
+    start, dnu = 0.0, 10000.0 / N
+    Q = O.partition(L.E, L.g, T)
+    freq = start + np.arange(N) * dnu
+    ref = np.zeros(N)
+    R.gxr_doppler(2, freq, N, dnu, T, Q, 18.0153, nu.shape[0], E, nu, A, g, 0, N, ref)
+    p = O.params("doppler", T=T, Q=Q, cutoff=100.0)  # DOPPLER_CUTOFF (defines.h:16)
+    out, _ = O.cross_section(p, start, dnu, N, nu, E, A, g, nthreads=2)
+    assert np.max(np.abs(out - ref * (0.5 / dnu))) <= 1e-14 * out.max()
+
+
+def test_doppler_band_strength_telescopes(O):
+    L, (nu, E, A, g, _, _) = lines_small(0.001, "doppler_2e5")
+    T, N = 1000.0, 400000
+    start, dnu = 0.0, 10000.0 / N
+    Q = O.partition(L.E, L.g, T)
+    keep = (nu > 50) & (nu < 9950)
+    p = O.params("doppler", T=T, Q=Q, cutoff=5.0)
+    out, _ = O.cross_section(p, start, dnu, N, nu[keep], E[keep], A[keep], g[keep])
+    tot = O.total_intensity(T, Q, nu[keep], E[keep], A[keep], g[keep])
+    assert abs(out.sum() * dnu - tot) / tot < 1e-8  # FP64 erf-difference rounding ~1e-10
+
+
+def test_doppler_single_line_closed_form(O):
+    # one line: sigma_j = S * 0.5/dnu * (erf(xp) - erf(xm)), computed here with scipy
+    nu, E, A, g = np.array([1000.004]), np.array([200.0]), np.array([3.0]), np.array([5], np.int32)
+    T, Q, N, start, dnu = 500.0, 100.0, 400, 999.0, 0.005
+    p = O.params("doppler", T=T, Q=Q, cutoff=0.8, mass=28.0)
+    out, ne = O.cross_section(p, start, dnu, N, nu, E, A, g)
+    S, _, iw = O.line_params(p, nu, E, A, g, np.zeros(1), np.zeros(1))
+    c2 = 6.6260693e-27 * 2.99792458e10 / 1.380658e-16
+    S_ref = 1 / (8 * 3.14159265359 * 2.99792458e10) * 3.0 * 5 * math.exp(-c2 * 200 / T) * \
+        (1 - math.exp(-c2 * 1000.004 / T)) / (1000.004 ** 2 * Q)
+    assert S[0] == pytest.approx(S_ref, rel=1e-14)
+    dpw = math.sqrt(2 * 0.69314718056 * 1.380658e-16 * T * 6.0221412927e23 / 28.0) / 2.99792458e10
+    assert iw[0] == pytest.approx(1 / (1000.004 * dpw), rel=1e-14)
+    f = start + np.arange(N) * dnu
+    d = np.abs(1000.004 - f)
+    want = S_ref * 0.5 / dnu * (erf(iw[0] * (d + dnu / 2)) - erf(iw[0] * (d - dnu / 2)))
+    ib = round((1000.004 - 0.8 - start) / dnu)
+    ie = round((1000.004 + 0.8 - start) / dnu)
+    want[:ib] = 0
+    want[ie:] = 0
+    assert ne == ie - ib
+    assert np.max(np.abs(out - want)) < 1e-12 * want.max()
+
+
+def test_voigt_line_integral_and_symmetry(O):
+    # a strong isolated line on a fine grid: sum(sigma) dnu -> S/sqrt(pi) * sqrt(pi)... the
+    # oracle's S carries 1/sqrt(pi) and K integrates to sqrt(pi) over x, so the
+    # integral over nu is S_voigt * sqrt(pi) (less the far-wing truncation).
+    nu, E, A, g = np.array([2000.0]), np.array([0.0]), np.array([1.0]), np.array([1], np.int32)
+    gam, nex = np.array([0.05]), np.array([0.5])
+    T, P, N, start, dnu = 296.0, 1.0, 400000, 1980.0, 1e-4
+    p = O.params("voigt", T=T, P=P, Q=1.0, cutoff=20.0)
+    out, _ = O.cross_section(p, start, dnu, N, nu, E, A, g, gam, nex)
+    S, gL, iw = O.line_params(p, nu, E, A, g, gam, nex)
+    integral = out.sum() * dnu
+    # Lorentz wings beyond +-20 cm^-1 carry (2/pi) * gL / 20 of the area
+    want = S[0] * math.sqrt(math.pi) * (1 - 2 / math.pi * gL[0] / 20.0)
+    assert integral == pytest.approx(want, rel=2e-4)
+    j0 = round((2000.0 - start) / dnu)
+    assert np.allclose(out[j0 - 5000:j0], out[j0 + 5000:j0:-1], rtol=1e-12)
+
+
+def test_cross_section_at_equals_full(O):
+    L, (nu, E, A, g, gam, nex) = lines_small(0.001)
+    o = np.argsort(nu, kind="stable")
+    nu, E, A, g, gam, nex = nu[o], E[o], A[o], g[o], gam[o], nex[o]
+    T, P, N = 600.0, 0.3, 100000
+    start, dnu = 0.0, 10000.0 / N
+    Q = O.partition(L.E, L.g, T)
+    p = O.params("voigt", T=T, P=P, Q=Q, cutoff=25.0)
+    full, _ = O.cross_section(p, start, dnu, N, nu, E, A, g, gam, nex)
+    pts = np.random.default_rng(1).integers(0, N, 3000).astype(np.int32)
+    at = O.cross_section_at(p, start, dnu, N, nu, E, A, g, gam, nex, pts)
+    assert np.array_equal(at, full[pts])
+
+
+def test_neval_counts_windows(O):
+    L, (nu, E, A, g, gam, nex) = lines_small(0.001)
+    N, start, dnu = 100000, 0.0, 0.1
+    p = O.params("voigt", T=296.0, P=1.0, Q=100.0, cutoff=25.0)
+    ib, ie = O.windows(nu, 25.0, start, dnu, 0, N)
+    assert O.neval(p, start, dnu, N, nu, E, A, g) == int(np.maximum(ie - ib, 0).sum())
