@@ -19,6 +19,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libgexs_hip.so")
 
 SOURCES = ["kernels.hip", "sort.hip", "serve.hip", "capi.hip"]
+HOST_SOURCES = ["HipManager.cpp", "host/readers.cpp", "host/cross_section.cpp"]
+CLI = os.path.join(HERE, "gexs_xsec")
 ARCH = os.environ.get("GEXS_OFFLOAD_ARCH", "gfx950")
 
 
@@ -53,6 +55,19 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed on {src}")
         objs.append(obj)
+    cxx = shutil.which("g++") or "g++"
+    hflags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"),
+              "-I", CSRC]
+    for src in HOST_SOURCES:
+        obj = os.path.join(bdir, os.path.basename(src).replace(".cpp", ".o"))
+        cmd = [cxx, *hflags, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed on {src}")
+        objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -60,6 +75,13 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libgexs_hip.so failed")
     os.replace(tmp, LIB)
+    cmd = [cxx, *hflags, os.path.join(CSRC, "host", "xsec_main.cpp"), "-o", CLI + ".tmp",
+           "-L", HERE, "-lgexs_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of gexs_xsec failed")
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

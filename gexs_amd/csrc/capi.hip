@@ -62,11 +62,11 @@ struct DevArr {
 struct PosKey {
     double cutoff = NAN, start = NAN, dfreq = NAN;
     int win_lo = -1, win_hi = -1;
-    int64_t n = -1;
+    int64_t l0 = -1, n = -1;  // line sub-range [l0, l0 + n) of the current lines
     uint64_t gen = 0;
     bool operator==(const PosKey& o) const {
         return cutoff == o.cutoff && start == o.start && dfreq == o.dfreq && win_lo == o.win_lo &&
-               win_hi == o.win_hi && n == o.n && gen == o.gen;
+               win_hi == o.win_hi && l0 == o.l0 && n == o.n && gen == o.gen;
     }
 };
 
@@ -99,7 +99,7 @@ struct gexs_ctx {
     uint64_t gen = 0;
     DevArr nu, E, A, gamma, nexp, g;
     // unsorted upload staging and sort scratch
-    DevArr r_nu, r_E, r_A, r_gamma, r_nexp, r_g, keys, perm;
+    DevArr r_nu, r_E, r_A, r_gamma, r_nexp, r_g, keys, perm, scratch8;
     gxk::SortBuffers sb;
 
     // resident states (for the device partition function)
@@ -194,12 +194,23 @@ int set_device(gexs_ctx* c) {
 }
 
 // sort the staged (r_*) lines by nu into the current arrays
-int sort_and_commit(gexs_ctx* c, int64_t n, bool has_gamma) {
+// drop_zero: lines at nu == 0 are removed (the driver never selects them,
+// Profiles/BaseProfile.cpp:156); the manager path keeps every line it is given.
+int sort_and_commit(gexs_ctx* c, int64_t n, bool has_gamma, bool drop_zero) {
     hipStream_t s = c->stream;
     HIPCHK(c, c->keys.ensure((size_t)n * 8));
     HIPCHK(c, c->perm.ensure((size_t)n * 4));
-    HIPCHK(c, gxk::make_nu_keys(s, c->r_nu.as<double>(), n, c->keys.as<uint64_t>(), c->perm.as<uint32_t>()));
+    HIPCHK(c, c->scratch8.ensure(64));
+    HIPCHK(c, gxk::fill_u64(s, c->scratch8.as<unsigned long long>(), 1, 0ull));
+    HIPCHK(c, gxk::make_nu_keys(s, c->r_nu.as<double>(), n, c->keys.as<uint64_t>(), c->perm.as<uint32_t>(),
+                                drop_zero ? 1 : 0, c->scratch8.as<unsigned long long>()));
     HIPCHK(c, gxk::radix_sort_u64(s, c->keys.as<uint64_t>(), c->perm.as<uint32_t>(), n, c->sb, 0, 64));
+    if (drop_zero) {
+        unsigned long long zeros = 0;
+        HIPCHK(c, hipMemcpyAsync(&zeros, c->scratch8.p, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        n -= (int64_t)zeros;
+    }
     HIPCHK(c, c->nu.ensure((size_t)n * 8));
     HIPCHK(c, c->E.ensure((size_t)n * 8));
     HIPCHK(c, c->A.ensure((size_t)n * 8));
@@ -223,18 +234,21 @@ int sort_and_commit(gexs_ctx* c, int64_t n, bool has_gamma) {
 }
 
 // window + nearest point of every current line for one grid window
-int ensure_pos(gexs_ctx* c, double cutoff, double start, double dfreq, int win_lo, int win_hi) {
+// lines [l0, l0 + n) of the current lines take part in the pass
+int ensure_pos(gexs_ctx* c, double cutoff, double start, double dfreq, int win_lo, int win_hi,
+               int64_t l0, int64_t n) {
     PosKey key;
     key.cutoff = cutoff;
     key.start = start;
     key.dfreq = dfreq;
     key.win_lo = win_lo;
     key.win_hi = win_hi;
-    key.n = c->n;
+    key.l0 = l0;
+    key.n = n;
     key.gen = c->gen;
     if (key == c->pos_key) return GEXS_OK;
-    HIPCHK(c, c->pos.ensure((size_t)std::max<int64_t>(c->n, 1) * sizeof(LinePos)));
-    HIPCHK(c, gxk::line_pos(c->stream, c->nu.as<double>(), (int)c->n, cutoff, start, dfreq, win_lo,
+    HIPCHK(c, c->pos.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(LinePos)));
+    HIPCHK(c, gxk::line_pos(c->stream, c->nu.as<double>() + l0, (int)n, cutoff, start, dfreq, win_lo,
                             win_hi, c->pos.as<LinePos>()));
     c->pos_key = key;
     c->items_key = PosKey();  // tiles depend on positions
@@ -259,7 +273,7 @@ int ensure_items(gexs_ctx* c, int njobs) {
     WorkItem* dst = sorted ? c->items_tmp.as<WorkItem>() : c->items.as<WorkItem>();
     for (int j = 0; j < njobs; ++j) {
         int nt = 0;
-        HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), (int)c->n, win_lo, win_hi, span, j, dst,
+        HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), (int)key.n, win_lo, win_hi, span, j, dst,
                              sorted ? c->cost_key.as<uint32_t>() : nullptr,
                              sorted ? c->item_idx.as<uint32_t>() : nullptr, j * per, &nt));
     }
@@ -277,6 +291,21 @@ int ensure_items(gexs_ctx* c, int njobs) {
     return GEXS_OK;
 }
 
+int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
+    if (c->n == 0) {
+        *l0 = *l1 = 0;
+        return GEXS_OK;
+    }
+    HIPCHK(c, c->scratch8.ensure(64));
+    HIPCHK(c, gxk::nu_range(c->stream, c->nu.as<double>(), c->n, lo, hi, c->scratch8.as<long long>()));
+    long long r[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(r, c->scratch8.p, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *l0 = r[0];
+    *l1 = r[1];
+    return GEXS_OK;
+}
+
 int upload_tp(gexs_ctx* c, const std::vector<TPParams>& tps) {
     HIPCHK(c, c->tp.ensure(tps.size() * sizeof(TPParams)));
     HIPCHK(c, hipMemcpyAsync(c->tp.p, tps.data(), tps.size() * sizeof(TPParams), hipMemcpyHostToDevice,
@@ -289,7 +318,8 @@ int upload_tp(gexs_ctx* c, const std::vector<TPParams>& tps) {
 int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<double*>& outs,
               bool timed) {
     const int njobs = (int)tps.size();
-    const int n = (int)c->n;
+    const int n = (int)c->pos_key.n;
+    const int64_t l0 = c->pos_key.l0;
     hipStream_t s = c->stream;
     int rc;
     if ((rc = upload_tp(c, tps)) != GEXS_OK) return rc;
@@ -300,12 +330,15 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
     HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
     HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 1, 0ull));
-    const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
-    const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(), n,
-                            c->pos.as<LinePos>(), c->tp.as<TPParams>(), njobs,
+    const double* gam = c->has_gamma ? c->gamma.as<double>() + l0 : nullptr;
+    const double* nex = c->has_gamma ? c->nexp.as<double>() + l0 : nullptr;
+    const double* nu = c->nu.as<double>() + l0;
+    const double* E = c->E.as<double>() + l0;
+    const double* A = c->A.as<double>() + l0;
+    const int32_t* g = c->g.as<int32_t>() + l0;
+    HIPCHK(c, gxk::line_max(s, nu, E, A, g, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(), njobs,
                             c->scale.as<unsigned long long>()));
-    HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+    HIPCHK(c, gxk::line_rec(s, nu, E, A, g,
                             gam, nex, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(), njobs,
                             c->scale.as<unsigned long long>(), c->rec.as<VRec>(), n,
                             c->neval.as<unsigned long long>()));
@@ -332,20 +365,21 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
 }
 
 int run_fp64(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
-    const int n = (int)c->n;
+    const int n = (int)c->pos_key.n;
+    const int64_t l0 = c->pos_key.l0;
     hipStream_t s = c->stream;
     int rc;
     if ((rc = upload_tp(c, std::vector<TPParams>{tp})) != GEXS_OK) return rc;
     HIPCHK(c, c->lf64.ensure((size_t)std::max(n, 1) * sizeof(LineF64)));
     HIPCHK(c, c->neval.ensure(8));
     HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), 1, 0ull));
-    const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
-    const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    HIPCHK(c, gxk::line_f64(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            gam, nex, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(),
+    const double* gam = c->has_gamma ? c->gamma.as<double>() + l0 : nullptr;
+    const double* nex = c->has_gamma ? c->nexp.as<double>() + l0 : nullptr;
+    HIPCHK(c, gxk::line_f64(s, c->nu.as<double>() + l0, c->E.as<double>() + l0, c->A.as<double>() + l0,
+                            c->g.as<int32_t>() + l0, gam, nex, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(),
                             c->lf64.as<LineF64>(), c->neval.as<unsigned long long>()));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-    HIPCHK(c, gxk::fp64_points(s, c->profile, c->nu.as<double>(), c->pos.as<LinePos>(), c->lf64.as<LineF64>(), n,
+    HIPCHK(c, gxk::fp64_points(s, c->profile, c->nu.as<double>() + l0, c->pos.as<LinePos>(), c->lf64.as<LineF64>(), n,
                                tp.start, tp.dfreq, tp.win_lo, tp.win_hi, out));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     c->last.n_tiles = 0;
@@ -497,7 +531,7 @@ static int transfer_common(gexs_ctx* c, int64_t n, const double* E, const double
             HIPCHK(c, hipMemcpyAsync(c->r_nexp.p, nexp, (size_t)n * 8, kind, s));
         }
     }
-    if ((rc = sort_and_commit(c, n, has_gamma)) != GEXS_OK) return rc;
+    if ((rc = sort_and_commit(c, n, has_gamma, false)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(s));  // caller's buffers are free on return
     return GEXS_OK;
 }
@@ -533,7 +567,7 @@ int gexs_execute(gexs_ctx* c, int npoints, int64_t nlines, int start_idx, double
     const double dfreq = c->k.dfreq;
     const int win_lo = start_idx, win_hi = start_idx + npoints;
     TPParams tp = make_tp(c, T, P, Q, c->start, dfreq, win_lo, win_hi);
-    if ((rc = ensure_pos(c, tp.cutoff, c->start, dfreq, win_lo, win_hi)) != GEXS_OK) return rc;
+    if ((rc = ensure_pos(c, tp.cutoff, c->start, dfreq, win_lo, win_hi, 0, c->n)) != GEXS_OK) return rc;
     if ((rc = next_events(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
     if (c->k.precision == GEXS_FP64)
@@ -616,15 +650,16 @@ int gexs_timing(gexs_ctx* c, int reset, double* kernel_ms, double* total_ms, int
 
 int gexs_line_windows(gexs_ctx* c, int64_t n, double* nu_sorted, int32_t* ib, int32_t* ie) {
     if (!c) return GEXS_EINVAL;
-    if (n != c->n) return c->fail(GEXS_EINVAL, "n differs from the number of current lines");
+    if (!(c->pos_key.gen == c->gen && c->pos_key.n >= 0))
+        return c->fail(GEXS_ESTATE, "no pass over the current lines yet (execute first)");
+    if (n != c->pos_key.n) return c->fail(GEXS_EINVAL, "n differs from the number of lines of the last pass");
     int rc;
     if ((rc = set_device(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (n == 0) return GEXS_OK;
-    if (nu_sorted) HIPCHK(c, hipMemcpy(nu_sorted, c->nu.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (nu_sorted)
+        HIPCHK(c, hipMemcpy(nu_sorted, c->nu.as<double>() + c->pos_key.l0, (size_t)n * 8, hipMemcpyDeviceToHost));
     if (ib || ie) {
-        if (!(c->pos_key.n == c->n && c->pos_key.gen == c->gen))
-            return c->fail(GEXS_ESTATE, "no windows for the current lines yet (execute first)");
         std::vector<LinePos> lp((size_t)n);
         HIPCHK(c, hipMemcpy(lp.data(), c->pos.p, (size_t)n * sizeof(LinePos), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) {
@@ -632,6 +667,15 @@ int gexs_line_windows(gexs_ctx* c, int64_t n, double* nu_sorted, int32_t* ib, in
             if (ie) ie[i] = lp[(size_t)i].ie;
         }
     }
+    return GEXS_OK;
+}
+
+int gexs_pass_lines(gexs_ctx* c, int64_t* first, int64_t* count) {
+    if (!c || !first || !count) return GEXS_EINVAL;
+    if (!(c->pos_key.gen == c->gen && c->pos_key.n >= 0))
+        return c->fail(GEXS_ESTATE, "no pass over the current lines yet");
+    *first = c->pos_key.l0;
+    *count = c->pos_key.n;
     return GEXS_OK;
 }
 
@@ -697,7 +741,7 @@ int gexs_load_list(gexs_ctx* c, const gexs_states* st, const gexs_transitions* t
             HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, c->r_nexp.as<double>()));
         }
     }
-    if ((rc = sort_and_commit(c, n, has_gamma)) != GEXS_OK) return rc;
+    if ((rc = sort_and_commit(c, n, has_gamma, true)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(s));
     return GEXS_OK;
 }
@@ -745,7 +789,12 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     for (int i = 1; i < ntp; ++i)
         if (effective_cutoff(c->k, tp[i].P) != cutoff)
             return c->fail(GEXS_EINVAL, "a batch needs one cutoff (set gexs_constants.cutoff > 0)");
-    if ((rc = ensure_pos(c, cutoff, grid->start, dfreq, 0, grid->npoints)) != GEXS_OK) return rc;
+    // line selection (Profiles/BaseProfile.cpp:152-157 as a filter on both sides):
+    // start - HW <= nu <= end + HW over the nu-sorted list (nu == 0 was dropped at load)
+    int64_t l0 = 0, l1 = c->n;
+    if ((rc = select_range(c, grid->start - c->k.half_width, grid->end + c->k.half_width, &l0, &l1)) != GEXS_OK)
+        return rc;
+    if ((rc = ensure_pos(c, cutoff, grid->start, dfreq, 0, grid->npoints, l0, l1 - l0)) != GEXS_OK) return rc;
     const size_t np = (size_t)grid->npoints;
     HIPCHK(c, c->batch_out.ensure(np * (size_t)ntp * 8));
     std::vector<TPParams> tps((size_t)ntp);

@@ -59,9 +59,13 @@ hipError_t radix_sort_u32(hipStream_t s, uint32_t* keys, uint32_t* vals, int64_t
 hipError_t sort_buffers_alloc(SortBuffers& buf, int64_t n, int key_bytes);
 void sort_buffers_free(SortBuffers& buf);
 
-// nu (double) -> order-preserving uint64 key, and the identity permutation
+// nu (double) -> order-preserving uint64 key, and the identity permutation;
+// drop_zero sends nu == 0 lines to the end and counts them into *zeros
 hipError_t make_nu_keys(hipStream_t s, const double* nu, int64_t n, uint64_t* keys,
-                        uint32_t* vals);
+                        uint32_t* vals, int drop_zero, unsigned long long* zeros);
+// out[0..1] = [first nu >= lo, first nu > hi) of an ascending array
+hipError_t nu_range(hipStream_t s, const double* nu, int64_t n, double lo, double hi,
+                    long long* out);
 // out[i] = in[perm[i]] for 8- and 4-byte elements
 hipError_t gather_f64(hipStream_t s, const double* in, const uint32_t* perm, int64_t n,
                       double* out);

@@ -124,14 +124,41 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* __restrict__
     }
 }
 
+// With drop_zero, lines at nu == 0 get the largest key (they sort last) and
+// are counted, so the caller can cut them off the end.
 __global__ void k_nu_keys(const double* __restrict__ nu, int64_t n, uint64_t* __restrict__ keys,
-                          uint32_t* __restrict__ vals) {
+                          uint32_t* __restrict__ vals, int drop_zero,
+                          unsigned long long* __restrict__ zeros) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t u = (uint64_t)__double_as_longlong(nu[i]);
+    const double v = nu[i];
+    uint64_t u = (uint64_t)__double_as_longlong(v);
     u = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+    if (drop_zero && v == 0.0) {
+        u = ~0ull;
+        atomicAdd(zeros, 1ull);
+    }
     keys[i] = u;
     vals[i] = (uint32_t)i;
+}
+
+// [lower_bound(nu >= lo), upper_bound(nu <= hi)) of an ascending array
+__global__ void k_nu_range(const double* __restrict__ nu, int64_t n, double lo, double hi,
+                           long long* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    int64_t a = 0, b = n;
+    while (a < b) {
+        int64_t m = (a + b) >> 1;
+        if (nu[m] < lo) a = m + 1; else b = m;
+    }
+    out[0] = a;
+    int64_t c = a;
+    b = n;
+    while (c < b) {
+        int64_t m = (c + b) >> 1;
+        if (nu[m] <= hi) c = m + 1; else b = m;
+    }
+    out[1] = c;
 }
 
 template <typename T>
@@ -240,10 +267,16 @@ hipError_t radix_sort_u32(hipStream_t s, uint32_t* keys, uint32_t* vals, int64_t
 }
 
 hipError_t make_nu_keys(hipStream_t s, const double* nu, int64_t n, uint64_t* keys,
-                        uint32_t* vals) {
+                        uint32_t* vals, int drop_zero, unsigned long long* zeros) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_nu_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nu, n, keys,
-                       vals);
+                       vals, drop_zero, zeros);
+    return hipGetLastError();
+}
+
+hipError_t nu_range(hipStream_t s, const double* nu, int64_t n, double lo, double hi,
+                    long long* out) {
+    hipLaunchKernelGGL(k_nu_range, dim3(1), dim3(64), 0, s, nu, n, lo, hi, out);
     return hipGetLastError();
 }
 

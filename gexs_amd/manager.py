@@ -134,8 +134,16 @@ class HipManager:
                                       C.byref(n)))
         return k.value, t.value, n.value
 
-    def line_windows(self, n):
-        """(sorted nu, ib, ie) of the current lines as the device holds them."""
+    def pass_lines(self):
+        """(first, count) of the nu-sorted lines the last pass used (its selection)."""
+        a, b = C.c_int64(), C.c_int64()
+        self._chk(self._L.gexs_pass_lines(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def line_windows(self, n=None):
+        """(sorted nu, ib, ie) of the last pass's lines as the device holds them."""
+        if n is None:
+            n = self.pass_lines()[1]
         nu = np.empty(n, np.float64)
         ib = np.empty(n, np.int32)
         ie = np.empty(n, np.int32)

@@ -132,10 +132,15 @@ int gexs_last_stats(gexs_ctx* ctx, gexs_stats* out);
  * (at most 2048 are kept).  Waits for the ctx stream.  reset != 0 clears. */
 int gexs_timing(gexs_ctx* ctx, int reset, double* kernel_ms, double* total_ms, int* count);
 
-/* The current lines in device order (ascending nu) with the window [ib, ie)
- * each received in the last execute / run_batch (OpenMPManager.cpp:399-400).
- * Any pointer may be NULL; n must equal the number of current lines. */
+/* The lines of the last execute / run_batch in device order (ascending nu)
+ * with the window [ib, ie) each received (OpenMPManager.cpp:399-400).  Any
+ * pointer may be NULL; n must equal the pass's line count (gexs_pass_lines). */
 int gexs_line_windows(gexs_ctx* ctx, int64_t n, double* nu_sorted, int32_t* ib, int32_t* ie);
+/* The line selection of the last pass: lines [first, first + count) of the
+ * nu-sorted current lines (all lines for gexs_execute; for gexs_run_batch the
+ * lines with start - half_width <= nu <= end + half_width,
+ * Profiles/BaseProfile.cpp:152-157). */
+int gexs_pass_lines(gexs_ctx* ctx, int64_t* first, int64_t* count);
 
 /* Tuning of the FP32C profile kernel: points per lane (1, 2, 4 or 8; a
  * wave covers 64*ppl grid points) and the dispatch order of the tiles
@@ -160,8 +165,9 @@ typedef struct gexs_transitions {
     const float* nexp;      /* temperature exponent */
 } gexs_transitions;
 
-/* Upload, assemble lines (nu = E_u - E_l, E'' = E_l, g' = g_u), keep lines
- * with nu > 0 and sort them by nu.  Replaces any previous list. */
+/* Upload, assemble lines (nu = E_u - E_l, E'' = E_l, g' = g_u), drop lines
+ * with nu == 0 (never selected, Profiles/BaseProfile.cpp:156) and sort the
+ * rest by nu on the device.  Replaces any previous list. */
 int gexs_load_list(gexs_ctx* ctx, const gexs_states* st, const gexs_transitions* tr);
 /* Q(T) on the device from the resident states, FP64, fixed reduction order
  * (exomol_functions.cpp:50-73). */

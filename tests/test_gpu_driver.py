@@ -1,0 +1,139 @@
+"""The C++ driver end to end: keyword input -> reader -> selection -> chunks ->
+HipManager -> output text, against the FP64 oracle.
+
+ExoMol inputs are written from a seeded list (states, unsorted transitions and
+a broadener file); HITRAN inputs are 160-character records built here.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gexs_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "gexs_amd", "gexs_xsec")
+
+
+def run_cli(text, cwd):
+    r = subprocess.run([CLI], input=text, capture_output=True, text=True, cwd=cwd, timeout=300)
+    assert r.returncode == 0, r.stderr
+    blocks = {}
+    band = None
+    for line in r.stdout.splitlines():
+        if line.startswith("["):
+            k, rest = line[1:].split("]", 1)
+            f, s = rest.split()
+            blocks.setdefault(int(k), []).append(float(s))
+        elif line.startswith("Our summed intensity"):
+            band = line
+    return {k: np.array(v) for k, v in blocks.items()}, band, r.stderr
+
+
+def broadeners(J):
+    """a0 per lower J and a1 for a few (J'', J') pairs; the reader's lookup."""
+    a0 = {j: (0.03 + 0.001 * j, 0.5 + 0.004 * j) for j in range(0, 61, 1) if j % 3 != 2}
+    a1 = {(j, j + 1): (0.09, 0.7) for j in range(0, 60, 5)}
+    return a0, a1
+
+
+def gamma_n(L, a0, a1, dg=0.07, dn=0.5):
+    Jl = L.J[L.lower.astype(np.int64)]
+    Ju = L.J[L.upper.astype(np.int64)]
+    g = np.full(L.n_trans, dg)
+    n = np.full(L.n_trans, dn)
+    for i, (jl, ju) in enumerate(zip(Jl, Ju)):
+        if (jl, ju) in a1:
+            g[i], n[i] = a1[(jl, ju)]
+        elif jl in a0:
+            g[i], n[i] = a0[jl]
+    return g, n
+
+
+def test_exomol_driver_vs_oracle(O, tmp_path):
+    L = synth.workload_list("voigt_1e6", scale=0.01)
+    a0, a1 = broadeners(L.J)
+    with open(tmp_path / "m.states", "w") as f:
+        for i, (e, g, j) in enumerate(zip(L.E, L.g, L.J)):
+            f.write(f"{i + 1:12d} {e!r} {g:6d} {j:7d}\n")
+    with open(tmp_path / "m.trans", "w") as f:
+        for u, l, a in zip(L.upper, L.lower, L.A):
+            f.write(f"{u + 1} {l + 1} {float(a)!r}\n")
+    with open(tmp_path / "m.broad", "w") as f:
+        for j, (g, n) in a0.items():
+            f.write(f"a0 {g!r} {n!r} {j}\n")
+        for (jl, ju), (g, n) in a1.items():
+            f.write(f"a1 {g!r} {n!r} {jl} {ju}\n")
+    tps = [(296.0, 1.0), (1200.0, 0.2)]
+    N = 200_000
+    cfg = "\n".join([
+        "EXOMOL", "STATE-FILE m.states", "TRANS-FILES", "m.trans", "END", "BROADENERS",
+        "m.broad", "END", "PROFILE VOIGT", f"N-POINTS {N}", "NU-RANGE 0.0 10000.0",
+        "MEAN-MASS 18.0153", "GAMMA 0.07", "GAMMA-N 0.5", "CUTOFF 25", "T-REF 296",
+        f"PRESSURE-SCALE {1 / 1.01325!r}", "CHUNK-LINES 1500", "TEMPERATURE-PRESSURE-GRID",
+        *[f"{t} {p}" for t, p in tps], "END", ""])
+    out, band, err = run_cli(cfg, tmp_path)
+    assert "selected 10000, chunks 7" in err  # 10000 selected lines / 1500 per chunk
+    nu, E, A, g, _, _ = L.lines()
+    gam, nex = gamma_n(L, a0, a1)
+    keep = (nu >= -100.0) & (nu <= 10100.0) & (nu != 0)
+    args = (nu[keep], E[keep], A[keep], g[keep], gam[keep], nex[keep])
+    for k, (T, P) in enumerate(tps):
+        Q = O.partition(L.E, L.g, T)
+        p = O.params("voigt", T=T, P=P, Q=Q, cutoff=25.0, p_scale=1 / 1.01325)
+        ref, _ = O.cross_section(p, 0.0, 10000.0 / N, N, *args, nthreads=8)
+        err = np.max(np.abs(out[k] - ref)) / ref.max()
+        bnd = abs(out[k].sum() - ref.sum()) / ref.sum()
+        print(f"T={T} P={P}: peak-rel {err:.2e} band {bnd:.2e}", flush=True)
+        assert err <= 1e-4 and bnd <= 1e-6
+    assert band is not None
+
+
+def par_record(nu, S, A, gair, gself, E, nair, gup):
+    """One 160-character HITRAN record with the reader's field widths."""
+    f = [f"{1:2d}", "1", f"{nu:12.6f}", f"{S:10.3E}", f"{A:10.3E}", f"{gair:5.4f}"[1:].rjust(5),
+         f"{gself:5.3f}".rjust(5), f"{E:10.4f}", f"{nair:4.2f}", f"{0.0:8.6f}", " " * 15,
+         " " * 15, " " * 15, " " * 15, "000000", " " * 12, " ", f"{gup:7.1f}", f"{gup:7.1f}"]
+    rec = "".join(f)
+    assert len(rec) == 160
+    return rec
+
+
+def test_hitran_driver_vs_oracle(O, tmp_path):
+    rng = np.random.default_rng(11)
+    n = 400
+    nu = np.sort(rng.uniform(1000.0, 1100.0, n))
+    E = rng.uniform(0.0, 3000.0, n)
+    A = 10 ** rng.uniform(-3, 1, n)
+    gair = rng.uniform(0.05, 0.09, n)
+    gself = rng.uniform(0.2, 0.4, n)
+    nair = rng.uniform(0.5, 0.8, n)
+    gup = rng.integers(1, 30, n).astype(float)
+    recs = [par_record(*row) for row in zip(nu, np.zeros(n), A, gair, gself, E, nair, gup)]
+    (tmp_path / "x.par").write_text("\n".join(recs) + "\n")
+    (tmp_path / "q.txt").write_text("200 150.0\n300 240.0\n1000 1300.0\n")
+    # what the records hold after fixed-width formatting
+    nu_r = np.array([float(r[3:15]) for r in recs])
+    A_r = np.array([float(r[25:35]) for r in recs])
+    ga_r = np.array([float(r[35:40]) for r in recs])
+    gs_r = np.array([float(r[40:45]) for r in recs])
+    E_r = np.array([float(r[45:55]) for r in recs])
+    n_r = np.array([float(r[55:59]) for r in recs])
+    g_r = np.array([float(r[146:153]) for r in recs]).astype(np.int32)
+    mix, T, P, N = 0.8, 500.0, 0.5, 100_000
+    cfg = "\n".join([
+        "HITRAN", "TRANS-FILES", "x.par", "END", "PROFILE VOIGT", f"N-POINTS {N}",
+        "NU-RANGE 990.0 1110.0", "MEAN-MASS 18.0153", f"HITRAN-MIXTURE-AIR {mix}", "CUTOFF 10",
+        "PARTITION 240.0", "PARTITION-FILE q.txt", "TEMPERATURE-PRESSURE-GRID", f"{T} {P}", "END",
+        ""])
+    out, band, err = run_cli(cfg, tmp_path)
+    Q = 240.0 + (1300.0 - 240.0) * (T - 300.0) / 700.0
+    gam = ga_r * mix + gs_r * (1 - mix)
+    p = O.params("voigt", T=T, P=P, Q=Q, cutoff=10.0, p_scale=1.0)
+    ref, _ = O.cross_section(p, 990.0, 120.0 / N, N, nu_r, E_r, A_r, g_r, gam, n_r, nthreads=8)
+    e = np.max(np.abs(out[0] - ref)) / ref.max()
+    assert e <= 1e-4, e
+    assert abs(out[0].sum() - ref.sum()) / ref.sum() <= 1e-6

@@ -282,3 +282,22 @@ def test_batch_equals_manager(O):
         assert np.array_equal(batch[k], out), k
         ref, _, _ = run_oracle(O, "voigt", lines, t, p, Q[k], start, end, N, 25.0, mass=28.0)
         check(batch[k], ref, 1e-4, 1e-6, f"batch T={t} P={p}")
+
+
+def test_batch_line_selection_bit_exact(O):
+    # selection of the resident-list path = the driver's filter (BaseProfile.cpp:152-157)
+    L = synth.workload_list("voigt_tp_table_1e7", scale=0.002)
+    nu = L.lines()[0]
+    N, start, end, hw = 145_000, 500.0, 15000.0, 100.0
+    m = HipManager("voigt")
+    m.InitializeConstants(hw, 296.0, 1.0, (end - start) / N, 28.0, 1.0, 296.0, P_SCALE,
+                          cutoff=25.0)
+    m.load_list(L)
+    m.run_batch([(1000.0, 1.0)], start, end, N)
+    first, count = m.pass_lines()
+    sel = np.array([O.lib().gxo_select(v, start, end, hw) for v in nu], dtype=bool)
+    want = np.sort(nu[sel], kind="stable")
+    nus, ib, ie = m.line_windows(count)
+    assert count == sel.sum() and np.array_equal(nus, want)
+    rib, rie = O.windows(want, 25.0, start, (end - start) / N, 0, N)
+    assert np.array_equal(ib, rib) and np.array_equal(ie, rie)
