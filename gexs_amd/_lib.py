@@ -24,6 +24,7 @@ EXPORTS = [
     "gexs_transfer_freq", "gexs_transfer_lines", "gexs_transfer_lines_device", "gexs_execute",
     "gexs_transfer_results", "gexs_ready", "gexs_sync", "gexs_last_error", "gexs_last_stats",
     "gexs_timing", "gexs_line_windows", "gexs_pass_lines", "gexs_set_tuning", "gexs_load_list", "gexs_partition", "gexs_run_batch",
+    "gexs_set_launch", "gexs_autotune",
     "gexs_serve_start", "gexs_submit", "gexs_poll", "gexs_wait", "gexs_release",
     "gexs_serve_stop",
 ]
@@ -65,6 +66,12 @@ class Grid(C.Structure):
 
 class TP(C.Structure):
     _fields_ = [("T", C.c_double), ("P", C.c_double), ("Q", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("points_per_lane", C.c_int32), ("order", C.c_int32),
+                ("persistent_blocks", C.c_int32), ("reserved", C.c_int32),
+                ("kernel_ms", C.c_double)]
 
 
 class Request(C.Structure):
@@ -112,6 +119,9 @@ def load():
         "gexs_load_list": (i32, [P, C.POINTER(States), C.POINTER(Transitions)]),
         "gexs_partition": (i32, [P, P, i32, P]),
         "gexs_run_batch": (i32, [P, C.POINTER(TP), i32, C.POINTER(Grid), P]),
+        "gexs_set_launch": (i32, [P, i32, i32]),
+        "gexs_autotune": (i32, [P, C.POINTER(TP), i32, C.POINTER(Grid), C.POINTER(Tuning),
+                                C.POINTER(Tuning), i32, C.POINTER(i32)]),
         "gexs_serve_start": (i32, [P, i32, i64]),
         "gexs_submit": (i32, [P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
         "gexs_poll": (i32, [P, C.c_uint64, C.POINTER(Result)]),

@@ -14,6 +14,9 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <cstdio>
+#include <cstring>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -57,6 +60,48 @@ struct DevArr {
     }
     template <typename T>
     T* as() const { return (T*)p; }
+};
+
+// Pinned host staging for small uploads (parameters, job tables): a ring of
+// slots, each reused only after the copy that last read it has completed.
+struct Staging {
+    struct Slot {
+        void* p = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    std::vector<Slot> ring;
+    size_t next = 0;
+    ~Staging() {
+        for (auto& s : ring) {
+            if (s.p) (void)hipHostFree(s.p);
+            if (s.done) (void)hipEventDestroy(s.done);
+        }
+    }
+    // copies src to dst on stream through a pinned slot
+    hipError_t upload(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+        if (bytes == 0) return hipSuccess;
+        if (ring.empty()) ring.resize(64);
+        Slot& sl = ring[next];
+        next = (next + 1) % ring.size();
+        hipError_t e;
+        if (sl.pending && (e = hipEventSynchronize(sl.done)) != hipSuccess) return e;
+        if (sl.bytes < bytes) {
+            if (sl.p) (void)hipHostFree(sl.p);
+            sl.p = nullptr;
+            size_t want = std::max<size_t>(bytes, 4096);
+            if ((e = hipHostMalloc(&sl.p, want, hipHostMallocDefault)) != hipSuccess) return e;
+            sl.bytes = want;
+        }
+        if (!sl.done && (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
+            return e;
+        std::memcpy(sl.p, src, bytes);
+        if ((e = hipMemcpyAsync(dst, sl.p, bytes, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+        if ((e = hipEventRecord(sl.done, stream)) != hipSuccess) return e;
+        sl.pending = true;
+        return hipSuccess;
+    }
 };
 
 struct PosKey {
@@ -118,12 +163,23 @@ struct gexs_ctx {
     gexs_stats last{};
     bool last_neval_pending = false;
 
-    // batch graph cache
+    // batched pass: pinned tables, cached selection and captured graph
+    hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;
+    bool use_graph = false;
+    void* h_tp = nullptr;
+    size_t h_tp_cap = 0;
+    void* h_tab = nullptr;
+    size_t h_tab_cap = 0;
+    double sel_lo = NAN, sel_hi = NAN;
+    uint64_t sel_gen = 0;
+    int64_t sel_l0 = 0, sel_l1 = 0;
     DevArr batch_out;
 
     std::unique_ptr<gxs::Server> server;
+    Staging staging;
+    DevArr pos_jd, pos_tp, jd;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -233,37 +289,50 @@ int sort_and_commit(gexs_ctx* c, int64_t n, bool has_gamma, bool drop_zero) {
     return GEXS_OK;
 }
 
-// window + nearest point of every current line for one grid window
+// window + nearest point of every current line for one grid window:
 // lines [l0, l0 + n) of the current lines take part in the pass
-int ensure_pos(gexs_ctx* c, double cutoff, double start, double dfreq, int win_lo, int win_hi,
-               int64_t l0, int64_t n) {
+int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
     PosKey key;
-    key.cutoff = cutoff;
-    key.start = start;
-    key.dfreq = dfreq;
-    key.win_lo = win_lo;
-    key.win_hi = win_hi;
+    key.cutoff = tp.cutoff;
+    key.start = tp.start;
+    key.dfreq = tp.dfreq;
+    key.win_lo = tp.win_lo;
+    key.win_hi = tp.win_hi;
     key.l0 = l0;
     key.n = n;
     key.gen = c->gen;
     if (key == c->pos_key) return GEXS_OK;
     HIPCHK(c, c->pos.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(LinePos)));
-    HIPCHK(c, gxk::line_pos(c->stream, c->nu.as<double>() + l0, (int)n, cutoff, start, dfreq, win_lo,
-                            win_hi, c->pos.as<LinePos>()));
+    JobDesc d{};
+    d.l0 = l0;
+    d.n = (int32_t)n;
+    d.q_slot = -1;
+    HIPCHK(c, c->pos_jd.ensure(sizeof(JobDesc)));
+    HIPCHK(c, c->pos_tp.ensure(sizeof(TPParams)));
+    HIPCHK(c, c->staging.upload(c->pos_jd.p, &d, sizeof d, c->stream));
+    HIPCHK(c, c->staging.upload(c->pos_tp.p, &tp, sizeof tp, c->stream));
+    HIPCHK(c, gxk::line_pos(c->stream, c->nu.as<double>(), c->pos_jd.as<JobDesc>(), 1, n,
+                            c->pos_tp.as<TPParams>(), c->pos.as<LinePos>()));
     c->pos_key = key;
     c->items_key = PosKey();  // tiles depend on positions
     return GEXS_OK;
 }
 
-// work items for njobs jobs that share one grid window; sorted heaviest first
+int tiles_per_job(const gexs_ctx* c) {
+    const int span = GX_WAVE * c->ppl;
+    const int w = c->pos_key.win_hi - c->pos_key.win_lo;
+    return w > 0 ? (w + span - 1) / span : 0;
+}
+
+// work items for the njobs jobs of c->jd / c->tp (one grid window); sorted
+// heaviest first when order == 1
 int ensure_items(gexs_ctx* c, int njobs) {
     const int span = GX_WAVE * c->ppl;
     const PosKey key = c->pos_key;
     if (key == c->items_key && c->items_span == span && c->items_order == c->order &&
         c->items_njobs == njobs)
         return GEXS_OK;
-    const int win_lo = key.win_lo, win_hi = key.win_hi;
-    const int per = win_hi > win_lo ? (win_hi - win_lo + span - 1) / span : 0;
+    const int per = tiles_per_job(c);
     const int total = per * njobs;
     HIPCHK(c, c->items.ensure((size_t)std::max(total, 1) * sizeof(WorkItem)));
     HIPCHK(c, c->items_tmp.ensure((size_t)std::max(total, 1) * sizeof(WorkItem)));
@@ -271,12 +340,9 @@ int ensure_items(gexs_ctx* c, int njobs) {
     HIPCHK(c, c->item_idx.ensure((size_t)std::max(total, 1) * 4));
     const bool sorted = c->order == 1 && total > 1;
     WorkItem* dst = sorted ? c->items_tmp.as<WorkItem>() : c->items.as<WorkItem>();
-    for (int j = 0; j < njobs; ++j) {
-        int nt = 0;
-        HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), (int)key.n, win_lo, win_hi, span, j, dst,
-                             sorted ? c->cost_key.as<uint32_t>() : nullptr,
-                             sorted ? c->item_idx.as<uint32_t>() : nullptr, j * per, &nt));
-    }
+    HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, per,
+                         c->tp.as<TPParams>(), span, dst, sorted ? c->cost_key.as<uint32_t>() : nullptr,
+                         sorted ? c->item_idx.as<uint32_t>() : nullptr));
     if (sorted) {
         HIPCHK(c, gxk::radix_sort_u32(c->stream, c->cost_key.as<uint32_t>(), c->item_idx.as<uint32_t>(),
                                       total, c->sb, 0, 32));
@@ -308,13 +374,12 @@ int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
 
 int upload_tp(gexs_ctx* c, const std::vector<TPParams>& tps) {
     HIPCHK(c, c->tp.ensure(tps.size() * sizeof(TPParams)));
-    HIPCHK(c, hipMemcpyAsync(c->tp.p, tps.data(), tps.size() * sizeof(TPParams), hipMemcpyHostToDevice,
-                             c->stream));
+    HIPCHK(c, c->staging.upload(c->tp.p, tps.data(), tps.size() * sizeof(TPParams), c->stream));
     return GEXS_OK;
 }
 
-// The FP32C profile pass for njobs (T,P) over the current lines and one
-// window; outs[j] is the grid of job j (points indexed from 0).
+// The FP32C profile pass for njobs (T,P) over the lines and window of
+// c->pos_key; outs[j] is the grid of job j (points indexed from 0).
 int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<double*>& outs,
               bool timed) {
     const int njobs = (int)tps.size();
@@ -323,6 +388,19 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     hipStream_t s = c->stream;
     int rc;
     if ((rc = upload_tp(c, tps)) != GEXS_OK) return rc;
+    const int per = tiles_per_job(c);
+    std::vector<JobDesc> jds((size_t)njobs);
+    for (int j = 0; j < njobs; ++j) {
+        jds[j].l0 = l0;
+        jds[j].pos_off = 0;
+        jds[j].rec_off = (int64_t)j * n;
+        jds[j].n = n;
+        jds[j].items_off = j * per;
+        jds[j].ntiles = per;
+        jds[j].q_slot = -1;
+    }
+    HIPCHK(c, c->jd.ensure(jds.size() * sizeof(JobDesc)));
+    HIPCHK(c, c->staging.upload(c->jd.p, jds.data(), jds.size() * sizeof(JobDesc), s));
     HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRec)));
     HIPCHK(c, c->scale.ensure((size_t)njobs * 8));
     HIPCHK(c, c->neval.ensure((size_t)njobs * 8));
@@ -330,17 +408,14 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
     HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
     HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 1, 0ull));
-    const double* gam = c->has_gamma ? c->gamma.as<double>() + l0 : nullptr;
-    const double* nex = c->has_gamma ? c->nexp.as<double>() + l0 : nullptr;
-    const double* nu = c->nu.as<double>() + l0;
-    const double* E = c->E.as<double>() + l0;
-    const double* A = c->A.as<double>() + l0;
-    const int32_t* g = c->g.as<int32_t>() + l0;
-    HIPCHK(c, gxk::line_max(s, nu, E, A, g, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(), njobs,
+    const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
+    const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n, c->tp.as<TPParams>(),
                             c->scale.as<unsigned long long>()));
-    HIPCHK(c, gxk::line_rec(s, nu, E, A, g,
-                            gam, nex, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(), njobs,
-                            c->scale.as<unsigned long long>(), c->rec.as<VRec>(), n,
+    HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            gam, nex, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n,
+                            c->tp.as<TPParams>(), c->scale.as<unsigned long long>(), c->rec.as<VRec>(),
                             c->neval.as<unsigned long long>()));
     std::vector<Job> jobs((size_t)njobs);
     for (int j = 0; j < njobs; ++j) {
@@ -350,10 +425,10 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
         jobs[j].n = n;
         jobs[j].profile = c->profile;
         jobs[j].scale_slot = j;
-        jobs[j].pad = 0;
+        jobs[j].accumulate = 1;
     }
     HIPCHK(c, c->jobs.ensure(jobs.size() * sizeof(Job)));
-    HIPCHK(c, hipMemcpyAsync(c->jobs.p, jobs.data(), jobs.size() * sizeof(Job), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->staging.upload(c->jobs.p, jobs.data(), jobs.size() * sizeof(Job), s));
     if ((rc = ensure_items(c, njobs)) != GEXS_OK) return rc;
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->items.as<WorkItem>(), c->n_items,
@@ -439,6 +514,9 @@ int gexs_destroy(gexs_ctx* c) {
     c->results.clear();
     gxk::sort_buffers_free(c->sb);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->h_tp) (void)hipHostFree(c->h_tp);
+    if (c->h_tab) (void)hipHostFree(c->h_tab);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -502,6 +580,7 @@ static int transfer_common(gexs_ctx* c, int64_t n, const double* E, const double
                            const double* A, const int32_t* g, const double* gamma,
                            const double* nexp, hipMemcpyKind kind) {
     if (!c || n < 0 || (n > 0 && (!E || !nu || !A || !g))) return GEXS_EINVAL;
+    if (c->server) return c->fail(GEXS_ESTATE, "lines cannot change while serving");
     if (n > 0x7FFFFFFF) return c->fail(GEXS_EINVAL, "more than 2^31-1 lines in one chunk");
     if (c->capacity > 0 && n > c->capacity) return c->fail(GEXS_ENOMEM, "chunk exceeds line capacity");
     const bool needs_gamma = (c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD);
@@ -567,7 +646,7 @@ int gexs_execute(gexs_ctx* c, int npoints, int64_t nlines, int start_idx, double
     const double dfreq = c->k.dfreq;
     const int win_lo = start_idx, win_hi = start_idx + npoints;
     TPParams tp = make_tp(c, T, P, Q, c->start, dfreq, win_lo, win_hi);
-    if ((rc = ensure_pos(c, tp.cutoff, c->start, dfreq, win_lo, win_hi, 0, c->n)) != GEXS_OK) return rc;
+    if ((rc = ensure_pos(c, tp, 0, c->n)) != GEXS_OK) return rc;
     if ((rc = next_events(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
     if (c->k.precision == GEXS_FP64)
@@ -693,6 +772,7 @@ int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order) {
 int gexs_load_list(gexs_ctx* c, const gexs_states* st, const gexs_transitions* tr) {
     if (!c || !st || !tr || st->n <= 0 || tr->n < 0 || !st->E || !st->g) return GEXS_EINVAL;
     if (tr->n > 0 && (!tr->upper || !tr->lower || !tr->A)) return GEXS_EINVAL;
+    if (c->server) return c->fail(GEXS_ESTATE, "the list cannot change while serving");
     if (tr->n > 0x7FFFFFFF) return c->fail(GEXS_EINVAL, "more than 2^31-1 transitions");
     if (st->n > 0xFFFFFFFFll) return c->fail(GEXS_EINVAL, "more than 2^32 states");
     const bool has_gamma = tr->gamma != nullptr && tr->nexp != nullptr;
@@ -763,27 +843,69 @@ int gexs_partition(gexs_ctx* c, const double* T, int nT, double* Q_out) {
     return GEXS_OK;
 }
 
+static int pinned(gexs_ctx* c, void** p, size_t* cap, size_t want) {
+    if (*p && *cap >= want) return GEXS_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    want = std::max<size_t>(want, 4096);
+    HIPCHK(c, hipHostMalloc(p, want, hipHostMallocDefault));
+    *cap = want;
+    return GEXS_OK;
+}
+
+// Device work of one FP32C batch.  Only kernels and copies from the ctx's
+// pinned tables, so it can be captured into a graph: the (T,P) table is read
+// from c->h_tp when the copy executes, the rest of the tables are fixed for a
+// given plan.
+static int enqueue_batch(gexs_ctx* c, int njobs, int nq, size_t jd_b, size_t jobs_b) {
+    hipStream_t s = c->stream;
+    const int n = (int)c->pos_key.n;
+    HIPCHK(c, hipMemcpyAsync(c->tp.p, c->h_tp, (size_t)njobs * sizeof(TPParams), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->jd.p, c->h_tab, (size_t)njobs * sizeof(JobDesc), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->jobs.p, (char*)c->h_tab + jd_b, (size_t)njobs * sizeof(Job),
+                             hipMemcpyHostToDevice, s));
+    if (nq > 0) {
+        const int nblk = 256;
+        HIPCHK(c, hipMemcpyAsync(c->q_T.p, (char*)c->h_tab + jd_b + jobs_b, (size_t)nq * 8,
+                                 hipMemcpyHostToDevice, s));
+        HIPCHK(c, gxk::partition(s, c->st_E.as<double>(), c->st_g.as<int32_t>(), c->n_states, c->q_T.as<double>(),
+                                 nq, c->q_part.as<double>(), nblk, c->q_Q.as<double>()));
+        HIPCHK(c, gxk::take_q(s, c->jd.as<JobDesc>(), njobs, c->q_Q.as<double>(), c->tp.as<TPParams>()));
+    }
+    HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 1, 0ull));
+    const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
+    const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n, c->tp.as<TPParams>(),
+                            c->scale.as<unsigned long long>()));
+    HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            gam, nex, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n,
+                            c->tp.as<TPParams>(), c->scale.as<unsigned long long>(), c->rec.as<VRec>(),
+                            c->neval.as<unsigned long long>()));
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->items.as<WorkItem>(), c->n_items,
+                                 c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
+                                 c->counter.as<unsigned int>(), c->persistent_blocks));
+    return GEXS_OK;
+}
+
 int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid, double* out) {
     if (!c || !tp || ntp <= 0 || !grid || !out || grid->npoints <= 0 || !(grid->end > grid->start))
         return GEXS_EINVAL;
     if (!c->have_k) return c->fail(GEXS_ESTATE, "run_batch before init_constants");
+    if (c->server) return c->fail(GEXS_ESTATE, "run_batch while serving");
     int rc;
     if ((rc = set_device(c)) != GEXS_OK) return rc;
     const double dfreq = (grid->end - grid->start) / (double)grid->npoints;  // BaseProfile.cpp:10
-    std::vector<double> Qs((size_t)ntp);
-    std::vector<double> needT;
+    int nq = 0;
     for (int i = 0; i < ntp; ++i) {
-        if (!(tp[i].T > 0.0)) return c->fail(GEXS_EINVAL, "T must be positive");
-        Qs[i] = tp[i].Q;
-        if (!(tp[i].Q > 0.0)) needT.push_back(tp[i].T);
+        if (!(tp[i].T > 0.0) || !(tp[i].P >= 0.0)) return c->fail(GEXS_EINVAL, "T must be positive, P >= 0");
+        if (!(tp[i].Q > 0.0)) nq++;
     }
-    if (!needT.empty()) {
-        std::vector<double> q(needT.size());
-        if ((rc = gexs_partition(c, needT.data(), (int)needT.size(), q.data())) != GEXS_OK) return rc;
-        size_t w = 0;
-        for (int i = 0; i < ntp; ++i)
-            if (!(tp[i].Q > 0.0)) Qs[i] = q[w++];
-    }
+    if (nq > 0 && c->n_states <= 0)
+        return c->fail(GEXS_ESTATE, "Q(T) on the device needs a resident state table (gexs_load_list)");
     // one cutoff for the whole batch (positions and tiles are shared)
     const double cutoff = effective_cutoff(c->k, tp[0].P);
     for (int i = 1; i < ntp; ++i)
@@ -791,34 +913,138 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
             return c->fail(GEXS_EINVAL, "a batch needs one cutoff (set gexs_constants.cutoff > 0)");
     // line selection (Profiles/BaseProfile.cpp:152-157 as a filter on both sides):
     // start - HW <= nu <= end + HW over the nu-sorted list (nu == 0 was dropped at load)
-    int64_t l0 = 0, l1 = c->n;
-    if ((rc = select_range(c, grid->start - c->k.half_width, grid->end + c->k.half_width, &l0, &l1)) != GEXS_OK)
-        return rc;
-    if ((rc = ensure_pos(c, cutoff, grid->start, dfreq, 0, grid->npoints, l0, l1 - l0)) != GEXS_OK) return rc;
-    const size_t np = (size_t)grid->npoints;
-    HIPCHK(c, c->batch_out.ensure(np * (size_t)ntp * 8));
-    std::vector<TPParams> tps((size_t)ntp);
-    std::vector<double*> outs((size_t)ntp);
-    for (int i = 0; i < ntp; ++i) {
-        tps[i] = make_tp(c, tp[i].T, tp[i].P, Qs[i], grid->start, dfreq, 0, grid->npoints);
-        outs[i] = c->batch_out.as<double>() + (size_t)i * np;
+    const double slo = grid->start - c->k.half_width, shi = grid->end + c->k.half_width;
+    if (!(c->sel_lo == slo && c->sel_hi == shi && c->sel_gen == c->gen)) {
+        if ((rc = select_range(c, slo, shi, &c->sel_l0, &c->sel_l1)) != GEXS_OK) return rc;
+        c->sel_lo = slo;
+        c->sel_hi = shi;
+        c->sel_gen = c->gen;
     }
-    HIPCHK(c, gxk::fill_f64(c->stream, c->batch_out.as<double>(), (int64_t)np * ntp, 0.0));
+    const int64_t l0 = c->sel_l0, l1 = c->sel_l1;
+    {
+        TPParams ptp = make_tp(c, tp[0].T, tp[0].P, 1.0, grid->start, dfreq, 0, grid->npoints);
+        if ((rc = ensure_pos(c, ptp, l0, l1 - l0)) != GEXS_OK) return rc;
+    }
+    const size_t np = (size_t)grid->npoints;
+    const int n = (int)c->pos_key.n;
+    HIPCHK(c, c->batch_out.ensure(np * (size_t)ntp * 8));
+    c->last = gexs_stats{};
+    c->last.n_lines = n;
+    if (c->k.precision == GEXS_FP64) {
+        std::vector<double> Qs((size_t)ntp);
+        std::vector<double> needT;
+        for (int i = 0; i < ntp; ++i)
+            if (!(tp[i].Q > 0.0)) needT.push_back(tp[i].T);
+        std::vector<double> q(needT.size());
+        if (!needT.empty() && (rc = gexs_partition(c, needT.data(), (int)needT.size(), q.data())) != GEXS_OK)
+            return rc;
+        for (int i = 0, w = 0; i < ntp; ++i) Qs[i] = tp[i].Q > 0.0 ? tp[i].Q : q[(size_t)w++];
+        HIPCHK(c, gxk::fill_f64(c->stream, c->batch_out.as<double>(), (int64_t)np * ntp, 0.0));
+        if ((rc = next_events(c)) != GEXS_OK) return rc;
+        HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
+        for (int i = 0; i < ntp; ++i) {
+            TPParams t = make_tp(c, tp[i].T, tp[i].P, Qs[i], grid->start, dfreq, 0, grid->npoints);
+            if ((rc = run_fp64(c, t, c->batch_out.as<double>() + (size_t)i * np, i == 0)) != GEXS_OK) return rc;
+        }
+        HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        unsigned long long ne = 0;
+        HIPCHK(c, hipMemcpy(&ne, c->neval.p, 8, hipMemcpyDeviceToHost));
+        c->last.n_eval = (int64_t)ne * ntp;  // the per-job count is the same for every job
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) == hipSuccess) c->last.kernel_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev_start, c->ev_k1) == hipSuccess) c->last.total_ms = ms;
+        c->last_neval_pending = false;
+        return GEXS_OK;
+    }
+    // ---- FP32C: host tables in pinned memory
+    const int per = tiles_per_job(c);
+    const size_t jd_b = ((size_t)ntp * sizeof(JobDesc) + 255) / 256 * 256;
+    const size_t jobs_b = ((size_t)ntp * sizeof(Job) + 255) / 256 * 256;
+    if ((rc = pinned(c, &c->h_tp, &c->h_tp_cap, (size_t)ntp * sizeof(TPParams))) != GEXS_OK) return rc;
+    if ((rc = pinned(c, &c->h_tab, &c->h_tab_cap, jd_b + jobs_b + (size_t)ntp * 8)) != GEXS_OK) return rc;
+    TPParams* htp = (TPParams*)c->h_tp;
+    JobDesc* hjd = (JobDesc*)c->h_tab;
+    Job* hjobs = (Job*)((char*)c->h_tab + jd_b);
+    double* hT = (double*)((char*)c->h_tab + jd_b + jobs_b);
+    HIPCHK(c, c->tp.ensure((size_t)ntp * sizeof(TPParams)));
+    HIPCHK(c, c->jd.ensure((size_t)ntp * sizeof(JobDesc)));
+    HIPCHK(c, c->jobs.ensure((size_t)ntp * sizeof(Job)));
+    HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * ntp * sizeof(VRec)));
+    HIPCHK(c, c->scale.ensure((size_t)ntp * 8));
+    HIPCHK(c, c->neval.ensure((size_t)ntp * 8));
+    HIPCHK(c, c->counter.ensure(256));
+    HIPCHK(c, c->q_T.ensure((size_t)ntp * 8));
+    HIPCHK(c, c->q_part.ensure((size_t)ntp * 256 * 8));
+    HIPCHK(c, c->q_Q.ensure((size_t)ntp * 8));
+    // previous graph launches have completed (run_batch is synchronous), so
+    // the pinned tables may be rewritten
+    int qs = 0;
+    for (int j = 0; j < ntp; ++j) {
+        htp[j] = make_tp(c, tp[j].T, tp[j].P, tp[j].Q > 0.0 ? tp[j].Q : 1.0, grid->start, dfreq, 0, grid->npoints);
+        JobDesc d{};
+        d.l0 = l0;
+        d.pos_off = 0;
+        d.rec_off = (int64_t)j * n;
+        d.n = n;
+        d.items_off = j * per;
+        d.ntiles = per;
+        d.q_slot = tp[j].Q > 0.0 ? -1 : qs;
+        if (d.q_slot >= 0) hT[qs++] = tp[j].T;
+        hjd[j] = d;
+        Job jb{};
+        jb.pos = c->pos.as<LinePos>();
+        jb.rec = c->rec.as<VRec>() + (size_t)j * n;
+        jb.out = c->batch_out.as<double>() + (size_t)j * np;
+        jb.n = n;
+        jb.profile = c->profile;
+        jb.scale_slot = j;
+        jb.accumulate = 0;  // every point of every grid is written once
+        hjobs[j] = jb;
+    }
+    // the tiles kernel reads window and job tables from the device copies
+    HIPCHK(c, hipMemcpyAsync(c->tp.p, c->h_tp, (size_t)ntp * sizeof(TPParams), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->jd.p, c->h_tab, (size_t)ntp * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
+    if ((rc = ensure_items(c, ntp)) != GEXS_OK) return rc;
     if ((rc = next_events(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
-    c->last = gexs_stats{};
-    c->last.n_lines = c->n;
-    if (c->k.precision == GEXS_FP64) {
-        for (int i = 0; i < ntp; ++i)
-            if ((rc = run_fp64(c, tps[i], outs[i], i == 0)) != GEXS_OK) return rc;
-        HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
+    if (c->use_graph) {
+        char key[256];
+        std::snprintf(key, sizeof key, "%d|%d|%llu|%d|%d|%d|%p|%p|%p|%lld|%lld|%d|%d", ntp, qs,
+                      (unsigned long long)c->gen, c->ppl, c->order, c->persistent_blocks, c->rec.p,
+                      c->batch_out.p, c->items.p, (long long)l0, (long long)n, grid->npoints, per);
+        const std::string k2 = std::string(key) + "|" + std::to_string(grid->start) + "|" +
+                               std::to_string(grid->end) + "|" + std::to_string(cutoff);
+        if (!c->graph_exec || c->graph_key != k2) {
+            if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+            if (c->graph) (void)hipGraphDestroy(c->graph);
+            c->graph_exec = nullptr;
+            c->graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_batch(c, ntp, qs, jd_b, jobs_b);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc != GEXS_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return c->hipfail(e, "hipStreamEndCapture");
+            c->graph = g;
+            HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+            c->graph_key = k2;
+        }
+        HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
+        HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     } else {
-        if ((rc = run_fp32c(c, tps, outs, true)) != GEXS_OK) return rc;
+        HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
+        if ((rc = enqueue_batch(c, ntp, qs, jd_b, jobs_b)) != GEXS_OK) return rc;
     }
+    HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
+    c->last.n_tiles = c->n_items;
     HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    // N_eval summed over the batch
-    std::vector<unsigned long long> ne((size_t)(c->k.precision == GEXS_FP64 ? 1 : ntp), 0ull);
+    std::vector<unsigned long long> ne((size_t)ntp, 0ull);
     HIPCHK(c, hipMemcpy(ne.data(), c->neval.p, ne.size() * 8, hipMemcpyDeviceToHost));
     int64_t tot = 0;
     for (auto v : ne) tot += (int64_t)v;
@@ -830,32 +1056,156 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     return GEXS_OK;
 }
 
+int gexs_set_launch(gexs_ctx* c, int persistent_blocks, int use_graph) {
+    if (!c || persistent_blocks < 0) return GEXS_EINVAL;
+    c->persistent_blocks = persistent_blocks;
+    c->use_graph = use_graph != 0;
+    return GEXS_OK;
+}
+
+int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid, gexs_tuning* best,
+                  gexs_tuning* report, int report_cap, int* n_report) {
+    if (!c || !tp || ntp <= 0 || !grid || grid->npoints <= 0) return GEXS_EINVAL;
+    if (c->k.precision != GEXS_FP32C) return c->fail(GEXS_ESTATE, "autotune applies to the FP32C path");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    const int cus = std::max(1, prop.multiProcessorCount);
+    DevArr tmp;
+    HIPCHK(c, tmp.ensure((size_t)grid->npoints * ntp * 8));
+    const int ppls[4] = {1, 2, 4, 8};
+    const int pers[3] = {0, 4 * cus, 8 * cus};
+    const bool graph = c->use_graph;
+    c->use_graph = false;
+    gexs_tuning top{};
+    top.kernel_ms = 1e300;
+    int nrep = 0;
+    for (int pi = 0; pi < 4; ++pi)
+        for (int order = 0; order < 2; ++order)
+            for (int pb = 0; pb < 3; ++pb) {
+                c->ppl = ppls[pi];
+                c->order = order;
+                c->persistent_blocks = pers[pb];
+                double ms = 0.0;
+                for (int rep = 0; rep < 2; ++rep) {  // first run builds positions and tiles
+                    if ((rc = gexs_run_batch(c, tp, ntp, grid, tmp.as<double>())) != GEXS_OK) {
+                        c->use_graph = graph;
+                        return rc;
+                    }
+                    ms = c->last.kernel_ms;
+                }
+                gexs_tuning t{};
+                t.points_per_lane = ppls[pi];
+                t.order = order;
+                t.persistent_blocks = pers[pb];
+                t.kernel_ms = ms;
+                if (report && nrep < report_cap) report[nrep] = t;
+                nrep++;
+                if (ms < top.kernel_ms) top = t;
+            }
+    c->ppl = top.points_per_lane;
+    c->order = top.order;
+    c->persistent_blocks = top.persistent_blocks;
+    c->use_graph = graph;
+    if (best) *best = top;
+    if (n_report) *n_report = nrep;
+    return GEXS_OK;
+}
+
 // --------------------------------------------------------------- serving
 int gexs_serve_start(gexs_ctx* c, int max_inflight, int64_t mem_budget) {
-    (void)max_inflight;
-    (void)mem_budget;
-    if (!c) return GEXS_EINVAL;
-    return c->fail(GEXS_ESTATE, "serving is not available in this build");
+    if (!c || max_inflight <= 0 || mem_budget <= 0) return GEXS_EINVAL;
+    if (c->server) return c->fail(GEXS_ESTATE, "server already running");
+    if (!c->have_k) return c->fail(GEXS_ESTATE, "serve_start before init_constants");
+    if (c->n_states <= 0) return c->fail(GEXS_ESTATE, "serving needs a resident list (gexs_load_list)");
+    if ((c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD) && !c->has_gamma)
+        return c->fail(GEXS_ESTATE, "Voigt serving needs gamma and n");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    gxs::ListView v;
+    v.nu = c->nu.as<double>();
+    v.E = c->E.as<double>();
+    v.A = c->A.as<double>();
+    v.g = c->g.as<int32_t>();
+    v.gamma = c->has_gamma ? c->gamma.as<double>() : nullptr;
+    v.nexp = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    v.n = c->n;
+    v.sE = c->st_E.as<double>();
+    v.sg = c->st_g.as<int32_t>();
+    v.n_states = c->n_states;
+    v.nu_host.resize((size_t)c->n);
+    if (c->n > 0) HIPCHK(c, hipMemcpy(v.nu_host.data(), c->nu.p, (size_t)c->n * 8, hipMemcpyDeviceToHost));
+    gxs::Physics ph;
+    ph.profile = c->profile;
+    ph.mass = c->k.mean_mass;
+    ph.t_ref = c->k.ref_temp;
+    ph.p_scale = c->k.ref_press;
+    ph.cutoff = c->k.cutoff;
+    ph.half_width = c->k.half_width;
+    ph.gauss_hwhm = c->k.gauss_hwhm;
+    ph.width_conv = c->k.width_conv;
+    ph.pressure_mode = c->k.pressure_mode;
+    ph.ppl = c->ppl;
+    std::unique_ptr<gxs::Server> srv(new (std::nothrow) gxs::Server(c->device, std::move(v), ph, max_inflight,
+                                                                    mem_budget));
+    if (!srv) return GEXS_ENOMEM;
+    std::string err;
+    if ((rc = srv->start(&err)) != 0) return c->fail(rc, err);
+    c->server = std::move(srv);
+    return GEXS_OK;
 }
-int gexs_submit(gexs_ctx* c, const gexs_request*, uint64_t*) {
-    if (!c) return GEXS_EINVAL;
-    return c->fail(GEXS_ESTATE, "serving not started");
+
+int gexs_submit(gexs_ctx* c, const gexs_request* req, uint64_t* id) {
+    if (!c || !req || !id) return GEXS_EINVAL;
+    if (!c->server) return c->fail(GEXS_ESTATE, "serving not started");
+    gxs::Request r{req->T, req->P, req->start, req->end, req->dfreq};
+    std::string err;
+    int rc = c->server->submit(r, id, &err);
+    if (rc != 0) return c->fail(rc, err);
+    return GEXS_OK;
 }
-int gexs_poll(gexs_ctx* c, uint64_t, gexs_result*) {
-    if (!c) return GEXS_EINVAL;
-    return c->fail(GEXS_ESTATE, "serving not started");
+
+static int fill_result(gexs_ctx* c, int rc, gxs::Result* r, gexs_result* out) {
+    if (rc == -6) return c->fail(GEXS_ENOTFOUND, "unknown request id");
+    if (rc != 0) return c->fail(rc, "request lookup failed");
+    out->status = r->status;
+    out->npoints = r->status == 0 ? r->npoints : 0;
+    out->sigma = r->status == 0 ? r->sigma.data() : nullptr;
+    out->latency_ms = r->latency_ms;
+    out->device_ms = r->device_ms;
+    return GEXS_OK;
 }
-int gexs_wait(gexs_ctx* c, uint64_t, gexs_result*) {
-    if (!c) return GEXS_EINVAL;
-    return c->fail(GEXS_ESTATE, "serving not started");
+
+int gexs_poll(gexs_ctx* c, uint64_t id, gexs_result* out) {
+    if (!c || !out) return GEXS_EINVAL;
+    if (!c->server) return c->fail(GEXS_ESTATE, "serving not started");
+    gxs::Result* r = nullptr;
+    int rc = c->server->poll(id, &r);
+    return fill_result(c, rc, r, out);
 }
-int gexs_release(gexs_ctx* c, uint64_t) {
-    if (!c) return GEXS_EINVAL;
-    return c->fail(GEXS_ESTATE, "serving not started");
+
+int gexs_wait(gexs_ctx* c, uint64_t id, gexs_result* out) {
+    if (!c || !out) return GEXS_EINVAL;
+    if (!c->server) return c->fail(GEXS_ESTATE, "serving not started");
+    gxs::Result* r = nullptr;
+    int rc = c->server->wait(id, &r);
+    return fill_result(c, rc, r, out);
 }
+
+int gexs_release(gexs_ctx* c, uint64_t id) {
+    if (!c) return GEXS_EINVAL;
+    if (!c->server) return c->fail(GEXS_ESTATE, "serving not started");
+    int rc = c->server->release(id);
+    if (rc == -6) return c->fail(GEXS_ENOTFOUND, "unknown request id");
+    if (rc != 0) return c->fail(GEXS_ESTATE, "request still running");
+    return GEXS_OK;
+}
+
 int gexs_serve_stop(gexs_ctx* c) {
     if (!c) return GEXS_EINVAL;
-    c->server.reset();
+    c->server.reset();  // drains in-flight batches and joins the scheduler
     return GEXS_OK;
 }
 

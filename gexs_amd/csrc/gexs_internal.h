@@ -67,7 +67,18 @@ struct __attribute__((aligned(16))) Job {
     int32_t n;
     int32_t profile;
     int32_t scale_slot;  // index into the scale array (uint64 bit pattern of max amplitude)
-    int32_t pad;
+    int32_t accumulate;  // 1: out += tile sums (chunks accumulate); 0: out = tile sums
+};
+
+// where the data of one cross-section (job) of a pass live
+struct JobDesc {
+    int64_t l0;         // first line of the job among the current (nu-sorted) lines
+    int64_t pos_off;    // first LinePos of the job
+    int64_t rec_off;    // first VRec of the job
+    int32_t n;          // number of lines
+    int32_t items_off;  // first work item
+    int32_t ntiles;     // number of work items (wave-tiles)
+    int32_t q_slot;     // >= 0: Q(T) is taken from this slot of a device partition pass
 };
 
 // per-(T,P) physics of one job, FP64

@@ -47,8 +47,10 @@ ExomolStateReader::ExomolStateReader(const std::string& states_file,
         if (q == p) continue;    // blank line
         p = q;
         double e = std::strtod(p, &q);
+        if (q == p) throw std::runtime_error("unreadable energy in " + states_file + ": " + buf);
         p = q;
         long g = std::strtol(p, &q, 10);
+        if (q == p) throw std::runtime_error("unreadable degeneracy in " + states_file + ": " + buf);
         p = q;
         long j = std::strtol(p, &q, 10);
         E_.push_back(e);
@@ -102,6 +104,7 @@ bool ExomolStateReader::ReadNextState(double& nu, double& gns, double& e_i, doub
         long idi = std::strtol(p, &q, 10);
         p = q;
         double a = std::strtod(p, &q);
+        if (q == p) throw std::runtime_error(std::string("unreadable transition record: ") + buf);
         if (idf < 1 || idi < 1 || idf > (long)E_.size() || idi > (long)E_.size())
             throw std::runtime_error("transition refers to a state outside the .states file");
         const size_t f = (size_t)(idf - 1), i = (size_t)(idi - 1);

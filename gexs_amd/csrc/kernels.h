@@ -7,19 +7,21 @@
 
 namespace gxk {
 
-hipError_t line_pos(hipStream_t s, const double* nu, int n, double cutoff, double start,
-                    double dfreq, int win_lo, int win_hi, LinePos* pos);
+// Per-job launchers: blockIdx.y runs over the njobs jobs of jd / tp; max_n
+// (max_tiles) bounds the lines (tiles) of any one job.
+hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njobs,
+                    int64_t max_n, const TPParams* tp, LinePos* pos);
 hipError_t line_max(hipStream_t s, const double* nu, const double* E, const double* A,
-                    const int* g, int n, const LinePos* pos, const TPParams* tp, int ntp,
-                    unsigned long long* scale_bits);
+                    const int* g, const LinePos* pos, const JobDesc* jd, int njobs, int64_t max_n,
+                    const TPParams* tp, unsigned long long* scale_bits);
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
-                    const int* g, const double* gamma, const double* nexp, int n,
-                    const LinePos* pos, const TPParams* tp, int ntp,
-                    const unsigned long long* scale_bits, VRec* rec, int64_t rec_stride,
-                    unsigned long long* neval);
-hipError_t tiles(hipStream_t s, const LinePos* pos, int n, int win_lo, int win_hi, int span,
-                 int job, WorkItem* items, uint32_t* cost_key, uint32_t* item_idx,
-                 int item_base, int* ntiles_out);
+                    const int* g, const double* gamma, const double* nexp, const LinePos* pos,
+                    const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
+                    const unsigned long long* scale_bits, VRec* rec, unsigned long long* neval);
+hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
+                 const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
+                 uint32_t* item_idx);
+hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, TPParams* tp);
 hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx, int n,
                          WorkItem* out);
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, const WorkItem* items, int nitems,

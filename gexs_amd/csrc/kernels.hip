@@ -80,47 +80,56 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
 }  // namespace
 
 // ------------------------------------------------------------- k_line_pos
-__global__ __launch_bounds__(256) void k_line_pos(const double* __restrict__ nu, int n,
-                                                  double cutoff, double start, double dfreq,
-                                                  int win_lo, int win_hi,
+// blockIdx.y = job; each job has its own lines, grid and window
+__global__ __launch_bounds__(256) void k_line_pos(const double* __restrict__ nu,
+                                                  const JobDesc* __restrict__ jd,
+                                                  const TPParams* __restrict__ tp,
                                                   LinePos* __restrict__ pos) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double v = nu[i] == 0.0 ? 1e-6 : nu[i];
-    // OpenMPManager.cpp:399-400: round(((nu - cut) - start)/dnu), clamped
-    double b = round((v - cutoff - start) / dfreq);
-    double e = round((v + cutoff - start) / dfreq);
-    double lo = (double)win_lo, hi = (double)win_hi;
-    double bb = (b < lo) ? lo : b;
-    double ee = (hi < e) ? hi : e;
-    double c = round((v - start) / dfreq);
-    // keep c within int range; far-away lines never reach the evaluation
-    c = fmin(fmax(c, -1.0e9), 1.0e9);
-    LinePos lp;
-    lp.ib = (int)bb;
-    lp.ie = (int)ee;
-    lp.c = (int)c;
-    lp.cf = (float)c;
-    pos[i] = lp;
+    const JobDesc d = jd[blockIdx.y];
+    const TPParams p = tp[blockIdx.y];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += gridDim.x * blockDim.x) {
+        const double x = nu[d.l0 + i];
+        const double v = x == 0.0 ? 1e-6 : x;
+        // OpenMPManager.cpp:399-400: round(((nu - cut) - start)/dnu), clamped
+        double b = round((v - p.cutoff - p.start) / p.dfreq);
+        double e = round((v + p.cutoff - p.start) / p.dfreq);
+        const double lo = (double)p.win_lo, hi = (double)p.win_hi;
+        double bb = (b < lo) ? lo : b;
+        double ee = (hi < e) ? hi : e;
+        // keep the integers in range; far-away lines have empty windows anyway
+        bb = fmin(bb, 2.0e9);
+        ee = fmax(ee, -2.0e9);
+        double c = round((v - p.start) / p.dfreq);
+        c = fmin(fmax(c, -1.0e9), 1.0e9);
+        LinePos lp;
+        lp.ib = (int)bb;
+        lp.ie = (int)ee;
+        lp.c = (int)c;
+        lp.cf = (float)c;
+        pos[d.pos_off + i] = lp;
+    }
 }
 
 // ------------------------------------------------------------- k_line_max
 __global__ __launch_bounds__(256) void k_line_max(const double* __restrict__ nu,
                                                   const double* __restrict__ E,
                                                   const double* __restrict__ A,
-                                                  const int* __restrict__ g, int n,
+                                                  const int* __restrict__ g,
                                                   const LinePos* __restrict__ pos,
-                                                  const TPParams* __restrict__ tp, int ntp,
+                                                  const JobDesc* __restrict__ jd,
+                                                  const TPParams* __restrict__ tp,
                                                   unsigned long long* __restrict__ scale_bits) {
-    int k = blockIdx.y;
+    const int k = blockIdx.y;
     const TPParams p = tp[k];
+    const JobDesc d = jd[k];
     double m = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const LinePos lp = pos[i];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += gridDim.x * blockDim.x) {
+        const LinePos lp = pos[d.pos_off + i];
         if (lp.ie <= lp.ib) continue;  // only lines that reach the grid set the scale
-        double S = line_strength(p, nu[i], E[i], A[i], g[i]);
-        double amp = line_amp(p, S, inv_width(p, nu[i]));
+        const int64_t l = d.l0 + i;
+        double S = line_strength(p, nu[l], E[l], A[l], g[l]);
+        double amp = line_amp(p, S, inv_width(p, nu[l]));
         if (isfinite(amp)) m = fmax(m, amp);
     }
     m = wave_max(m);
@@ -142,22 +151,25 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
                                                   const double* __restrict__ A,
                                                   const int* __restrict__ g,
                                                   const double* __restrict__ gamma,
-                                                  const double* __restrict__ nexp, int n,
+                                                  const double* __restrict__ nexp,
                                                   const LinePos* __restrict__ pos,
-                                                  const TPParams* __restrict__ tp, int ntp,
+                                                  const JobDesc* __restrict__ jd,
+                                                  const TPParams* __restrict__ tp,
                                                   const unsigned long long* __restrict__ scale_bits,
-                                                  VRec* __restrict__ rec, int64_t rec_stride,
+                                                  VRec* __restrict__ rec,
                                                   unsigned long long* __restrict__ neval) {
 #pragma clang fp contract(off)
-    int k = blockIdx.y;
+    const int k = blockIdx.y;
     const TPParams p = tp[k];
+    const JobDesc d = jd[k];
     const double scale = scale_of(scale_bits[k]);
     long long ne = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < d.n; ii += gridDim.x * blockDim.x) {
+        const int64_t i = d.l0 + ii;
         double v = nu[i] == 0.0 ? 1e-6 : nu[i];
         double S = line_strength(p, nu[i], E[i], A[i], g[i]);
         double iw = inv_width(p, nu[i]);
-        LinePos lp = pos[i];
+        LinePos lp = pos[d.pos_off + ii];
         if (S != 0.0 && lp.ie > lp.ib) ne += lp.ie - lp.ib;
         // offset of the line from its nearest grid point, FP64 (f_c = start + c*dfreq)
         double fc = p.start + (double)lp.c * p.dfreq;
@@ -188,7 +200,7 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
             r.k15 = 0;
         }
         r.Kg = (float)amp;
-        rec[(int64_t)k * rec_stride + i] = r;
+        rec[d.rec_off + ii] = r;
     }
     ne = wave_sum_ll(ne);
     if (neval != nullptr && (threadIdx.x & 63) == 0 && ne != 0)
@@ -198,28 +210,35 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
 // ---------------------------------------------------------------- k_tiles
 // One work item per wave-tile: lines whose window meets [w0, w1) form the
 // contiguous range [lo, hi) because ib and ie are non-decreasing in nu.
-__global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos, int n,
-                                               int win_lo, int win_hi, int span, int ntiles,
-                                               int job, WorkItem* __restrict__ items,
+// blockIdx.y = job.
+__global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
+                                               const JobDesc* __restrict__ jd,
+                                               const TPParams* __restrict__ tp, int span,
+                                               WorkItem* __restrict__ items,
                                                uint32_t* __restrict__ cost_key,
-                                               uint32_t* __restrict__ item_idx, int item_base) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    int w0 = win_lo + t * span;
-    int w1 = min(w0 + span, win_hi);
+                                               uint32_t* __restrict__ item_idx) {
+    const int job = blockIdx.y;
+    const JobDesc d = jd[job];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.ntiles) return;
+    const int win_lo = tp[job].win_lo, win_hi = tp[job].win_hi;
+    const LinePos* jp = pos + d.pos_off;
+    const int n = d.n;
+    const int w0 = win_lo + t * span;
+    const int w1 = min(w0 + span, win_hi);
     // lo = first i with ie > w0
     int a = 0, b = n;
     while (a < b) {
         int m = (a + b) >> 1;
-        if (pos[m].ie > w0) b = m; else a = m + 1;
+        if (jp[m].ie > w0) b = m; else a = m + 1;
     }
-    int lo = a;
+    const int lo = a;
     // hi = first i with ib >= w1
     a = lo;
     b = n;
     while (a < b) {
         int m = (a + b) >> 1;
-        if (pos[m].ib >= w1) b = m; else a = m + 1;
+        if (jp[m].ib >= w1) b = m; else a = m + 1;
     }
     WorkItem it;
     it.lo = lo;
@@ -228,12 +247,20 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos, 
     it.w1 = w1;
     it.job = job;
     it.pad[0] = it.pad[1] = it.pad[2] = 0;
-    items[item_base + t] = it;
+    const int slot = d.items_off + t;
+    items[slot] = it;
     if (cost_key) {
-        uint32_t c = (uint32_t)max(a - lo, 0);
-        cost_key[item_base + t] = 0xFFFFFFFFu - c;  // ascending key = heaviest first
-        item_idx[item_base + t] = (uint32_t)(item_base + t);
+        const uint32_t c = (uint32_t)max(a - lo, 0);
+        cost_key[slot] = 0xFFFFFFFFu - c;  // ascending key = heaviest first
+        item_idx[slot] = (uint32_t)slot;
     }
+}
+
+// tp[k].Q = Q[q_slot] for jobs whose Q comes from a device partition pass
+__global__ void k_take_q(const JobDesc* __restrict__ jd, int njobs, const double* __restrict__ Q,
+                         TPParams* __restrict__ tp) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < njobs && jd[k].q_slot >= 0) tp[k].Q = Q[jd[k].q_slot];
 }
 
 __global__ __launch_bounds__(256) void k_permute_items(const WorkItem* __restrict__ in,
@@ -329,7 +356,13 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, doub
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
         const int j = w0 + lane + GX_WAVE * q;
-        if (j < w1) jb.out[j] += ((double)sum[q] - (double)cmp[q]) * inv_scale;
+        const double v = ((double)sum[q] - (double)cmp[q]) * inv_scale;
+        if (j < w1) {
+            if (jb.accumulate)
+                jb.out[j] += v;
+            else
+                jb.out[j] = v;
+        }
     }
 }
 
@@ -503,44 +536,47 @@ namespace gxk {
 
 static inline int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
-hipError_t line_pos(hipStream_t s, const double* nu, int n, double cutoff, double start,
-                    double dfreq, int win_lo, int win_hi, LinePos* pos) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_line_pos, dim3(blocks_for(n, 256)), dim3(256), 0, s, nu, n, cutoff,
-                       start, dfreq, win_lo, win_hi, pos);
+hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njobs,
+                    int64_t max_n, const TPParams* tp, LinePos* pos) {
+    if (njobs <= 0 || max_n <= 0) return hipSuccess;
+    int gx = std::min(blocks_for(max_n, 256), 2048);
+    hipLaunchKernelGGL(k_line_pos, dim3(gx, njobs), dim3(256), 0, s, nu, jd, tp, pos);
     return hipGetLastError();
 }
 
 hipError_t line_max(hipStream_t s, const double* nu, const double* E, const double* A,
-                    const int* g, int n, const LinePos* pos, const TPParams* tp, int ntp,
-                    unsigned long long* scale_bits) {
-    if (n <= 0 || ntp <= 0) return hipSuccess;
-    int gx = std::min(blocks_for(n, 256), 1024);
-    hipLaunchKernelGGL(k_line_max, dim3(gx, ntp), dim3(256), 0, s, nu, E, A, g, n, pos, tp, ntp,
+                    const int* g, const LinePos* pos, const JobDesc* jd, int njobs, int64_t max_n,
+                    const TPParams* tp, unsigned long long* scale_bits) {
+    if (njobs <= 0 || max_n <= 0) return hipSuccess;
+    int gx = std::min(blocks_for(max_n, 256), 1024);
+    hipLaunchKernelGGL(k_line_max, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, pos, jd, tp,
                        scale_bits);
     return hipGetLastError();
 }
 
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
-                    const int* g, const double* gamma, const double* nexp, int n,
-                    const LinePos* pos, const TPParams* tp, int ntp,
-                    const unsigned long long* scale_bits, VRec* rec, int64_t rec_stride,
-                    unsigned long long* neval) {
-    if (n <= 0 || ntp <= 0) return hipSuccess;
-    int gx = std::min(blocks_for(n, 256), 2048);
-    hipLaunchKernelGGL(k_line_rec, dim3(gx, ntp), dim3(256), 0, s, nu, E, A, g, gamma, nexp, n,
-                       pos, tp, ntp, scale_bits, rec, rec_stride, neval);
+                    const int* g, const double* gamma, const double* nexp, const LinePos* pos,
+                    const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
+                    const unsigned long long* scale_bits, VRec* rec, unsigned long long* neval) {
+    if (njobs <= 0 || max_n <= 0) return hipSuccess;
+    int gx = std::min(blocks_for(max_n, 256), 2048);
+    hipLaunchKernelGGL(k_line_rec, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, gamma, nexp, pos,
+                       jd, tp, scale_bits, rec, neval);
     return hipGetLastError();
 }
 
-hipError_t tiles(hipStream_t s, const LinePos* pos, int n, int win_lo, int win_hi, int span,
-                 int job, WorkItem* items, uint32_t* cost_key, uint32_t* item_idx,
-                 int item_base, int* ntiles_out) {
-    int ntiles = win_hi > win_lo ? (win_hi - win_lo + span - 1) / span : 0;
-    *ntiles_out = ntiles;
-    if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, pos, n, win_lo,
-                       win_hi, span, ntiles, job, items, cost_key, item_idx, item_base);
+hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
+                 const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
+                 uint32_t* item_idx) {
+    if (njobs <= 0 || max_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(max_tiles, 256), njobs), dim3(256), 0, s, pos, jd,
+                       tp, span, items, cost_key, item_idx);
+    return hipGetLastError();
+}
+
+hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, TPParams* tp) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_take_q, dim3(blocks_for(njobs, 256)), dim3(256), 0, s, jd, njobs, Q, tp);
     return hipGetLastError();
 }
 

@@ -1,2 +1,423 @@
 // serve.hip -- continuous-batching request server (see serve.h).
 #include "serve.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "kernels.h"
+
+namespace gxs {
+
+namespace {
+constexpr size_t kAlign = 256;
+constexpr int kMaxBatch = 64;
+constexpr int kQBlocks = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+double cutoff_of(const Physics& ph, double P) {
+    return ph.cutoff > 0.0 ? ph.cutoff : std::min(25.0 * P, 100.0);  // OpenMPManager.cpp:386
+}
+
+hipError_t grow(void** p, size_t* cap, size_t want) {
+    if (*p && *cap >= want) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    want = std::max<size_t>(want, kAlign);
+    hipError_t e = hipMalloc(p, want);
+    *cap = e == hipSuccess ? want : 0;
+    return e;
+}
+}  // namespace
+
+// ------------------------------------------------------------------ Arena
+hipError_t Arena::init(size_t bytes) {
+    cap_ = bytes / kAlign * kAlign;
+    hipError_t e = hipMalloc((void**)&base_, std::max<size_t>(cap_, kAlign));
+    if (e != hipSuccess) {
+        base_ = nullptr;
+        cap_ = 0;
+        return e;
+    }
+    free_.clear();
+    free_[0] = cap_;
+    used_ = 0;
+    return hipSuccess;
+}
+
+Arena::~Arena() {
+    if (base_) (void)hipFree(base_);
+}
+
+bool Arena::alloc(size_t bytes, size_t* off) {
+    bytes = align_up(std::max<size_t>(bytes, 1));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second >= bytes) {
+            *off = it->first;
+            size_t rest = it->second - bytes;
+            size_t noff = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[noff] = rest;
+            used_ += bytes;
+            return true;
+        }
+    }
+    return false;
+}
+
+void Arena::free(size_t off, size_t bytes) {
+    bytes = align_up(std::max<size_t>(bytes, 1));
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    // coalesce with the next and the previous free block
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Server
+Server::Server(int device, ListView view, Physics phys, int max_inflight, int64_t mem_budget)
+    : device_(device), v_(std::move(view)), ph_(phys), max_inflight_(std::max(1, max_inflight)),
+      budget_(mem_budget) {}
+
+int Server::start(std::string* err) {
+    if (hipSetDevice(device_) != hipSuccess) {
+        *err = "hipSetDevice failed";
+        return -3;
+    }
+    if (arena_.init((size_t)budget_) != hipSuccess) {
+        *err = "server arena allocation failed";
+        return -2;
+    }
+    for (int i = 0; i < max_inflight_; ++i) {
+        std::unique_ptr<Slot> s(new Slot());
+        s->owner = this;
+        if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+            *err = "stream/event creation failed";
+            return -3;
+        }
+        slots_.push_back(std::move(s));
+    }
+    thr_ = std::thread(&Server::loop, this);
+    return 0;
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (thr_.joinable()) thr_.join();
+    (void)hipSetDevice(device_);
+    for (auto& s : slots_) {
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        void* bufs[] = {s->tab, s->items, s->small};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        if (s->host_out) (void)hipHostFree(s->host_out);
+        if (s->stage) (void)hipHostFree(s->stage);
+        if (s->ev0) (void)hipEventDestroy(s->ev0);
+        if (s->ev1) (void)hipEventDestroy(s->ev1);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+    }
+}
+
+size_t Server::footprint(const Req& q) const {
+    return align_up((size_t)q.n * sizeof(LinePos)) + align_up((size_t)q.n * sizeof(VRec)) +
+           align_up((size_t)q.npoints * 8);
+}
+
+int Server::submit(const Request& r, uint64_t* id, std::string* err) {
+    if (!(r.T > 0.0) || !(r.end > r.start) || !(r.dfreq > 0.0) || !(r.P >= 0.0)) {
+        *err = "request needs T > 0, P >= 0, end > start, dfreq > 0";
+        return -1;
+    }
+    const double npf = std::round((r.end - r.start) / r.dfreq);
+    if (!(npf >= 1.0) || npf > 2.0e9) {
+        *err = "request grid has no points or too many";
+        return -1;
+    }
+    std::unique_ptr<Req> q(new Req());
+    q->r = r;
+    q->npoints = (int)npf;
+    const int span = GX_WAVE * ph_.ppl;
+    q->ntiles = (q->npoints + span - 1) / span;
+    // line selection, Profiles/BaseProfile.cpp:152-157
+    const double lo = r.start - ph_.half_width, hi = r.end + ph_.half_width;
+    auto a = std::lower_bound(v_.nu_host.begin(), v_.nu_host.end(), lo);
+    auto b = std::upper_bound(a, v_.nu_host.end(), hi);
+    q->l0 = a - v_.nu_host.begin();
+    q->n = b - a;
+    q->foot = footprint(*q);
+    if ((int64_t)q->foot > budget_) {
+        *err = "request footprint exceeds the server memory budget";
+        return -2;
+    }
+    q->arrival = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> g(mu_);
+    if (stop_) {
+        *err = "server stopping";
+        return -4;
+    }
+    q->id = next_id_++;
+    *id = q->id;
+    pending_.push_back(q.get());
+    reqs_[q->id] = std::move(q);
+    cv_.notify_all();
+    return 0;
+}
+
+int Server::poll(uint64_t id, Result** out) {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = reqs_.find(id);
+    if (it == reqs_.end()) return -6;
+    *out = &it->second->res;
+    return 0;
+}
+
+int Server::wait(uint64_t id, Result** out) {
+    std::unique_lock<std::mutex> g(mu_);
+    auto it = reqs_.find(id);
+    if (it == reqs_.end()) return -6;
+    Req* q = it->second.get();
+    cv_done_.wait(g, [&] { return q->res.status != 1; });
+    *out = &q->res;
+    return 0;
+}
+
+int Server::release(uint64_t id) {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = reqs_.find(id);
+    if (it == reqs_.end()) return -6;
+    if (it->second->res.status == 1) return -4;  // still running
+    reqs_.erase(it);
+    return 0;
+}
+
+void Server::finish(Slot& s) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, s.ev0, s.ev1);
+    const auto now = std::chrono::steady_clock::now();
+    size_t hoff = 0;
+    for (Req* q : s.reqs) {
+        q->res.sigma.assign(s.host_out + hoff, s.host_out + hoff + q->npoints);
+        hoff += (size_t)q->npoints;
+        q->res.npoints = q->npoints;
+        q->res.device_ms = ms;
+        q->res.latency_ms = std::chrono::duration<double, std::milli>(now - q->arrival).count();
+        q->res.status = 0;
+        arena_.free(q->off, q->foot);
+    }
+    s.reqs.clear();
+    s.busy = false;
+}
+
+int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
+    const int nreq = (int)batch.size();
+    hipStream_t st = s.stream;
+    char* base = arena_.base();
+    // ---- host tables; each request's arena region holds its positions,
+    // records and output grid
+    std::vector<TPParams> tps((size_t)nreq);
+    std::vector<JobDesc> jds((size_t)nreq);
+    std::vector<Job> jobs((size_t)nreq);
+    std::vector<double> Ts((size_t)nreq);
+    int items_total = 0, max_tiles = 0;
+    int64_t max_n = 0;
+    size_t host_total = 0;
+    for (int k = 0; k < nreq; ++k) {
+        Req* q = batch[(size_t)k];
+        const size_t pos_b = q->off;
+        const size_t rec_b = pos_b + align_up((size_t)q->n * sizeof(LinePos));
+        const size_t out_b = rec_b + align_up((size_t)q->n * sizeof(VRec));
+        TPParams& p = tps[(size_t)k];
+        p = TPParams{};
+        p.T = q->r.T;
+        p.P = q->r.P;
+        p.Q = 1.0;  // set on the device from the partition pass
+        p.mass = ph_.mass;
+        p.t_ref = ph_.t_ref;
+        p.p_scale = ph_.p_scale;
+        p.cutoff = cutoff_of(ph_, q->r.P);
+        p.start = q->r.start;
+        p.dfreq = (q->r.end - q->r.start) / (double)q->npoints;  // BaseProfile.cpp:10
+        p.gauss_hwhm = ph_.gauss_hwhm;
+        p.profile = ph_.profile;
+        p.width_conv = ph_.width_conv;
+        p.pressure_mode = ph_.pressure_mode;
+        p.win_lo = 0;
+        p.win_hi = q->npoints;
+        JobDesc& d = jds[(size_t)k];
+        d.l0 = q->l0;
+        d.pos_off = (int64_t)(pos_b / sizeof(LinePos));
+        d.rec_off = (int64_t)(rec_b / sizeof(VRec));
+        d.n = (int32_t)q->n;
+        d.items_off = items_total;
+        d.ntiles = q->ntiles;
+        d.q_slot = k;
+        Job& jb = jobs[(size_t)k];
+        jb.pos = (const LinePos*)(base + pos_b);
+        jb.rec = (const VRec*)(base + rec_b);
+        jb.out = (double*)(base + out_b);
+        jb.n = (int32_t)q->n;
+        jb.profile = ph_.profile;
+        jb.scale_slot = k;
+        jb.accumulate = 0;  // each point is written exactly once
+        Ts[(size_t)k] = q->r.T;
+        items_total += q->ntiles;
+        max_tiles = std::max(max_tiles, q->ntiles);
+        max_n = std::max<int64_t>(max_n, q->n);
+        host_total += (size_t)q->npoints;
+    }
+    const size_t tp_b = align_up(tps.size() * sizeof(TPParams));
+    const size_t jd_b = align_up(jds.size() * sizeof(JobDesc));
+    const size_t jobs_b = align_up(jobs.size() * sizeof(Job));
+    const size_t t_b = align_up(Ts.size() * 8);
+    const size_t tab_b = tp_b + jd_b + jobs_b + t_b;
+    // small arrays: scale, neval (kMaxBatch u64 each), counter, Q, partials
+    const size_t sm_scale = 0, sm_neval = kMaxBatch * 8, sm_counter = 2 * kMaxBatch * 8,
+                 sm_q = sm_counter + 256, sm_part = sm_q + kMaxBatch * 8,
+                 sm_b = sm_part + (size_t)kMaxBatch * kQBlocks * 8;
+    hipError_t e = hipSuccess;
+    size_t dummy_cap = 0;
+    if (!s.small && (e = grow(&s.small, &dummy_cap, sm_b)) != hipSuccess) goto fail;
+    if ((e = grow(&s.tab, &s.tab_cap, tab_b)) != hipSuccess) goto fail;
+    if ((e = grow(&s.items, &s.items_cap, (size_t)std::max(items_total, 1) * sizeof(WorkItem))) !=
+        hipSuccess)
+        goto fail;
+    if (s.stage_cap < tab_b) {
+        if (s.stage) (void)hipHostFree(s.stage);
+        s.stage = nullptr;
+        s.stage_cap = 0;
+        if ((e = hipHostMalloc(&s.stage, tab_b, hipHostMallocDefault)) != hipSuccess) goto fail;
+        s.stage_cap = tab_b;
+    }
+    if (s.host_cap < host_total) {
+        if (s.host_out) (void)hipHostFree(s.host_out);
+        s.host_out = nullptr;
+        s.host_cap = 0;
+        if ((e = hipHostMalloc((void**)&s.host_out, std::max<size_t>(host_total, 1) * 8,
+                               hipHostMallocDefault)) != hipSuccess)
+            goto fail;
+        s.host_cap = host_total;
+    }
+    {
+        // the slot's previous batch has completed (the slot was free), so its
+        // staging buffer may be rewritten
+        char* h = (char*)s.stage;
+        std::memcpy(h, tps.data(), tps.size() * sizeof(TPParams));
+        std::memcpy(h + tp_b, jds.data(), jds.size() * sizeof(JobDesc));
+        std::memcpy(h + tp_b + jd_b, jobs.data(), jobs.size() * sizeof(Job));
+        std::memcpy(h + tp_b + jd_b + jobs_b, Ts.data(), Ts.size() * 8);
+        char* tab = (char*)s.tab;
+        TPParams* d_tp = (TPParams*)tab;
+        JobDesc* d_jd = (JobDesc*)(tab + tp_b);
+        Job* d_jobs = (Job*)(tab + tp_b + jd_b);
+        double* d_T = (double*)(tab + tp_b + jd_b + jobs_b);
+        char* sm = (char*)s.small;
+        unsigned long long* d_scale = (unsigned long long*)(sm + sm_scale);
+        unsigned long long* d_neval = (unsigned long long*)(sm + sm_neval);
+        unsigned int* d_counter = (unsigned int*)(sm + sm_counter);
+        double* d_Q = (double*)(sm + sm_q);
+        double* d_part = (double*)(sm + sm_part);
+        WorkItem* d_items = (WorkItem*)s.items;
+        if ((e = hipEventRecord(s.ev0, st)) != hipSuccess) goto fail;
+        if ((e = hipMemcpyAsync(tab, s.stage, tab_b, hipMemcpyHostToDevice, st)) != hipSuccess) goto fail;
+        if ((e = gxk::partition(st, v_.sE, v_.sg, v_.n_states, d_T, nreq, d_part, kQBlocks, d_Q)) !=
+            hipSuccess)
+            goto fail;
+        if ((e = gxk::take_q(st, d_jd, nreq, d_Q, d_tp)) != hipSuccess) goto fail;
+        if ((e = gxk::fill_u64(st, d_scale, nreq, 0ull)) != hipSuccess) goto fail;
+        if ((e = gxk::fill_u64(st, d_neval, nreq, 0ull)) != hipSuccess) goto fail;
+        if ((e = gxk::line_pos(st, v_.nu, d_jd, nreq, max_n, d_tp, (LinePos*)base)) != hipSuccess) goto fail;
+        if ((e = gxk::line_max(st, v_.nu, v_.E, v_.A, v_.g, (const LinePos*)base, d_jd, nreq, max_n, d_tp,
+                               d_scale)) != hipSuccess)
+            goto fail;
+        if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, (const LinePos*)base, d_jd,
+                               nreq, max_n, d_tp, d_scale, (VRec*)base, d_neval)) != hipSuccess)
+            goto fail;
+        if ((e = gxk::tiles(st, (const LinePos*)base, d_jd, nreq, max_tiles, d_tp, GX_WAVE * ph_.ppl,
+                            d_items, nullptr, nullptr)) != hipSuccess)
+            goto fail;
+        if ((e = gxk::profile_tiles(st, ph_.profile, ph_.ppl, d_items, items_total, d_jobs, d_scale,
+                                    d_counter, 0)) != hipSuccess)
+            goto fail;
+        size_t hoff = 0;
+        for (int k = 0; k < nreq; ++k) {
+            const size_t np = (size_t)batch[(size_t)k]->npoints;
+            if ((e = hipMemcpyAsync(s.host_out + hoff, jobs[(size_t)k].out, np * 8, hipMemcpyDeviceToHost,
+                                    st)) != hipSuccess)
+                goto fail;
+            hoff += np;
+        }
+        if ((e = hipEventRecord(s.ev1, st)) != hipSuccess) goto fail;
+    }
+    s.reqs = batch;
+    s.busy = true;
+    batches_++;
+    return 0;
+fail:
+    *err = std::string("batch launch: ") + hipGetErrorString(e);
+    return -3;
+}
+
+void Server::loop() {
+    (void)hipSetDevice(device_);
+    std::unique_lock<std::mutex> g(mu_);
+    for (;;) {
+        bool inflight = false;
+        for (auto& s : slots_) inflight |= s->busy;
+        if (stop_ && !inflight && pending_.empty()) break;
+        // completions
+        for (auto& s : slots_) {
+            if (s->busy && hipEventQuery(s->ev1) == hipSuccess) {
+                finish(*s);
+                cv_done_.notify_all();
+            }
+        }
+        // admission: FIFO while the arena has room and a slot is free
+        for (auto& s : slots_) {
+            if (s->busy || pending_.empty()) continue;
+            std::vector<Req*> batch;
+            while (!pending_.empty() && (int)batch.size() < kMaxBatch) {
+                Req* q = pending_.front();
+                if (!arena_.alloc(q->foot, &q->off)) break;
+                pending_.pop_front();
+                batch.push_back(q);
+            }
+            if (batch.empty()) break;
+            std::string err;
+            if (launch(*s, batch, &err) != 0) {
+                for (Req* q : batch) {
+                    q->res.status = -3;
+                    arena_.free(q->off, q->foot);
+                }
+                loop_err_ = err;
+                cv_done_.notify_all();
+            }
+        }
+        inflight = false;
+        for (auto& s : slots_) inflight |= s->busy;
+        if (inflight)
+            cv_.wait_for(g, std::chrono::microseconds(50));
+        else if (!stop_ && pending_.empty())
+            cv_.wait(g);
+        else if (!pending_.empty())
+            cv_.wait_for(g, std::chrono::microseconds(50));
+    }
+}
+
+}  // namespace gxs

@@ -168,6 +168,25 @@ class HipManager:
                               ga.ctypes.data, nx.ctypes.data)
         self._chk(self._L.gexs_load_list(self._h, C.byref(st), C.byref(tr)))
 
+    def set_launch(self, persistent_blocks=0, use_graph=False):
+        self._chk(self._L.gexs_set_launch(self._h, int(persistent_blocks), 1 if use_graph else 0))
+
+    def autotune(self, tps, start, end, npoints):
+        """Pick the fastest profile-kernel launch shape for this batch; returns
+        (best, [all candidates]) as dicts."""
+        arr = (_lib.TP * len(tps))()
+        for i, t in enumerate(tps):
+            arr[i] = _lib.TP(float(t[0]), float(t[1]), float(t[2]) if len(t) > 2 else 0.0)
+        grid = _lib.Grid(float(start), float(end), int(npoints), 0)
+        best = _lib.Tuning()
+        rep = (_lib.Tuning * 64)()
+        nrep = C.c_int()
+        self._chk(self._L.gexs_autotune(self._h, arr, len(tps), C.byref(grid), C.byref(best), rep,
+                                        64, C.byref(nrep)))
+        conv = lambda t: dict(points_per_lane=t.points_per_lane, order=t.order,  # noqa: E731
+                              persistent_blocks=t.persistent_blocks, kernel_ms=t.kernel_ms)
+        return conv(best), [conv(rep[i]) for i in range(min(nrep.value, 64))]
+
     def partition(self, temps):
         T = np.ascontiguousarray(temps, dtype=np.float64)
         Q = np.empty_like(T)
@@ -184,3 +203,36 @@ class HipManager:
             out = np.empty((len(tps), int(npoints)), dtype=np.float64)
         self._chk(self._L.gexs_run_batch(self._h, arr, len(tps), C.byref(grid), _ptr(out)))
         return out
+
+    # ------------------------------------------------------------- serving
+    def serve_start(self, max_inflight=2, mem_budget=4 << 30):
+        """Start the continuous-batching server over the resident list."""
+        self._chk(self._L.gexs_serve_start(self._h, int(max_inflight), int(mem_budget)))
+
+    def submit(self, T, P, start, end, dfreq):
+        rid = C.c_uint64()
+        req = _lib.Request(float(T), float(P), float(start), float(end), float(dfreq))
+        self._chk(self._L.gexs_submit(self._h, C.byref(req), C.byref(rid)))
+        return rid.value
+
+    def _result(self, r):
+        if r.status != 0:
+            return dict(status=r.status)
+        sigma = np.ctypeslib.as_array(r.sigma, shape=(r.npoints,)).copy()
+        return dict(status=0, sigma=sigma, latency_ms=r.latency_ms, device_ms=r.device_ms)
+
+    def poll(self, rid):
+        r = _lib.Result()
+        self._chk(self._L.gexs_poll(self._h, C.c_uint64(rid), C.byref(r)))
+        return self._result(r)
+
+    def wait(self, rid, release=True):
+        r = _lib.Result()
+        self._chk(self._L.gexs_wait(self._h, C.c_uint64(rid), C.byref(r)))
+        out = self._result(r)
+        if release:
+            self._chk(self._L.gexs_release(self._h, C.c_uint64(rid)))
+        return out
+
+    def serve_stop(self):
+        self._chk(self._L.gexs_serve_stop(self._h))

@@ -187,14 +187,44 @@ typedef struct gexs_tp {
 } gexs_tp;
 
 /* Cross-sections of the resident list at ntp (T,P) points over one grid, in
- * one launch sequence.  out (host or device pointer) receives ntp*npoints
- * doubles, row-major by (T,P). */
+ * one launch sequence (partition function, records, one profile kernel over
+ * all (T,P) x tiles).  out (host or device pointer) receives ntp*npoints
+ * doubles, row-major by (T,P).  Synchronous.  Lines are selected by
+ * start - half_width <= nu <= end + half_width. */
 int gexs_run_batch(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* grid,
                    double* out);
+/* Launch shape of batched passes: persistent_blocks > 0 runs the profile
+ * kernel as that many workgroups pulling tiles from a counter (0: one wave
+ * per tile); use_graph != 0 captures a batch's device work into a HIP graph
+ * on first use and replays it while the plan (sizes, grid, tuning) is
+ * unchanged, rewriting only the (T,P) table. */
+int gexs_set_launch(gexs_ctx* ctx, int persistent_blocks, int use_graph);
+
+/* Autotuner of the FP32C profile kernel: runs the batch once per candidate
+ * (points per lane 1, 2, 4, 8 = wave-tile widths of 64..512 points; grid or
+ * heaviest-first tile order; one wave per tile or 4 / 8 persistent
+ * workgroups per CU), times the profile kernel with HIP events, and keeps
+ * the fastest on the ctx.  report (may be NULL) receives up to report_cap
+ * candidates; *n_report their count. */
+typedef struct gexs_tuning {
+    int32_t points_per_lane;
+    int32_t order;
+    int32_t persistent_blocks;
+    int32_t reserved;
+    double kernel_ms;
+} gexs_tuning;
+int gexs_autotune(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* grid,
+                  gexs_tuning* best, gexs_tuning* report, int report_cap, int* n_report);
 
 /* ---------------------------------------------------------------- serving
  * Continuous batching over the resident list: requests are admitted by HBM
- * footprint as they arrive and run while others are on the device. */
+ * footprint as they arrive and run while others are on the device.
+ * gexs_serve_start: max_inflight concurrent batches (one stream each), a
+ * device arena of mem_budget bytes from which each admitted request takes
+ * 48 B per selected line + 8 B per grid point until it completes.  The grid
+ * of a request has round((end - start)/dfreq) points with step
+ * (end - start)/npoints.  While serving, only submit / poll / wait /
+ * release / serve_stop may be called on the ctx (the list is read-only). */
 typedef struct gexs_request {
     double T, P;
     double start, end;   /* range, cm^-1 */

@@ -58,10 +58,10 @@ def test_exomol_driver_vs_oracle(O, tmp_path):
     a0, a1 = broadeners(L.J)
     with open(tmp_path / "m.states", "w") as f:
         for i, (e, g, j) in enumerate(zip(L.E, L.g, L.J)):
-            f.write(f"{i + 1:12d} {e!r} {g:6d} {j:7d}\n")
+            f.write(f"{i + 1:12d} {float(e)!r} {int(g):6d} {int(j):7d}\n")
     with open(tmp_path / "m.trans", "w") as f:
         for u, l, a in zip(L.upper, L.lower, L.A):
-            f.write(f"{u + 1} {l + 1} {float(a)!r}\n")
+            f.write(f"{int(u) + 1} {int(l) + 1} {float(a)!r}\n")
     with open(tmp_path / "m.broad", "w") as f:
         for j, (g, n) in a0.items():
             f.write(f"a0 {g!r} {n!r} {j}\n")

@@ -1,0 +1,77 @@
+"""Serving: continuous batching over a resident list.
+
+A request's result must be the same alone and in a mix (bit for bit: every
+request owns its records, tiles and output, and no reduction crosses
+requests), must match the oracle (peak-relative 1e-4, band strength 1e-6),
+and requests larger than the memory budget are refused at submit.
+"""
+import numpy as np
+import pytest
+
+from gexs_amd import synth
+from gexs_amd._lib import GexsError
+from gexs_amd.manager import HipManager
+
+pytestmark = pytest.mark.gpu
+
+P_SCALE = synth.P_SCALE_BAR
+
+
+@pytest.fixture(scope="module")
+def server():
+    L = synth.workload_list("voigt_tp_table_1e7", scale=0.002)
+    m = HipManager("voigt")
+    m.InitializeConstants(100.0, 296.0, 1.0, 0.01, 28.0, 1.0, 296.0, P_SCALE, cutoff=25.0)
+    m.load_list(L)
+    m.serve_start(max_inflight=3, mem_budget=1 << 30)
+    yield L, m
+    m.serve_stop()
+    m.close()
+
+
+REQS = [(300.0, 1.0, 1000.0, 1500.0, 0.01), (2500.0, 1e-4, 3000.0, 3100.0, 0.01),
+        (900.0, 50.0, 500.0, 5500.0, 1.0), (1500.0, 0.1, 8000.0, 8300.0, 0.1),
+        (200.0, 1e-5, 600.0, 700.0, 0.01), (3000.0, 3.0, 12000.0, 14500.0, 0.1)]
+
+
+def test_alone_equals_mix(server):
+    L, m = server
+    alone = []
+    for r in REQS:
+        alone.append(m.wait(m.submit(*r))["sigma"])
+    ids = [m.submit(*r) for r in REQS * 3]
+    mixed = [m.wait(i)["sigma"] for i in ids]
+    for k, s in enumerate(mixed):
+        assert np.array_equal(s, alone[k % len(REQS)]), k
+
+
+def test_served_vs_oracle(server, O):
+    L, m = server
+    nu, E, A, g, _, _ = L.lines()
+    gam, nex = L.gamma.astype(np.float64), L.nexp.astype(np.float64)
+    A = L.A.astype(np.float64)
+    for T, P, a, b, d in REQS:
+        res = m.wait(m.submit(T, P, a, b, d))
+        N = int(round((b - a) / d))
+        assert res["sigma"].shape == (N,)
+        keep = (nu >= a - 100.0) & (nu <= b + 100.0) & (nu != 0)
+        Q = O.partition(L.E, L.g, T)
+        p = O.params("voigt", T=T, P=P, Q=Q, mass=28.0, cutoff=25.0, p_scale=P_SCALE)
+        ref, _ = O.cross_section(p, a, (b - a) / N, N, nu[keep], E[keep], A[keep], g[keep],
+                                 gam[keep], nex[keep], nthreads=8)
+        if ref.max() == 0:
+            assert np.all(res["sigma"] == 0)
+            continue
+        err = np.max(np.abs(res["sigma"] - ref)) / ref.max()
+        band = abs(res["sigma"].sum() - ref.sum()) / ref.sum()
+        print(f"T={T} P={P} [{a},{b}) d={d}: err {err:.2e} band {band:.2e} "
+              f"latency {res['latency_ms']:.2f} ms", flush=True)
+        assert err <= 1e-4 and band <= 1e-6
+
+
+def test_admission_refuses_oversized(server):
+    L, m = server
+    with pytest.raises(GexsError):
+        m.submit(296.0, 1.0, 0.0, 40000.0, 1e-4)  # 4e8 points > 1 GiB budget
+    with pytest.raises(GexsError):
+        m.submit(-1.0, 1.0, 0.0, 10.0, 0.1)
