@@ -48,12 +48,14 @@ def parse():
     ap.add_argument("--order", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     return ap.parse_args()
 
 
 def cpu_baseline(lst, spec, T, P, Q, target_s):
-    """Reference CPU path on a contiguous nu slice of the list, all host cores."""
+    """Reference CPU path on a contiguous nu slice of the list, all host cores,
+    repeated until about target_s seconds of CPU work are timed."""
     from oracle import oracle as O
     nu, E, A, g, gam, nex = lst.lines()
     o = np.argsort(nu, kind="stable")
@@ -65,16 +67,12 @@ def cpu_baseline(lst, spec, T, P, Q, target_s):
     use_ref = O.ref() is not None
     p = O.params("voigt", T=T, P=P, Q=Q, mass=spec["mass"], cutoff=spec["cutoff"],
                  p_scale=1 / 1.01325)
-    # calibrate: a small slice first, then one sized for ~target_s
-    n0 = 2000
     mid = nu.shape[0] // 3
-    frac_lines = None
-    rate = None
-    for n_try in (n0, None):
-        if n_try is None:
-            n_try = int(min(nu.shape[0] - mid, max(n0, rate * target_s / (ev0 / n0))))
-        sl = slice(mid, mid + n_try)
-        ev = O.neval(p, start, dnu, N, nu[sl], E[sl], A[sl], g[sl])
+    n_lines = min(nu.shape[0] - mid, 200_000)
+    sl = slice(mid, mid + n_lines)
+    ev_one = O.neval(p, start, dnu, N, nu[sl], E[sl], A[sl], g[sl])
+    total_ev, total_t, reps = 0, 0.0, 0
+    while total_t < target_s and reps < 100:
         t0 = time.perf_counter()
         if use_ref:
             O.ref_voigt_tp(start, dnu, N, T, P, Q, spec["mass"], 296.0, 1 / 1.01325, nu[sl],
@@ -82,18 +80,27 @@ def cpu_baseline(lst, spec, T, P, Q, target_s):
         else:
             O.cross_section(p, start, dnu, N, nu[sl], E[sl], A[sl], g[sl], gam[sl], nex[sl],
                             nthreads=cores)
-        dt = time.perf_counter() - t0
-        if rate is None:
-            ev0 = ev
-        rate = ev / dt
-        frac_lines = n_try
-    return {"value": rate, "unit": "evals/s", "cores": cores,
+        total_t += time.perf_counter() - t0
+        total_ev += ev_one
+        reps += 1
+    return {"value": total_ev / total_t, "unit": "evals/s", "cores": cores,
             "kind": "reference" if use_ref else "port",
-            "sample": f"{frac_lines} consecutive nu-sorted lines from the middle of the list, "
-                      f"full grid, {ev:.3e} evals in {dt:.1f} s"
+            "sample": f"{n_lines} consecutive nu-sorted lines from the middle of the list on the "
+                      f"full grid ({ev_one:.3e} evals) x {reps} runs = {total_ev:.3e} evals in "
+                      f"{total_t:.1f} s"
                       + ("; OpenMPManager::ComputeVoigtTP + humlic compiled from the reference "
                          "sources (gammaL without pressure, as the reference CPU path)"
                          if use_ref else "; FP64 oracle port of ComputeVoigtTP")}
+
+
+def traffic_from_profile(workload):
+    """HBM bytes per launch of the profile kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
+    f = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        return json.load(fh).get("hbm_bytes_per_launch")
 
 
 def main():
@@ -124,10 +131,14 @@ def main():
                           cutoff=spec["cutoff"])
     m.TransferFreq(start + np.arange(N, dtype=np.float64) * dnu)
     t0 = time.perf_counter()
-    m.load_list(lst)  # H2D, device assembly and nu sort; outside the timed region
+    n = m.load_list(lst)  # H2D, device assembly and nu sort; outside the timed region
     Q = float(m.partition([T])[0])
     log(f"[bench] list resident + sorted in {time.perf_counter() - t0:.2f} s, Q(T)={Q:.6g}")
-    n = lst.n_trans
+    tuning = None
+    if not a.no_autotune and not a.ppl:
+        best, cands = m.autotune([(T, P, Q)], start, end, N)
+        tuning = {"best": best, "candidates": cands}
+        log(f"[bench] autotune: {best}")
     for _ in range(warm):
         m.ExecuteCrossSection(N, n, 0, T, P, Q)
     m.sync()
@@ -197,6 +208,7 @@ def main():
             "n_eval_per_step": n_eval,
             "xsec_per_s": steps / elapsed,
             "tiles": st["n_tiles"],
+            "tuning": tuning["best"] if tuning else {"points_per_lane": a.ppl or 4},
         },
         "roofline": {
             "bound": "valu",
@@ -204,7 +216,7 @@ def main():
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
-            "traffic": None,
+            "traffic": traffic_from_profile(a.workload),
             "kernel": "k_profile_tiles<VOIGT>",
             "kernel_ms_avg": k_avg,
             "flop_per_eval": flops,

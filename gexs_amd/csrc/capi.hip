@@ -826,6 +826,12 @@ int gexs_load_list(gexs_ctx* c, const gexs_states* st, const gexs_transitions* t
     return GEXS_OK;
 }
 
+int gexs_line_count(gexs_ctx* c, int64_t* n) {
+    if (!c || !n) return GEXS_EINVAL;
+    *n = c->n;
+    return GEXS_OK;
+}
+
 int gexs_partition(gexs_ctx* c, const double* T, int nT, double* Q_out) {
     if (!c || !T || !Q_out || nT <= 0) return GEXS_EINVAL;
     if (c->n_states <= 0) return c->fail(GEXS_ESTATE, "no resident state table (gexs_load_list)");

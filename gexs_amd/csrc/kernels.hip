@@ -292,55 +292,87 @@ typedef const LinePos* ConstPos;
 typedef const VRec* ConstRec;
 #endif
 
+// Region-I value (times the line amplitude) at x:
+// (Kl x^2 + Kl b) / (x^2 (x^2 + 2y^2 - 1) + b^2), b = 1/2 + y^2
+__device__ __forceinline__ float region1(float x, float Kl, float Klb, float e, float b2) {
+    const float x2 = x * x;
+    const float den = fmaf(x2, x2 + e, b2);
+    const float num = fmaf(x2, Kl, Klb);
+    return num * __builtin_amdgcn_rcpf(den);
+}
+
 template <int PROF, int PPL>
 __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
                                          int lane) {
     const ConstPos pos = (ConstPos)jb.pos;
     const ConstRec rec = (ConstRec)jb.rec;
     float sum[PPL], cmp[PPL], jf[PPL];
+    int jj[PPL];
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
         sum[q] = 0.0f;
         cmp[q] = 0.0f;
-        jf[q] = (float)(it.w0 + lane + GX_WAVE * q);
+        jj[q] = it.w0 + lane + GX_WAVE * q;
+        jf[q] = (float)jj[q];
     }
     const int w0 = it.w0, w1 = it.w1;
+    const int hi = it.hi;
     constexpr int CH = 32;  // lines per uncompensated partial sum
-    for (int i0 = it.lo; i0 < it.hi; i0 += CH) {
-        const int i1 = min(i0 + CH, it.hi);
+    // software pipeline: the record of line i+1 is loaded while line i is evaluated
+    LinePos lp_n;
+    VRec r_n;
+    if (it.lo < hi) {
+        lp_n = pos[it.lo];
+        r_n = rec[it.lo];
+    }
+    for (int i0 = it.lo; i0 < hi; i0 += CH) {
+        const int i1 = min(i0 + CH, hi);
         float part[PPL];
 #pragma unroll
         for (int q = 0; q < PPL; ++q) part[q] = 0.0f;
         for (int i = i0; i < i1; ++i) {
-            const LinePos lp = pos[i];
-            const VRec r = rec[i];
+            const LinePos lp = lp_n;
+            const VRec r = r_n;
+            if (i + 1 < hi) {
+                lp_n = pos[i + 1];
+                r_n = rec[i + 1];
+            }
             const bool full = (lp.ib <= w0) && (lp.ie >= w1);
             if (PROF == GX_PROF_VOIGT) {
                 const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
-                if (full && kmin >= r.k15) {
+                const float e = fmaf(2.0f * r.y, r.y, -1.0f);
+                if (kmin >= r.k15) {
                     // every point of the tile is in Humlicek region I
-                    const float e = fmaf(2.0f * r.y, r.y, -1.0f);
+                    if (full) {
 #pragma unroll
-                    for (int q = 0; q < PPL; ++q) {
-                        float x = fmaf(lp.cf - jf[q], r.A, r.B);
-                        float x2 = x * x;
-                        float den = fmaf(x2, x2 + e, r.b2);
-                        float num = fmaf(x2, r.Kl, r.Klb);
-                        part[q] = fmaf(num, __builtin_amdgcn_rcpf(den), part[q]);
+                        for (int q = 0; q < PPL; ++q)
+                            part[q] += region1(fmaf(lp.cf - jf[q], r.A, r.B), r.Kl, r.Klb, e, r.b2);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < PPL; ++q) {
+                            const float v = region1(fmaf(lp.cf - jf[q], r.A, r.B), r.Kl, r.Klb, e, r.b2);
+                            part[q] += (jj[q] >= lp.ib && jj[q] < lp.ie) ? v : 0.0f;
+                        }
                     }
                     continue;
                 }
+                // line centre near the tile: region I inline, regions II-IV per lane
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const float x = fabsf(fmaf(lp.cf - jf[q], r.A, r.B));
+                    float v = region1(x, r.Kl, r.Klb, e, r.b2);
+                    if (x + r.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, r.y);
+                    part[q] += (jj[q] >= lp.ib && jj[q] < lp.ie) ? v : 0.0f;
+                }
+                continue;
             }
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
-                const int j = w0 + lane + GX_WAVE * q;
-                if (j >= lp.ib && j < lp.ie) {
-                    float x = fabsf(fmaf(lp.cf - jf[q], r.A, r.B));
+                if (jj[q] >= lp.ib && jj[q] < lp.ie) {
+                    const float x = fabsf(fmaf(lp.cf - jf[q], r.A, r.B));
                     float k;
-                    if (PROF == GX_PROF_VOIGT) {
-                        k = gx::w4_f32(x, r.y);
-                    } else if (PROF == GX_PROF_VOIGT_QUAD) {
-                        float rr = x * x + r.y * r.y;
+                    if (PROF == GX_PROF_VOIGT_QUAD) {
+                        const float rr = x * x + r.y * r.y;
                         k = rr < 100.0f ? (float)gx::voigt916_f64((double)x, (double)r.y)
                                         : gx::gh11_f32(x, r.y);
                     } else {
@@ -349,19 +381,19 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, doub
                     part[q] = fmaf(r.Kg, k, part[q]);
                 }
             }
+            (void)full;
         }
 #pragma unroll
         for (int q = 0; q < PPL; ++q) kahan_add(sum[q], cmp[q], part[q]);
     }
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
-        const int j = w0 + lane + GX_WAVE * q;
         const double v = ((double)sum[q] - (double)cmp[q]) * inv_scale;
-        if (j < w1) {
+        if (jj[q] < w1) {
             if (jb.accumulate)
-                jb.out[j] += v;
+                jb.out[jj[q]] += v;
             else
-                jb.out[j] = v;
+                jb.out[jj[q]] = v;
         }
     }
 }

@@ -116,12 +116,12 @@ __device__ inline double w4_f64(double x, double y) {
     return exp(u.r) * cos(u.i) - re_div(num, den);
 }
 
-// W4 regions II-IV in FP32 (region I has its own fast path in the kernel)
-__device__ inline float w4_f32(float x, float y) {
+// W4 in FP32 for x >= 0 and x + y < 15 (regions II-IV; region I is
+// evaluated inline by the profile kernel)
+__device__ inline float w4_f32_inner(float x, float y) {
     typedef cplx<float> C;
     C t = {y, -x};
-    float s = x + y;  // x >= 0 here
-    if (s >= 15.0f) return w4_region1(x, y);
+    float s = x + y;
     C u = cmul(t, t);
     if (s >= 5.5f) {
         C num = cmul(t, cadd(1.410474f, cscale(u, .5641896f)));

@@ -167,6 +167,10 @@ class HipManager:
         tr = _lib.Transitions(up.shape[0], up.ctypes.data, lo.ctypes.data, A.ctypes.data,
                               ga.ctypes.data, nx.ctypes.data)
         self._chk(self._L.gexs_load_list(self._h, C.byref(st), C.byref(tr)))
+        n = C.c_int64()
+        self._chk(self._L.gexs_line_count(self._h, C.byref(n)))
+        self._n_lines = n.value
+        return n.value
 
     def set_launch(self, persistent_blocks=0, use_graph=False):
         self._chk(self._L.gexs_set_launch(self._h, int(persistent_blocks), 1 if use_graph else 0))

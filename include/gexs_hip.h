@@ -169,6 +169,8 @@ typedef struct gexs_transitions {
  * with nu == 0 (never selected, Profiles/BaseProfile.cpp:156) and sort the
  * rest by nu on the device.  Replaces any previous list. */
 int gexs_load_list(gexs_ctx* ctx, const gexs_states* st, const gexs_transitions* tr);
+/* Number of current lines (after load_list: the transitions with nu != 0). */
+int gexs_line_count(gexs_ctx* ctx, int64_t* n);
 /* Q(T) on the device from the resident states, FP64, fixed reduction order
  * (exomol_functions.cpp:50-73). */
 int gexs_partition(gexs_ctx* ctx, const double* T, int nT, double* Q_out);
