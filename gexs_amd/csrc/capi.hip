@@ -152,7 +152,9 @@ struct gexs_ctx {
     DevArr st_E, st_g, q_T, q_part, q_Q;
 
     // derived per grid / per (T,P)
-    DevArr pos, rec, lf64, items, items_tmp, cost_key, item_idx, jobs, tp, scale, neval, counter;
+    DevArr pos, rec, recx, kmax, lf64, items, items_tmp, cost_key, item_idx, jobs, tp, scale, neval,
+        counter;
+    int cus = 256;  // compute units of the device
     PosKey pos_key;
     PosKey items_key;
     int items_span = 0, items_order = -1, items_njobs = 0, n_items = 0;
@@ -324,8 +326,20 @@ int tiles_per_job(const gexs_ctx* c) {
     return w > 0 ? (w + span - 1) / span : 0;
 }
 
+// profile-kernel launch shape: order 2 = per-XCD queues (persistent);
+// persistent_blocks > 0 with order 0/1 = one shared queue
+int launch_mode(const gexs_ctx* c) {
+    if (c->order == 2) return 2;
+    return c->persistent_blocks > 0 ? 1 : 0;
+}
+int launch_blocks(const gexs_ctx* c) {
+    if (c->persistent_blocks > 0) return c->persistent_blocks;
+    return 8 * c->cus;
+}
+
 // work items for the njobs jobs of c->jd / c->tp (one grid window); sorted
-// heaviest first when order == 1
+// heaviest first when order == 1; with order == 2 in grid order and split
+// into GX_XCDS contiguous ranges of equal line count for the per-XCD queues
 int ensure_items(gexs_ctx* c, int njobs) {
     const int span = GX_WAVE * c->ppl;
     const PosKey key = c->pos_key;
@@ -348,6 +362,27 @@ int ensure_items(gexs_ctx* c, int njobs) {
                                       total, c->sb, 0, 32));
         HIPCHK(c, gxk::permute_items(c->stream, c->items_tmp.as<WorkItem>(), c->item_idx.as<uint32_t>(),
                                      total, c->items.as<WorkItem>()));
+    }
+    HIPCHK(c, c->counter.ensure(256));
+    if (c->order == 2 && total > 0) {
+        // cost-balanced split of the grid-ordered items over the XCD queues
+        std::vector<WorkItem> h((size_t)total);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->items.p, (size_t)total * sizeof(WorkItem), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double tot = 0.0;
+        for (const auto& w : h) tot += (double)std::max(w.hi - w.lo, 0) + 1.0;
+        uint32_t bounds[GX_XCDS + 1];
+        bounds[0] = 0;
+        double acc = 0.0;
+        int q = 1;
+        for (int i = 0; i < total && q < GX_XCDS; ++i) {
+            acc += (double)std::max(h[(size_t)i].hi - h[(size_t)i].lo, 0) + 1.0;
+            while (q < GX_XCDS && acc >= tot * q / GX_XCDS) bounds[q++] = (uint32_t)(i + 1);
+        }
+        while (q <= GX_XCDS) bounds[q++] = (uint32_t)total;
+        bounds[GX_XCDS] = (uint32_t)total;
+        HIPCHK(c, c->staging.upload((char*)c->counter.p + GX_XCDS * 4, bounds, sizeof bounds, c->stream));
     }
     c->items_key = key;
     c->items_span = span;
@@ -394,6 +429,7 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
         jds[j].l0 = l0;
         jds[j].pos_off = 0;
         jds[j].rec_off = (int64_t)j * n;
+        jds[j].recx_off = (int64_t)j * n;
         jds[j].n = n;
         jds[j].items_off = j * per;
         jds[j].ntiles = per;
@@ -402,12 +438,15 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     HIPCHK(c, c->jd.ensure(jds.size() * sizeof(JobDesc)));
     HIPCHK(c, c->staging.upload(c->jd.p, jds.data(), jds.size() * sizeof(JobDesc), s));
     HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRec)));
+    HIPCHK(c, c->recx.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRecX)));
+    HIPCHK(c, c->kmax.ensure((size_t)njobs * 8));
     HIPCHK(c, c->scale.ensure((size_t)njobs * 8));
     HIPCHK(c, c->neval.ensure((size_t)njobs * 8));
     HIPCHK(c, c->counter.ensure(256));
     HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
     HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 1, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->kmax.as<unsigned long long>(), njobs, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 4, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
     HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
@@ -416,11 +455,12 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n,
                             c->tp.as<TPParams>(), c->scale.as<unsigned long long>(), c->rec.as<VRec>(),
-                            c->neval.as<unsigned long long>()));
+                            c->recx.as<VRecX>(), c->kmax.as<int>(), c->neval.as<unsigned long long>()));
     std::vector<Job> jobs((size_t)njobs);
     for (int j = 0; j < njobs; ++j) {
         jobs[j].pos = c->pos.as<LinePos>();
         jobs[j].rec = c->rec.as<VRec>() + (size_t)j * n;
+        jobs[j].recx = c->recx.as<VRecX>() + (size_t)j * n;
         jobs[j].out = outs[j];
         jobs[j].n = n;
         jobs[j].profile = c->profile;
@@ -430,10 +470,12 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     HIPCHK(c, c->jobs.ensure(jobs.size() * sizeof(Job)));
     HIPCHK(c, c->staging.upload(c->jobs.p, jobs.data(), jobs.size() * sizeof(Job), s));
     if ((rc = ensure_items(c, njobs)) != GEXS_OK) return rc;
+    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, c->jd.as<JobDesc>(), c->pos.as<LinePos>(),
+                            c->kmax.as<int>(), c->profile));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->items.as<WorkItem>(), c->n_items,
                                  c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
-                                 c->counter.as<unsigned int>(), c->persistent_blocks));
+                                 c->counter.as<unsigned int>(), launch_mode(c), launch_blocks(c)));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     c->last.n_tiles = c->n_items;
     return GEXS_OK;
@@ -487,6 +529,11 @@ int gexs_create(int device, int profile, gexs_ctx** out) {
         return GEXS_EHIP;
     }
     c->tcount = 0;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->cus = prop.multiProcessorCount;
+    }
     // defaults of the reference (Input.cpp:2-15, BaseProfile.cpp:49-54, defines.h)
     c->k.half_width = 100.0;
     c->k.temperature = 296.0;
@@ -762,7 +809,7 @@ int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order) {
     if (!c) return GEXS_EINVAL;
     if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4 && points_per_lane != 8)
         return c->fail(GEXS_EINVAL, "points_per_lane must be 1, 2, 4 or 8");
-    if (order != 0 && order != 1) return c->fail(GEXS_EINVAL, "order must be 0 or 1");
+    if (order < 0 || order > 2) return c->fail(GEXS_EINVAL, "order must be 0, 1 or 2");
     c->ppl = points_per_lane;
     c->order = order;
     return GEXS_OK;
@@ -881,7 +928,8 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, size_t jd_b, size_t job
     }
     HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
     HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 1, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->kmax.as<unsigned long long>(), njobs, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 4, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
     HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
@@ -890,10 +938,12 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, size_t jd_b, size_t job
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n,
                             c->tp.as<TPParams>(), c->scale.as<unsigned long long>(), c->rec.as<VRec>(),
-                            c->neval.as<unsigned long long>()));
+                            c->recx.as<VRecX>(), c->kmax.as<int>(), c->neval.as<unsigned long long>()));
+    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, c->jd.as<JobDesc>(), c->pos.as<LinePos>(),
+                            c->kmax.as<int>(), c->profile));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->items.as<WorkItem>(), c->n_items,
                                  c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
-                                 c->counter.as<unsigned int>(), c->persistent_blocks));
+                                 c->counter.as<unsigned int>(), launch_mode(c), launch_blocks(c)));
     return GEXS_OK;
 }
 
@@ -978,6 +1028,8 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     HIPCHK(c, c->jd.ensure((size_t)ntp * sizeof(JobDesc)));
     HIPCHK(c, c->jobs.ensure((size_t)ntp * sizeof(Job)));
     HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * ntp * sizeof(VRec)));
+    HIPCHK(c, c->recx.ensure((size_t)std::max(n, 1) * ntp * sizeof(VRecX)));
+    HIPCHK(c, c->kmax.ensure((size_t)ntp * 8));
     HIPCHK(c, c->scale.ensure((size_t)ntp * 8));
     HIPCHK(c, c->neval.ensure((size_t)ntp * 8));
     HIPCHK(c, c->counter.ensure(256));
@@ -993,6 +1045,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
         d.l0 = l0;
         d.pos_off = 0;
         d.rec_off = (int64_t)j * n;
+        d.recx_off = (int64_t)j * n;
         d.n = n;
         d.items_off = j * per;
         d.ntiles = per;
@@ -1002,6 +1055,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
         Job jb{};
         jb.pos = c->pos.as<LinePos>();
         jb.rec = c->rec.as<VRec>() + (size_t)j * n;
+        jb.recx = c->recx.as<VRecX>() + (size_t)j * n;
         jb.out = c->batch_out.as<double>() + (size_t)j * np;
         jb.n = n;
         jb.profile = c->profile;
@@ -1088,8 +1142,9 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
     top.kernel_ms = 1e300;
     int nrep = 0;
     for (int pi = 0; pi < 4; ++pi)
-        for (int order = 0; order < 2; ++order)
+        for (int order = 0; order < 3; ++order)
             for (int pb = 0; pb < 3; ++pb) {
+                if (order == 2 && pb == 0) continue;  // per-XCD queues are persistent
                 c->ppl = ppls[pi];
                 c->order = order;
                 c->persistent_blocks = pers[pb];

@@ -30,6 +30,8 @@
 // points of a wave-tile = 64 * points-per-lane
 #define GX_WAVE 64
 #define GX_WG 256
+// XCDs of an MI355X; workgroups are dispatched to them round-robin
+#define GX_XCDS 8
 
 // per line, per grid: window [ib, ie) clamped to the job window, nearest
 // grid point c = round((nu - start)/dfreq) (as int and exact float)
@@ -38,24 +40,35 @@ struct __attribute__((aligned(16))) LinePos {
     float cf;
 };
 
-// per line, per (T,P): FP32 constants of the profile sum.
-// Voigt:       x_j = fma(c - j, A, B) (= (nu - f_j) / w), y = gammaL / w,
-//              region I:  K*Kg = (Kl*x^2 + Klb) / (x^2 (x^2 + 2y^2 - 1) + b2),
-//              Kl = Kg*0.5641896*y, Klb = Kl*(0.5 + y^2), b2 = (0.5 + y^2)^2,
-//              k15 = smallest |c - j| with |x| + y >= 15 for certain.
-//              Kg = S / w * scale.
-// Voigt-Quad:  A, B, y, Kg as Voigt.
-// bin-Gauss:   A, B as Voigt with 1/w = gG, y = H = gG*dfreq/2,
-//              Kg = S * 0.5/dfreq * scale.
+// per line, per (T,P): FP32 constants of the profile sum, read by the
+// branch-free loop of the profile kernel.
+// Voigt:       x_j = fma(cf - j, A, B) (= (nu - f_j) / w), y = gammaL / w,
+//              region I:  K*Kg = (Kl*x^2 + Klb) / (x^2 (x^2 + e) + b2),
+//              e = 2y^2 - 1, b = 1/2 + y^2, Kl = Kg*0.5641896*y, Klb = Kl*b,
+//              b2 = b^2, Kg = S / w * scale.
+// Voigt-Quad:  cf, A, B, Kg as Voigt.
+// bin-Gauss:   cf, A, B as Voigt with 1/w = gG, Kg = S * 0.5/dfreq * scale.
 struct __attribute__((aligned(32))) VRec {
-    int32_t k15;
-    float A, B, y, Kl, Klb, b2, Kg;
+    float cf, A, B, e, Kl, Klb, b2, Kg;
 };
 
-struct __attribute__((aligned(32))) WorkItem {
-    int32_t lo, hi;  // line range [lo, hi) of the job's (sorted) lines
-    int32_t w0, w1;  // grid points [w0, w1)
+// per line, per (T,P): what the checked loop needs beyond VRec.
+// Voigt: y and k15 = smallest |c - j| with |x| + y >= 15 for certain.
+// bin-Gauss: y = H, half the bin width in units of the width.
+struct __attribute__((aligned(8))) VRecX {
+    float y;
+    int32_t k15;
+};
+
+// One wave-tile of one job: grid points [w0, w1) and the lines [lo, hi)
+// whose windows meet them, split into segments
+//   [lo, s1) checked, [s1, s2) region I over the whole tile (no checks),
+//   [s2, s3) checked (line centres near the tile), [s3, s4) region I, [s4, hi) checked.
+struct __attribute__((aligned(16))) WorkItem {
+    int32_t lo, hi;
+    int32_t w0, w1;
     int32_t job;
+    int32_t s1, s2, s3, s4;
     int32_t pad[3];
 };
 
@@ -63,6 +76,7 @@ struct __attribute__((aligned(32))) WorkItem {
 struct __attribute__((aligned(16))) Job {
     const LinePos* pos;  // job's lines, [0, n)
     const VRec* rec;
+    const VRecX* recx;
     double* out;         // grid (index = grid point)
     int32_t n;
     int32_t profile;
@@ -75,6 +89,7 @@ struct JobDesc {
     int64_t l0;         // first line of the job among the current (nu-sorted) lines
     int64_t pos_off;    // first LinePos of the job
     int64_t rec_off;    // first VRec of the job
+    int64_t recx_off;   // first VRecX of the job
     int32_t n;          // number of lines
     int32_t items_off;  // first work item
     int32_t ntiles;     // number of work items (wave-tiles)

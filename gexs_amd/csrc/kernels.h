@@ -17,16 +17,23 @@ hipError_t line_max(hipStream_t s, const double* nu, const double* E, const doub
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
-                    const unsigned long long* scale_bits, VRec* rec, unsigned long long* neval);
+                    const unsigned long long* scale_bits, VRec* rec, VRecX* recx, int* kmax,
+                    unsigned long long* neval);
+// segment bounds of every work item (after line_rec has produced kmax per job)
+hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const JobDesc* jd,
+                    const LinePos* pos, const int* kmax, int profile);
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
                  uint32_t* item_idx);
 hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, TPParams* tp);
 hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx, int n,
                          WorkItem* out);
+// mode 0: one wave per item; 1: `blocks` persistent workgroups on one counter
+// (xq[0]); 2: `blocks` persistent workgroups on per-XCD queues (xq = 8 counters
+// then 9 item bounds)
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, const WorkItem* items, int nitems,
                          const Job* jobs, const unsigned long long* scale_bits,
-                         unsigned int* counter, int persistent_blocks);
+                         unsigned int* xq, int mode, int blocks);
 hipError_t line_f64(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, int n,
                     const LinePos* pos, const TPParams* tp, LineF64* out,

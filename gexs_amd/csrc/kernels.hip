@@ -156,55 +156,68 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
                                                   const JobDesc* __restrict__ jd,
                                                   const TPParams* __restrict__ tp,
                                                   const unsigned long long* __restrict__ scale_bits,
-                                                  VRec* __restrict__ rec,
+                                                  VRec* __restrict__ rec, VRecX* __restrict__ recx,
+                                                  int* __restrict__ kmax,
                                                   unsigned long long* __restrict__ neval) {
 #pragma clang fp contract(off)
     const int k = blockIdx.y;
     const TPParams p = tp[k];
     const JobDesc d = jd[k];
     const double scale = scale_of(scale_bits[k]);
+    const bool voigt = p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD;
     long long ne = 0;
+    int km = 0;
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < d.n; ii += gridDim.x * blockDim.x) {
         const int64_t i = d.l0 + ii;
         double v = nu[i] == 0.0 ? 1e-6 : nu[i];
         double S = line_strength(p, nu[i], E[i], A[i], g[i]);
         double iw = inv_width(p, nu[i]);
-        LinePos lp = pos[d.pos_off + ii];
-        if (S != 0.0 && lp.ie > lp.ib) ne += lp.ie - lp.ib;
+        const LinePos lp = pos[d.pos_off + ii];
+        const bool reaches = lp.ie > lp.ib;
+        if (S != 0.0 && reaches) ne += lp.ie - lp.ib;
         // offset of the line from its nearest grid point, FP64 (f_c = start + c*dfreq)
-        double fc = p.start + (double)lp.c * p.dfreq;
-        double delta = v - fc;
-        VRec r;
-        double Af = p.dfreq * iw;
-        double Bf = delta * iw;
-        r.A = (float)Af;
-        r.B = (float)Bf;
+        const double fc = p.start + (double)lp.c * p.dfreq;
+        const double delta = v - fc;
+        const double Af = p.dfreq * iw;
+        const double Bf = delta * iw;
         double amp = line_amp(p, S, iw) * scale;
         if (!isfinite(amp)) amp = 0.0;
-        if (p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD) {
-            double y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
-            double b = 0.5 + y * y;
-            double kl = amp * 0.5641896 * y;
-            r.y = (float)y;
+        VRec r;
+        VRecX x;
+        r.cf = lp.cf;
+        r.A = (float)Af;
+        r.B = (float)Bf;
+        r.Kg = (float)amp;
+        if (voigt) {
+            const double y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
+            const double b = 0.5 + y * y;
+            const double kl = amp * 0.5641896 * y;
+            r.e = (float)(2.0 * y * y - 1.0);
             r.Kl = (float)kl;
             r.Klb = (float)(kl * b);
             r.b2 = (float)(b * b);
-            // |x| >= |k| A - |B|; region I needs |x| + y >= 15 (with margin)
-            double kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
-            r.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e9 ? 1000000000 : (int)ceil(kk));
+            x.y = (float)y;
+            // |x| >= |c - j| A - |B|; region I needs |x| + y >= 15 (with margin)
+            const double kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
+            x.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e8 ? 100000000 : (int)ceil(kk));
+            if (reaches) km = max(km, x.k15);
         } else {
-            r.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
+            r.e = 0.0f;
             r.Kl = 0.0f;
             r.Klb = 0.0f;
             r.b2 = 0.0f;
-            r.k15 = 0;
+            x.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
+            x.k15 = 0;
         }
-        r.Kg = (float)amp;
         rec[d.rec_off + ii] = r;
+        recx[d.recx_off + ii] = x;
     }
     ne = wave_sum_ll(ne);
-    if (neval != nullptr && (threadIdx.x & 63) == 0 && ne != 0)
-        atomicAdd(&neval[k], (unsigned long long)ne);
+    for (int o = 32; o > 0; o >>= 1) km = max(km, __shfl_xor(km, o));
+    if ((threadIdx.x & 63) == 0) {
+        if (neval != nullptr && ne != 0) atomicAdd(&neval[k], (unsigned long long)ne);
+        if (kmax != nullptr && km > 0) atomicMax(&kmax[k], km);
+    }
 }
 
 // ---------------------------------------------------------------- k_tiles
@@ -246,6 +259,7 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.w0 = w0;
     it.w1 = w1;
     it.job = job;
+    it.s1 = it.s2 = it.s3 = it.s4 = a;  // all lines checked until k_segments runs
     it.pad[0] = it.pad[1] = it.pad[2] = 0;
     const int slot = d.items_off + t;
     items[slot] = it;
@@ -254,6 +268,61 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
         cost_key[slot] = 0xFFFFFFFFu - c;  // ascending key = heaviest first
         item_idx[slot] = (uint32_t)slot;
     }
+}
+
+// Segment bounds of every work item (WorkItem): with ib and ie
+// non-decreasing in nu, the lines whose window covers the whole tile are
+// [lf, rf), lf = first with ie >= w1, rf = first with ib > w0; among them the
+// lines whose centre c lies within kmax of the tile, [c0, c1), may have
+// points outside Humlicek region I.  kmax = max k15 over the job's lines.
+__global__ __launch_bounds__(256) void k_segments(WorkItem* __restrict__ items, int nitems,
+                                                  const JobDesc* __restrict__ jd,
+                                                  const LinePos* __restrict__ pos,
+                                                  const int* __restrict__ kmax, int voigt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nitems) return;
+    WorkItem it = items[t];
+    int s1 = it.hi, s2 = it.hi, s3 = it.hi, s4 = it.hi;
+    if (voigt && it.lo < it.hi) {
+        const LinePos* jp = pos + jd[it.job].pos_off;
+        int a = it.lo, b = it.hi;
+        while (a < b) {  // lf
+            const int m = (a + b) >> 1;
+            if (jp[m].ie >= it.w1) b = m; else a = m + 1;
+        }
+        const int lf = a;
+        a = it.lo;
+        b = it.hi;
+        while (a < b) {  // rf
+            const int m = (a + b) >> 1;
+            if (jp[m].ib > it.w0) b = m; else a = m + 1;
+        }
+        const int rf = a;
+        if (lf < rf) {
+            const int K = kmax[it.job];
+            const long long cl = (long long)it.w0 - K, ch = (long long)it.w1 + K;
+            a = lf;
+            b = rf;
+            while (a < b) {  // c0
+                const int m = (a + b) >> 1;
+                if ((long long)jp[m].c >= cl) b = m; else a = m + 1;
+            }
+            const int c0 = a;
+            b = rf;
+            while (a < b) {  // c1
+                const int m = (a + b) >> 1;
+                if ((long long)jp[m].c >= ch) b = m; else a = m + 1;
+            }
+            s1 = lf;
+            s2 = c0;
+            s3 = a;
+            s4 = rf;
+        }
+    }
+    items[t].s1 = s1;
+    items[t].s2 = s2;
+    items[t].s3 = s3;
+    items[t].s4 = s4;
 }
 
 // tp[k].Q = Q[q_slot] for jobs whose Q comes from a device partition pass
@@ -287,9 +356,11 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) LinePos* ConstPos;
 typedef const __attribute__((address_space(4))) VRec* ConstRec;
+typedef const __attribute__((address_space(4))) VRecX* ConstRecX;
 #else
 typedef const LinePos* ConstPos;
 typedef const VRec* ConstRec;
+typedef const VRecX* ConstRecX;
 #endif
 
 // Region-I value (times the line amplitude) at x:
@@ -301,113 +372,139 @@ __device__ __forceinline__ float region1(float x, float Kl, float Klb, float e, 
     return num * __builtin_amdgcn_rcpf(den);
 }
 
+constexpr int kChunk = 32;  // lines per uncompensated FP32 partial sum
+
+template <int PPL>
+struct TileAcc {
+    float sum[PPL], cmp[PPL], part[PPL], jf[PPL];
+    int jj[PPL];
+    int n = 0;  // lines in part since the last fold
+    __device__ __forceinline__ void fold() {
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) {
+            kahan_add(sum[q], cmp[q], part[q]);
+            part[q] = 0.0f;
+        }
+        n = 0;
+    }
+    __device__ __forceinline__ void tick() {
+        if (++n == kChunk) fold();
+    }
+};
+
+// lines [a, b) whose window covers the tile and whose every point is in
+// Humlicek region I: no per-line decisions, one 32-B scalar load per line
+template <int PPL>
+__device__ __forceinline__ void voigt_fast(TileAcc<PPL>& t, ConstRec rec, int a, int b) {
+    for (int i = a; i < b; ++i) {
+        const VRec r = rec[i];
+#pragma unroll
+        for (int q = 0; q < PPL; ++q)
+            t.part[q] += region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.Klb, r.e, r.b2);
+        t.tick();
+    }
+}
+
+// any line: window masks per lane, region I inline, regions II-IV per lane
+template <int PROF, int PPL>
+__device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, ConstRec rec,
+                                              ConstRecX recx, int a, int b, int w0, int w1) {
+    for (int i = a; i < b; ++i) {
+        const LinePos lp = pos[i];
+        const VRec r = rec[i];
+        const VRecX rx = recx[i];
+        if (PROF == GX_PROF_VOIGT) {
+            const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
+            if (kmin >= rx.k15) {
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.Klb, r.e, r.b2);
+                    t.part[q] += (t.jj[q] >= lp.ib && t.jj[q] < lp.ie) ? v : 0.0f;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
+                    float v = region1(x, r.Kl, r.Klb, r.e, r.b2);
+                    if (x + rx.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, rx.y);
+                    t.part[q] += (t.jj[q] >= lp.ib && t.jj[q] < lp.ie) ? v : 0.0f;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                if (t.jj[q] >= lp.ib && t.jj[q] < lp.ie) {
+                    const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
+                    float k;
+                    if (PROF == GX_PROF_VOIGT_QUAD) {
+                        const float rr = x * x + rx.y * rx.y;
+                        k = rr < 100.0f ? (float)gx::voigt916_f64((double)x, (double)rx.y)
+                                        : gx::gh11_f32(x, rx.y);
+                    } else {
+                        k = gx::bin_erf_f32(x, rx.y);
+                    }
+                    t.part[q] = fmaf(r.Kg, k, t.part[q]);
+                }
+            }
+        }
+        t.tick();
+    }
+}
+
 template <int PROF, int PPL>
 __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
                                          int lane) {
     const ConstPos pos = (ConstPos)jb.pos;
     const ConstRec rec = (ConstRec)jb.rec;
-    float sum[PPL], cmp[PPL], jf[PPL];
-    int jj[PPL];
+    const ConstRecX recx = (ConstRecX)jb.recx;
+    TileAcc<PPL> t;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
-        sum[q] = 0.0f;
-        cmp[q] = 0.0f;
-        jj[q] = it.w0 + lane + GX_WAVE * q;
-        jf[q] = (float)jj[q];
+        t.sum[q] = 0.0f;
+        t.cmp[q] = 0.0f;
+        t.part[q] = 0.0f;
+        t.jj[q] = it.w0 + lane + GX_WAVE * q;
+        t.jf[q] = (float)t.jj[q];
     }
     const int w0 = it.w0, w1 = it.w1;
-    const int hi = it.hi;
-    constexpr int CH = 32;  // lines per uncompensated partial sum
-    // software pipeline: the record of line i+1 is loaded while line i is evaluated
-    LinePos lp_n;
-    VRec r_n;
-    if (it.lo < hi) {
-        lp_n = pos[it.lo];
-        r_n = rec[it.lo];
+    if (PROF == GX_PROF_VOIGT) {
+        // summation stays in line (nu) order across the segments
+        lines_checked<PROF, PPL>(t, pos, rec, recx, it.lo, it.s1, w0, w1);
+        voigt_fast<PPL>(t, rec, it.s1, it.s2);
+        lines_checked<PROF, PPL>(t, pos, rec, recx, it.s2, it.s3, w0, w1);
+        voigt_fast<PPL>(t, rec, it.s3, it.s4);
+        lines_checked<PROF, PPL>(t, pos, rec, recx, it.s4, it.hi, w0, w1);
+    } else {
+        lines_checked<PROF, PPL>(t, pos, rec, recx, it.lo, it.hi, w0, w1);
     }
-    for (int i0 = it.lo; i0 < hi; i0 += CH) {
-        const int i1 = min(i0 + CH, hi);
-        float part[PPL];
-#pragma unroll
-        for (int q = 0; q < PPL; ++q) part[q] = 0.0f;
-        for (int i = i0; i < i1; ++i) {
-            const LinePos lp = lp_n;
-            const VRec r = r_n;
-            if (i + 1 < hi) {
-                lp_n = pos[i + 1];
-                r_n = rec[i + 1];
-            }
-            const bool full = (lp.ib <= w0) && (lp.ie >= w1);
-            if (PROF == GX_PROF_VOIGT) {
-                const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
-                const float e = fmaf(2.0f * r.y, r.y, -1.0f);
-                if (kmin >= r.k15) {
-                    // every point of the tile is in Humlicek region I
-                    if (full) {
-#pragma unroll
-                        for (int q = 0; q < PPL; ++q)
-                            part[q] += region1(fmaf(lp.cf - jf[q], r.A, r.B), r.Kl, r.Klb, e, r.b2);
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < PPL; ++q) {
-                            const float v = region1(fmaf(lp.cf - jf[q], r.A, r.B), r.Kl, r.Klb, e, r.b2);
-                            part[q] += (jj[q] >= lp.ib && jj[q] < lp.ie) ? v : 0.0f;
-                        }
-                    }
-                    continue;
-                }
-                // line centre near the tile: region I inline, regions II-IV per lane
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    const float x = fabsf(fmaf(lp.cf - jf[q], r.A, r.B));
-                    float v = region1(x, r.Kl, r.Klb, e, r.b2);
-                    if (x + r.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, r.y);
-                    part[q] += (jj[q] >= lp.ib && jj[q] < lp.ie) ? v : 0.0f;
-                }
-                continue;
-            }
-#pragma unroll
-            for (int q = 0; q < PPL; ++q) {
-                if (jj[q] >= lp.ib && jj[q] < lp.ie) {
-                    const float x = fabsf(fmaf(lp.cf - jf[q], r.A, r.B));
-                    float k;
-                    if (PROF == GX_PROF_VOIGT_QUAD) {
-                        const float rr = x * x + r.y * r.y;
-                        k = rr < 100.0f ? (float)gx::voigt916_f64((double)x, (double)r.y)
-                                        : gx::gh11_f32(x, r.y);
-                    } else {
-                        k = gx::bin_erf_f32(x, r.y);
-                    }
-                    part[q] = fmaf(r.Kg, k, part[q]);
-                }
-            }
-            (void)full;
-        }
-#pragma unroll
-        for (int q = 0; q < PPL; ++q) kahan_add(sum[q], cmp[q], part[q]);
-    }
+    t.fold();
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
-        const double v = ((double)sum[q] - (double)cmp[q]) * inv_scale;
-        if (jj[q] < w1) {
+        const double v = ((double)t.sum[q] - (double)t.cmp[q]) * inv_scale;
+        if (t.jj[q] < w1) {
             if (jb.accumulate)
-                jb.out[jj[q]] += v;
+                jb.out[t.jj[q]] += v;
             else
-                jb.out[jj[q]] = v;
+                jb.out[t.jj[q]] = v;
         }
     }
 }
 
 }  // namespace
 
+// mode 0: one wave per work item; 1: persistent waves pulling items from one
+// counter; 2: persistent waves pulling from the queue of their XCD (items in
+// grid order, split into GX_XCDS cost-balanced contiguous ranges, so the waves
+// of one XCD work on neighbouring tiles and share line records in its L2),
+// then from the other queues.  xq = [GX_XCDS counters | GX_XCDS + 1 bounds].
 template <int PROF, int PPL>
 __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
                                                          const unsigned long long* __restrict__ scale_bits,
-                                                         unsigned int* __restrict__ counter) {
+                                                         unsigned int* __restrict__ xq, int mode) {
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (counter == nullptr) {
+    if (mode == 0) {
         const int idx = blockIdx.x * (GX_WG / GX_WAVE) + wv;
         if (idx >= nitems) return;
         const WorkItem it = items[idx];
@@ -415,15 +512,21 @@ __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restr
         run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
         return;
     }
-    // persistent: waves pull items from a counter until the list is drained
-    for (;;) {
-        unsigned int t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1u);
-        const int idx = __builtin_amdgcn_readfirstlane((int)t);
-        if (idx >= nitems) return;
-        const WorkItem it = items[idx];
-        const Job jb = jobs[it.job];
-        run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+    const int nq = mode == 2 ? GX_XCDS : 1;
+    const int home = mode == 2 ? (int)(blockIdx.x % GX_XCDS) : 0;
+    for (int k = 0; k < nq; ++k) {
+        const int q = (home + k) % nq;
+        const int beg = mode == 2 ? (int)xq[GX_XCDS + q] : 0;
+        const int end = mode == 2 ? (int)xq[GX_XCDS + q + 1] : nitems;
+        for (;;) {
+            unsigned int t = 0;
+            if (lane == 0) t = atomicAdd(&xq[q], 1u);
+            const int idx = beg + __builtin_amdgcn_readfirstlane((int)t);
+            if (idx >= end) break;
+            const WorkItem it = items[idx];
+            const Job jb = jobs[it.job];
+            run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+        }
     }
 }
 
@@ -589,11 +692,20 @@ hipError_t line_max(hipStream_t s, const double* nu, const double* E, const doub
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
-                    const unsigned long long* scale_bits, VRec* rec, unsigned long long* neval) {
+                    const unsigned long long* scale_bits, VRec* rec, VRecX* recx, int* kmax,
+                    unsigned long long* neval) {
     if (njobs <= 0 || max_n <= 0) return hipSuccess;
     int gx = std::min(blocks_for(max_n, 256), 2048);
     hipLaunchKernelGGL(k_line_rec, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, gamma, nexp, pos,
-                       jd, tp, scale_bits, rec, neval);
+                       jd, tp, scale_bits, rec, recx, kmax, neval);
+    return hipGetLastError();
+}
+
+hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const JobDesc* jd,
+                    const LinePos* pos, const int* kmax, int profile) {
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 256)), dim3(256), 0, s, items, nitems,
+                       jd, pos, kmax, profile == GX_PROF_VOIGT ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -623,40 +735,39 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
 template <int PROF, int PPL>
 static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitems,
                                  const Job* jobs, const unsigned long long* scale_bits,
-                                 unsigned int* counter, int persistent_blocks) {
+                                 unsigned int* xq, int mode, int blocks) {
     if (nitems <= 0) return hipSuccess;
     const int wpb = GX_WG / GX_WAVE;
-    int nb = persistent_blocks > 0 ? std::min(persistent_blocks, blocks_for(nitems, wpb))
-                                   : blocks_for(nitems, wpb);
+    const int nb = mode == 0 ? blocks_for(nitems, wpb) : std::max(1, std::min(blocks, blocks_for(nitems, wpb)));
     hipLaunchKernelGGL((k_profile_tiles<PROF, PPL>), dim3(nb), dim3(GX_WG), 0, s, items, nitems,
-                       jobs, scale_bits, persistent_blocks > 0 ? counter : nullptr);
+                       jobs, scale_bits, xq, mode);
     return hipGetLastError();
 }
 
 template <int PROF>
 static hipError_t launch_prof(hipStream_t s, int ppl, const WorkItem* items, int nitems,
                               const Job* jobs, const unsigned long long* scale_bits,
-                              unsigned int* counter, int persistent_blocks) {
+                              unsigned int* xq, int mode, int blocks) {
     switch (ppl) {
-        case 1: return launch_tiles_t<PROF, 1>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
-        case 2: return launch_tiles_t<PROF, 2>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
-        case 4: return launch_tiles_t<PROF, 4>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
-        case 8: return launch_tiles_t<PROF, 8>(s, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+        case 1: return launch_tiles_t<PROF, 1>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 2: return launch_tiles_t<PROF, 2>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 4: return launch_tiles_t<PROF, 4>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 8: return launch_tiles_t<PROF, 8>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, const WorkItem* items, int nitems,
                          const Job* jobs, const unsigned long long* scale_bits,
-                         unsigned int* counter, int persistent_blocks) {
+                         unsigned int* xq, int mode, int blocks) {
     switch (profile) {
         case GX_PROF_VOIGT:
-            return launch_prof<GX_PROF_VOIGT>(s, ppl, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+            return launch_prof<GX_PROF_VOIGT>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
         case GX_PROF_VOIGT_QUAD:
-            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
         case GX_PROF_DOPPLER:
         case GX_PROF_GAUSSIAN:
-            return launch_prof<GX_PROF_DOPPLER>(s, ppl, items, nitems, jobs, scale_bits, counter, persistent_blocks);
+            return launch_prof<GX_PROF_DOPPLER>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
         default:
             return hipErrorInvalidValue;
     }

@@ -137,7 +137,7 @@ Server::~Server() {
 
 size_t Server::footprint(const Req& q) const {
     return align_up((size_t)q.n * sizeof(LinePos)) + align_up((size_t)q.n * sizeof(VRec)) +
-           align_up((size_t)q.npoints * 8);
+           align_up((size_t)q.n * sizeof(VRecX)) + align_up((size_t)q.npoints * 8);
 }
 
 int Server::submit(const Request& r, uint64_t* id, std::string* err) {
@@ -242,7 +242,8 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         Req* q = batch[(size_t)k];
         const size_t pos_b = q->off;
         const size_t rec_b = pos_b + align_up((size_t)q->n * sizeof(LinePos));
-        const size_t out_b = rec_b + align_up((size_t)q->n * sizeof(VRec));
+        const size_t recx_b = rec_b + align_up((size_t)q->n * sizeof(VRec));
+        const size_t out_b = recx_b + align_up((size_t)q->n * sizeof(VRecX));
         TPParams& p = tps[(size_t)k];
         p = TPParams{};
         p.T = q->r.T;
@@ -264,6 +265,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         d.l0 = q->l0;
         d.pos_off = (int64_t)(pos_b / sizeof(LinePos));
         d.rec_off = (int64_t)(rec_b / sizeof(VRec));
+        d.recx_off = (int64_t)(recx_b / sizeof(VRecX));
         d.n = (int32_t)q->n;
         d.items_off = items_total;
         d.ntiles = q->ntiles;
@@ -271,6 +273,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         Job& jb = jobs[(size_t)k];
         jb.pos = (const LinePos*)(base + pos_b);
         jb.rec = (const VRec*)(base + rec_b);
+        jb.recx = (const VRecX*)(base + recx_b);
         jb.out = (double*)(base + out_b);
         jb.n = (int32_t)q->n;
         jb.profile = ph_.profile;
@@ -289,8 +292,8 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     const size_t tab_b = tp_b + jd_b + jobs_b + t_b;
     // small arrays: scale, neval (kMaxBatch u64 each), counter, Q, partials
     const size_t sm_scale = 0, sm_neval = kMaxBatch * 8, sm_counter = 2 * kMaxBatch * 8,
-                 sm_q = sm_counter + 256, sm_part = sm_q + kMaxBatch * 8,
-                 sm_b = sm_part + (size_t)kMaxBatch * kQBlocks * 8;
+                 sm_q = sm_counter + 256, sm_kmax = sm_q + kMaxBatch * 8,
+                 sm_part = sm_kmax + kMaxBatch * 8, sm_b = sm_part + (size_t)kMaxBatch * kQBlocks * 8;
     hipError_t e = hipSuccess;
     size_t dummy_cap = 0;
     if (!s.small && (e = grow(&s.small, &dummy_cap, sm_b)) != hipSuccess) goto fail;
@@ -333,6 +336,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         unsigned int* d_counter = (unsigned int*)(sm + sm_counter);
         double* d_Q = (double*)(sm + sm_q);
         double* d_part = (double*)(sm + sm_part);
+        int* d_kmax = (int*)(sm + sm_kmax);
         WorkItem* d_items = (WorkItem*)s.items;
         if ((e = hipEventRecord(s.ev0, st)) != hipSuccess) goto fail;
         if ((e = hipMemcpyAsync(tab, s.stage, tab_b, hipMemcpyHostToDevice, st)) != hipSuccess) goto fail;
@@ -346,14 +350,19 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         if ((e = gxk::line_max(st, v_.nu, v_.E, v_.A, v_.g, (const LinePos*)base, d_jd, nreq, max_n, d_tp,
                                d_scale)) != hipSuccess)
             goto fail;
+        if ((e = gxk::fill_u64(st, (unsigned long long*)d_kmax, (nreq + 1) / 2, 0ull)) != hipSuccess) goto fail;
         if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, (const LinePos*)base, d_jd,
-                               nreq, max_n, d_tp, d_scale, (VRec*)base, d_neval)) != hipSuccess)
+                               nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval)) !=
+            hipSuccess)
             goto fail;
         if ((e = gxk::tiles(st, (const LinePos*)base, d_jd, nreq, max_tiles, d_tp, GX_WAVE * ph_.ppl,
                             d_items, nullptr, nullptr)) != hipSuccess)
             goto fail;
+        if ((e = gxk::segments(st, d_items, items_total, d_jd, (const LinePos*)base, d_kmax, ph_.profile)) !=
+            hipSuccess)
+            goto fail;
         if ((e = gxk::profile_tiles(st, ph_.profile, ph_.ppl, d_items, items_total, d_jobs, d_scale,
-                                    d_counter, 0)) != hipSuccess)
+                                    d_counter, 0, 0)) != hipSuccess)
             goto fail;
         size_t hoff = 0;
         for (int k = 0; k < nreq; ++k) {

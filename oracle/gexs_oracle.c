@@ -8,7 +8,7 @@
  * How each function is pinned (tests/test_oracle*.py):
  *   gxo_partition      two-level closed form; mpmath sum on synthetic states;
  *                      bit-for-bit against compute_partition of the reference
- *                      compiled by oracle/build_ref.sh (when buildable).
+ *                      compiled by `make -C oracle ref` (when buildable).
  *   gxo_select/window  brute force on tiny grids; exact k+0.5 round-half cases;
  *                      clamp at lo/hi; bit-for-bit against the reference build.
  *   gxo_humlic         scipy.special.wofz within the W4 error bound; special
@@ -310,32 +310,71 @@ static int64_t lower_bound_d(const double* a, int64_t n, double v) {
     return lo;
 }
 
+typedef struct {
+    const gxo_params* p;
+    double start, dnu;
+    int32_t lo, hi;
+    int64_t nlines;
+    const double *nu, *E, *A, *gamma, *nexp;
+    const int32_t* g;
+    const int32_t* pts;
+    double* out;
+    int64_t k0, k1;
+} gxo_at_job;
+
+static void* run_at(void* arg) {
+    gxo_at_job* jb = (gxo_at_job*)arg;
+    const gxo_params* p = jb->p;
+    int voigt = (p->profile == GXO_VOIGT || p->profile == GXO_VOIGT_QUAD);
+    for (int64_t k = jb->k0; k < jb->k1; ++k) {
+        int32_t j = jb->pts[k];
+        double fj = jb->start + (double)j * jb->dnu;
+        /* every line whose window holds j has |nu - f_j| <= cut + dnu */
+        double span = p->cutoff + 2.0 * jb->dnu;
+        int64_t i0 = lower_bound_d(jb->nu, jb->nlines, fj - span);
+        double acc = 0.0;
+        for (int64_t i = i0; i < jb->nlines && jb->nu[i] <= fj + span; ++i) {
+            double v = jb->nu[i] == 0.0 ? 1e-6 : jb->nu[i];
+            int32_t ib, ie;
+            gxo_window(v, p->cutoff, jb->start, jb->dnu, jb->lo, jb->hi, &ib, &ie);
+            if (j < ib || j >= ie) continue;
+            double S = gxo_strength(p, jb->nu[i], jb->E[i], jb->A[i], jb->g[i]);
+            if (S == 0.0) continue;
+            double gG = gxo_inv_width(p, jb->nu[i]);
+            double y = voigt ? gxo_gamma_l(p, jb->gamma[i], jb->nexp[i]) * gG : 0.0;
+            acc += line_term(p, v, S, gG, y, fj, jb->dnu);
+        }
+        jb->out[k] = voigt ? acc : acc * (0.5 / jb->dnu);
+    }
+    return NULL;
+}
+
 void gxo_cross_section_at(const gxo_params* p, double start, double dnu, int32_t start_idx,
                           int32_t nwin, int64_t nlines, const double* nu, const double* E,
                           const double* A, const int32_t* g, const double* gamma,
-                          const double* nexp, int64_t npts, const int32_t* pts, double* out) {
-    int voigt = (p->profile == GXO_VOIGT || p->profile == GXO_VOIGT_QUAD);
-    int32_t lo = start_idx, hi = start_idx + nwin;
-    for (int64_t k = 0; k < npts; ++k) {
-        int32_t j = pts[k];
-        double fj = start + (double)j * dnu;
-        /* every line whose window holds j has |nu - f_j| <= cut + dnu */
-        double span = p->cutoff + 2.0 * dnu;
-        int64_t i0 = lower_bound_d(nu, nlines, fj - span);
-        double acc = 0.0;
-        for (int64_t i = i0; i < nlines && nu[i] <= fj + span; ++i) {
-            double v = nu[i] == 0.0 ? 1e-6 : nu[i];
-            int32_t ib, ie;
-            gxo_window(v, p->cutoff, start, dnu, lo, hi, &ib, &ie);
-            if (j < ib || j >= ie) continue;
-            double S = gxo_strength(p, nu[i], E[i], A[i], g[i]);
-            if (S == 0.0) continue;
-            double gG = gxo_inv_width(p, nu[i]);
-            double y = voigt ? gxo_gamma_l(p, gamma[i], nexp[i]) * gG : 0.0;
-            acc += line_term(p, v, S, gG, y, fj, dnu);
-        }
-        out[k] = voigt ? acc : acc * (0.5 / dnu);
+                          const double* nexp, int64_t npts, const int32_t* pts, double* out,
+                          int32_t nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    gxo_at_job* jobs = (gxo_at_job*)calloc((size_t)nthreads, sizeof(gxo_at_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    int64_t block = (npts + nthreads - 1) / nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        gxo_at_job* jb = &jobs[t];
+        jb->p = p; jb->start = start; jb->dnu = dnu;
+        jb->lo = start_idx; jb->hi = start_idx + nwin; jb->nlines = nlines;
+        jb->nu = nu; jb->E = E; jb->A = A; jb->g = g; jb->gamma = gamma; jb->nexp = nexp;
+        jb->pts = pts; jb->out = out;
+        jb->k0 = (int64_t)t * block < npts ? (int64_t)t * block : npts;
+        jb->k1 = jb->k0 + block < npts ? jb->k0 + block : npts;
     }
+    if (nthreads == 1) {
+        run_at(&jobs[0]);
+    } else {
+        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, run_at, &jobs[t]);
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    }
+    free(jobs);
+    free(th);
 }
 
 int64_t gxo_neval(const gxo_params* p, double start, double dnu, int32_t start_idx,

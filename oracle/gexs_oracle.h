@@ -86,7 +86,8 @@ int64_t gxo_cross_section(const gxo_params* p, double start, double dnu, int32_t
 void gxo_cross_section_at(const gxo_params* p, double start, double dnu, int32_t start_idx,
                           int32_t nwin, int64_t nlines, const double* nu, const double* E,
                           const double* A, const int32_t* g, const double* gamma,
-                          const double* nexp, int64_t npts, const int32_t* pts, double* out);
+                          const double* nexp, int64_t npts, const int32_t* pts, double* out,
+                          int32_t nthreads);
 
 /* N_eval only (no profile evaluation). */
 int64_t gxo_neval(const gxo_params* p, double start, double dnu, int32_t start_idx,

@@ -70,7 +70,7 @@ def lib():
     L.gxo_cross_section_at.restype = None
     L.gxo_cross_section_at.argtypes = [P, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                        C.c_int64, _dp, _dp, _dp, _ip, _dp, _dp, C.c_int64,
-                                       _ip, _dp]
+                                       _ip, _dp, C.c_int32]
     L.gxo_neval.restype = C.c_int64
     L.gxo_neval.argtypes = [P, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int64,
                             _dp, _dp, _dp, _ip]
@@ -140,13 +140,14 @@ def cross_section(p, start, dnu, npoints, nu, E, A, g, gamma=None, nexp=None, nt
     return out, neval
 
 
-def cross_section_at(p, start, dnu, npoints, nu, E, A, g, gamma, nexp, pts):
+def cross_section_at(p, start, dnu, npoints, nu, E, A, g, gamma, nexp, pts, nthreads=1):
     """Values at grid points ``pts``; nu must be ascending."""
     nu, E, A, g = _f(nu), _f(E), _f(A), _i(g)
     gamma, nexp, pts = _f(gamma), _f(nexp), _i(pts)
     out = np.empty(pts.shape[0])
     lib().gxo_cross_section_at(C.byref(p), float(start), float(dnu), 0, int(npoints),
-                               nu.shape[0], nu, E, A, g, gamma, nexp, pts.shape[0], pts, out)
+                               nu.shape[0], nu, E, A, g, gamma, nexp, pts.shape[0], pts, out,
+                               int(nthreads))
     return out
 
 

@@ -88,7 +88,7 @@ def test_autotune_picks_a_candidate(batch_mgr):
     tps = table(8)
     ref = m.run_batch(tps, *GRID)
     best, cands = m.autotune(tps, *GRID)
-    assert len(cands) == 24
+    assert len(cands) == 32
     assert best["kernel_ms"] == min(c["kernel_ms"] for c in cands)
     print("autotune best", best, flush=True)
     out = m.run_batch(tps, *GRID)
