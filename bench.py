@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--workload", default="voigt_1e6")
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0: tuned default)")
     ap.add_argument("--order", type=int, default=1)
+    ap.add_argument("--split", type=int, default=1, help="waves per tile (1 or 4)")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -125,7 +126,7 @@ def main():
     steps, warm = a.steps, a.warmup
     m = HipManager(prof)
     if a.ppl:
-        m.set_tuning(a.ppl, a.order)
+        m.set_tuning(a.ppl, a.order, a.split)
     m.InitializeVectors(N, steps + warm)
     m.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF, synth.P_SCALE_BAR,
                           cutoff=spec["cutoff"])
@@ -139,6 +140,8 @@ def main():
         best, cands = m.autotune([(T, P, Q)], start, end, N)
         tuning = {"best": best, "candidates": cands}
         log(f"[bench] autotune: {best}")
+        for c in sorted(cands, key=lambda c: c["kernel_ms"]):
+            log(f"[bench]   candidate {c}")
     for _ in range(warm):
         m.ExecuteCrossSection(N, n, 0, T, P, Q)
     m.sync()

@@ -71,7 +71,7 @@ class TP(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("points_per_lane", C.c_int32), ("order", C.c_int32),
-                ("persistent_blocks", C.c_int32), ("reserved", C.c_int32),
+                ("persistent_blocks", C.c_int32), ("waves_per_tile", C.c_int32),
                 ("kernel_ms", C.c_double)]
 
 
@@ -116,7 +116,7 @@ def load():
         "gexs_timing": (i32, [P, i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(i32)]),
         "gexs_line_windows": (i32, [P, i64, P, P, P]),
         "gexs_pass_lines": (i32, [P, C.POINTER(i64), C.POINTER(i64)]),
-        "gexs_set_tuning": (i32, [P, i32, i32]),
+        "gexs_set_tuning": (i32, [P, i32, i32, i32]),
         "gexs_load_list": (i32, [P, C.POINTER(States), C.POINTER(Transitions)]),
         "gexs_line_count": (i32, [P, C.POINTER(i64)]),
         "gexs_partition": (i32, [P, P, i32, P]),

@@ -160,6 +160,7 @@ struct gexs_ctx {
     int items_span = 0, items_order = -1, items_njobs = 0, n_items = 0;
 
     int ppl = 4;
+    int split = 1;  // waves per tile
     int order = 1;
     int persistent_blocks = 0;
     gexs_stats last{};
@@ -473,7 +474,7 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, c->jd.as<JobDesc>(), c->pos.as<LinePos>(),
                             c->kmax.as<int>(), c->profile));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->items.as<WorkItem>(), c->n_items,
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
                                  c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
                                  c->counter.as<unsigned int>(), launch_mode(c), launch_blocks(c)));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
@@ -805,13 +806,15 @@ int gexs_pass_lines(gexs_ctx* c, int64_t* first, int64_t* count) {
     return GEXS_OK;
 }
 
-int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order) {
+int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order, int waves_per_tile) {
     if (!c) return GEXS_EINVAL;
     if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4 && points_per_lane != 8)
         return c->fail(GEXS_EINVAL, "points_per_lane must be 1, 2, 4 or 8");
     if (order < 0 || order > 2) return c->fail(GEXS_EINVAL, "order must be 0, 1 or 2");
+    if (waves_per_tile != 1 && waves_per_tile != 4) return c->fail(GEXS_EINVAL, "waves_per_tile must be 1 or 4");
     c->ppl = points_per_lane;
     c->order = order;
+    c->split = waves_per_tile;
     return GEXS_OK;
 }
 
@@ -941,7 +944,7 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, size_t jd_b, size_t job
                             c->recx.as<VRecX>(), c->kmax.as<int>(), c->neval.as<unsigned long long>()));
     HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, c->jd.as<JobDesc>(), c->pos.as<LinePos>(),
                             c->kmax.as<int>(), c->profile));
-    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->items.as<WorkItem>(), c->n_items,
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
                                  c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
                                  c->counter.as<unsigned int>(), launch_mode(c), launch_blocks(c)));
     return GEXS_OK;
@@ -1142,10 +1145,12 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
     top.kernel_ms = 1e300;
     int nrep = 0;
     for (int pi = 0; pi < 4; ++pi)
+        for (int split = 1; split <= 4; split += 3)
         for (int order = 0; order < 3; ++order)
             for (int pb = 0; pb < 3; ++pb) {
                 if (order == 2 && pb == 0) continue;  // per-XCD queues are persistent
                 c->ppl = ppls[pi];
+                c->split = split;
                 c->order = order;
                 c->persistent_blocks = pers[pb];
                 double ms = 0.0;
@@ -1158,6 +1163,7 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
                 }
                 gexs_tuning t{};
                 t.points_per_lane = ppls[pi];
+                t.waves_per_tile = split;
                 t.order = order;
                 t.persistent_blocks = pers[pb];
                 t.kernel_ms = ms;
@@ -1166,6 +1172,7 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
                 if (ms < top.kernel_ms) top = t;
             }
     c->ppl = top.points_per_lane;
+    c->split = top.waves_per_tile;
     c->order = top.order;
     c->persistent_blocks = top.persistent_blocks;
     c->use_graph = graph;
@@ -1209,6 +1216,7 @@ int gexs_serve_start(gexs_ctx* c, int max_inflight, int64_t mem_budget) {
     ph.width_conv = c->k.width_conv;
     ph.pressure_mode = c->k.pressure_mode;
     ph.ppl = c->ppl;
+    ph.split = c->split;
     std::unique_ptr<gxs::Server> srv(new (std::nothrow) gxs::Server(c->device, std::move(v), ph, max_inflight,
                                                                     mem_budget));
     if (!srv) return GEXS_ENOMEM;

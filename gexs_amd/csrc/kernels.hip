@@ -372,7 +372,7 @@ __device__ __forceinline__ float region1(float x, float Kl, float Klb, float e, 
     return num * __builtin_amdgcn_rcpf(den);
 }
 
-constexpr int kChunk = 32;  // lines per uncompensated FP32 partial sum
+constexpr int kChunk = 32;  // lines per uncompensated FP32 partial sum (at most kChunk + 3)
 
 template <int PPL>
 struct TileAcc {
@@ -387,32 +387,72 @@ struct TileAcc {
         }
         n = 0;
     }
-    __device__ __forceinline__ void tick() {
-        if (++n == kChunk) fold();
+    __device__ __forceinline__ void tick(int k) {
+        n += k;
+        if (n >= kChunk) fold();
     }
 };
 
+template <int PPL>
+__device__ __forceinline__ void region1_line(TileAcc<PPL>& t, const VRec& r) {
+#pragma unroll
+    for (int q = 0; q < PPL; ++q)
+        t.part[q] += region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.Klb, r.e, r.b2);
+}
+
 // lines [a, b) whose window covers the tile and whose every point is in
-// Humlicek region I: no per-line decisions, one 32-B scalar load per line
+// Humlicek region I: no per-line decisions.  Records come in groups of four
+// (128 B of scalar loads), the next group loading while this one is evaluated.
 template <int PPL>
 __device__ __forceinline__ void voigt_fast(TileAcc<PPL>& t, ConstRec rec, int a, int b) {
-    for (int i = a; i < b; ++i) {
-        const VRec r = rec[i];
-#pragma unroll
-        for (int q = 0; q < PPL; ++q)
-            t.part[q] += region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.Klb, r.e, r.b2);
-        t.tick();
+    int i = a;
+    if (b - a >= 8) {
+        VRec c0 = rec[i], c1 = rec[i + 1], c2 = rec[i + 2], c3 = rec[i + 3];
+        for (; i + 8 <= b; i += 4) {
+            const VRec n0 = rec[i + 4], n1 = rec[i + 5], n2 = rec[i + 6], n3 = rec[i + 7];
+            // keep the loads of the next group ahead of this group's arithmetic
+            __builtin_amdgcn_sched_barrier(0);
+            region1_line(t, c0);
+            region1_line(t, c1);
+            region1_line(t, c2);
+            region1_line(t, c3);
+            t.tick(4);
+            c0 = n0;
+            c1 = n1;
+            c2 = n2;
+            c3 = n3;
+        }
+        region1_line(t, c0);
+        region1_line(t, c1);
+        region1_line(t, c2);
+        region1_line(t, c3);
+        t.tick(4);
+        i += 4;
+    }
+    for (; i < b; ++i) {
+        region1_line(t, rec[i]);
+        t.tick(1);
     }
 }
 
-// any line: window masks per lane, region I inline, regions II-IV per lane
+// any line: window masks per lane, region I inline, regions II-IV per lane;
+// the next line's records load while this one is evaluated
 template <int PROF, int PPL>
 __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, ConstRec rec,
                                               ConstRecX recx, int a, int b, int w0, int w1) {
+    if (a >= b) return;
+    LinePos lpn = pos[a];
+    VRec rn = rec[a];
+    VRecX rxn = recx[a];
     for (int i = a; i < b; ++i) {
-        const LinePos lp = pos[i];
-        const VRec r = rec[i];
-        const VRecX rx = recx[i];
+        const LinePos lp = lpn;
+        const VRec r = rn;
+        const VRecX rx = rxn;
+        if (i + 1 < b) {
+            lpn = pos[i + 1];
+            rn = rec[i + 1];
+            rxn = recx[i + 1];
+        }
         if (PROF == GX_PROF_VOIGT) {
             const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
             if (kmin >= rx.k15) {
@@ -447,17 +487,18 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
                 }
             }
         }
-        t.tick();
+        t.tick(1);
     }
 }
 
+// Kahan sums of the lines [ca, cb) of the item (a sub-range of [lo, hi)) at
+// the 64*PPL points of the tile; lines stay in nu order across the segments
 template <int PROF, int PPL>
-__device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
-                                         int lane) {
+__device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, const Job& jb,
+                                          int lane, int ca, int cb) {
     const ConstPos pos = (ConstPos)jb.pos;
     const ConstRec rec = (ConstRec)jb.rec;
     const ConstRecX recx = (ConstRecX)jb.recx;
-    TileAcc<PPL> t;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
         t.sum[q] = 0.0f;
@@ -466,50 +507,108 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, doub
         t.jj[q] = it.w0 + lane + GX_WAVE * q;
         t.jf[q] = (float)t.jj[q];
     }
+    t.n = 0;
     const int w0 = it.w0, w1 = it.w1;
     if (PROF == GX_PROF_VOIGT) {
-        // summation stays in line (nu) order across the segments
-        lines_checked<PROF, PPL>(t, pos, rec, recx, it.lo, it.s1, w0, w1);
-        voigt_fast<PPL>(t, rec, it.s1, it.s2);
-        lines_checked<PROF, PPL>(t, pos, rec, recx, it.s2, it.s3, w0, w1);
-        voigt_fast<PPL>(t, rec, it.s3, it.s4);
-        lines_checked<PROF, PPL>(t, pos, rec, recx, it.s4, it.hi, w0, w1);
+        const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
+        const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
+        lines_checked<PROF, PPL>(t, pos, rec, recx, ca, s1, w0, w1);
+        voigt_fast<PPL>(t, rec, s1, s2);
+        lines_checked<PROF, PPL>(t, pos, rec, recx, s2, s3, w0, w1);
+        voigt_fast<PPL>(t, rec, s3, s4);
+        lines_checked<PROF, PPL>(t, pos, rec, recx, s4, cb, w0, w1);
     } else {
-        lines_checked<PROF, PPL>(t, pos, rec, recx, it.lo, it.hi, w0, w1);
+        lines_checked<PROF, PPL>(t, pos, rec, recx, ca, cb, w0, w1);
     }
     t.fold();
+}
+
+__device__ __forceinline__ void store_point(const Job& jb, int j, double v) {
+    if (jb.accumulate)
+        jb.out[j] += v;
+    else
+        jb.out[j] = v;
+}
+
+// one wave evaluates the whole item
+template <int PROF, int PPL>
+__device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
+                                         int lane) {
+    TileAcc<PPL> t;
+    sum_lines<PROF, PPL>(t, it, jb, lane, it.lo, it.hi);
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
         const double v = ((double)t.sum[q] - (double)t.cmp[q]) * inv_scale;
-        if (t.jj[q] < w1) {
-            if (jb.accumulate)
-                jb.out[t.jj[q]] += v;
-            else
-                jb.out[t.jj[q]] = v;
-        }
+        if (t.jj[q] < it.w1) store_point(jb, t.jj[q], v);
     }
+}
+
+constexpr int kSplit = GX_WG / GX_WAVE;  // waves of a workgroup
+
+// the kSplit waves of the workgroup take contiguous quarters of the item's
+// lines; their FP64 partial sums are added in wave order through LDS
+template <int PROF, int PPL>
+__device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb,
+                                               double inv_scale, int lane, int wv,
+                                               double* red) {
+    TileAcc<PPL> t;
+    const int n = max(it.hi - it.lo, 0);
+    const int ca = it.lo + (int)(((long long)n * wv) / kSplit);
+    const int cb = it.lo + (int)(((long long)n * (wv + 1)) / kSplit);
+    sum_lines<PROF, PPL>(t, it, jb, lane, ca, cb);
+#pragma unroll
+    for (int q = 0; q < PPL; ++q)
+        red[(size_t)wv * GX_WAVE * PPL + GX_WAVE * q + lane] = (double)t.sum[q] - (double)t.cmp[q];
+    __syncthreads();
+    for (int k = threadIdx.x; k < GX_WAVE * PPL; k += GX_WG) {
+        double v = red[k];
+#pragma unroll
+        for (int w = 1; w < kSplit; ++w) v += red[(size_t)w * GX_WAVE * PPL + k];
+        const int j = it.w0 + k;
+        if (j < it.w1) store_point(jb, j, v * inv_scale);
+    }
+    __syncthreads();  // red is reused by the next item
+}
+
+// block b runs on XCD b % GX_XCDS; map the blocks of one XCD to one
+// contiguous range of logical blocks so neighbouring tiles share its L2
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b % GX_XCDS, k = b / GX_XCDS;
+    const int per = nb / GX_XCDS, rem = nb % GX_XCDS;
+    return x * per + min(x, rem) + k;
 }
 
 }  // namespace
 
-// mode 0: one wave per work item; 1: persistent waves pulling items from one
-// counter; 2: persistent waves pulling from the queue of their XCD (items in
-// grid order, split into GX_XCDS cost-balanced contiguous ranges, so the waves
-// of one XCD work on neighbouring tiles and share line records in its L2),
-// then from the other queues.  xq = [GX_XCDS counters | GX_XCDS + 1 bounds].
-template <int PROF, int PPL>
+// mode 0: one item per wave (SPLIT 1) or per workgroup (SPLIT 4), blocks in
+// XCD-contiguous order; 1: persistent, items pulled from one counter; 2:
+// persistent, items pulled from the queue of the workgroup's XCD (items in
+// grid order, split into GX_XCDS cost-balanced contiguous ranges), then from
+// the other queues.  xq = [GX_XCDS counters | GX_XCDS + 1 bounds].
+template <int PROF, int PPL, int SPLIT>
 __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
                                                          const unsigned long long* __restrict__ scale_bits,
                                                          unsigned int* __restrict__ xq, int mode) {
+    __shared__ double red[SPLIT > 1 ? kSplit * GX_WAVE * PPL : 1];
+    __shared__ int grab;
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
     if (mode == 0) {
-        const int idx = blockIdx.x * (GX_WG / GX_WAVE) + wv;
-        if (idx >= nitems) return;
-        const WorkItem it = items[idx];
-        const Job jb = jobs[it.job];
-        run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+        if (SPLIT == 1) {
+            const int idx = lb * kSplit + wv;
+            if (idx >= nitems) return;
+            const WorkItem it = items[idx];
+            const Job jb = jobs[it.job];
+            run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+        } else {
+            if (lb >= nitems) return;
+            const WorkItem it = items[lb];
+            const Job jb = jobs[it.job];
+            run_item_split<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane, wv,
+                                      red);
+        }
         return;
     }
     const int nq = mode == 2 ? GX_XCDS : 1;
@@ -519,13 +618,25 @@ __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restr
         const int beg = mode == 2 ? (int)xq[GX_XCDS + q] : 0;
         const int end = mode == 2 ? (int)xq[GX_XCDS + q + 1] : nitems;
         for (;;) {
-            unsigned int t = 0;
-            if (lane == 0) t = atomicAdd(&xq[q], 1u);
-            const int idx = beg + __builtin_amdgcn_readfirstlane((int)t);
+            int idx;
+            if (SPLIT == 1) {
+                unsigned int t = 0;
+                if (lane == 0) t = atomicAdd(&xq[q], 1u);
+                idx = beg + __builtin_amdgcn_readfirstlane((int)t);
+            } else {
+                if (threadIdx.x == 0) grab = (int)atomicAdd(&xq[q], 1u);
+                __syncthreads();
+                idx = beg + grab;
+                __syncthreads();  // grab is rewritten by the next pull
+            }
             if (idx >= end) break;
             const WorkItem it = items[idx];
             const Job jb = jobs[it.job];
-            run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+            if (SPLIT == 1)
+                run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+            else
+                run_item_split<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane,
+                                          wv, red);
         }
     }
 }
@@ -732,42 +843,55 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
     return hipGetLastError();
 }
 
-template <int PROF, int PPL>
+template <int PROF, int PPL, int SPLIT>
 static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitems,
                                  const Job* jobs, const unsigned long long* scale_bits,
                                  unsigned int* xq, int mode, int blocks) {
     if (nitems <= 0) return hipSuccess;
-    const int wpb = GX_WG / GX_WAVE;
-    const int nb = mode == 0 ? blocks_for(nitems, wpb) : std::max(1, std::min(blocks, blocks_for(nitems, wpb)));
-    hipLaunchKernelGGL((k_profile_tiles<PROF, PPL>), dim3(nb), dim3(GX_WG), 0, s, items, nitems,
-                       jobs, scale_bits, xq, mode);
+    const int ipb = SPLIT == 1 ? GX_WG / GX_WAVE : 1;  // items per block
+    const int nb = mode == 0 ? blocks_for(nitems, ipb)
+                             : std::max(1, std::min(blocks, blocks_for(nitems, ipb)));
+    hipLaunchKernelGGL((k_profile_tiles<PROF, PPL, SPLIT>), dim3(nb), dim3(GX_WG), 0, s, items,
+                       nitems, jobs, scale_bits, xq, mode);
     return hipGetLastError();
 }
 
-template <int PROF>
-static hipError_t launch_prof(hipStream_t s, int ppl, const WorkItem* items, int nitems,
-                              const Job* jobs, const unsigned long long* scale_bits,
-                              unsigned int* xq, int mode, int blocks) {
+template <int PROF, int SPLIT>
+static hipError_t launch_ppl(hipStream_t s, int ppl, const WorkItem* items, int nitems,
+                             const Job* jobs, const unsigned long long* scale_bits,
+                             unsigned int* xq, int mode, int blocks) {
     switch (ppl) {
-        case 1: return launch_tiles_t<PROF, 1>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
-        case 2: return launch_tiles_t<PROF, 2>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
-        case 4: return launch_tiles_t<PROF, 4>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
-        case 8: return launch_tiles_t<PROF, 8>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t profile_tiles(hipStream_t s, int profile, int ppl, const WorkItem* items, int nitems,
-                         const Job* jobs, const unsigned long long* scale_bits,
+template <int PROF>
+static hipError_t launch_prof(hipStream_t s, int ppl, int split, const WorkItem* items,
+                              int nitems, const Job* jobs, const unsigned long long* scale_bits,
+                              unsigned int* xq, int mode, int blocks) {
+    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
+    if (split == kSplit)
+        return launch_ppl<PROF, kSplit>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
+    return hipErrorInvalidValue;
+}
+
+hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
+                         int nitems, const Job* jobs, const unsigned long long* scale_bits,
                          unsigned int* xq, int mode, int blocks) {
     switch (profile) {
         case GX_PROF_VOIGT:
-            return launch_prof<GX_PROF_VOIGT>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
+            return launch_prof<GX_PROF_VOIGT>(s, ppl, split, items, nitems, jobs, scale_bits, xq, mode, blocks);
         case GX_PROF_VOIGT_QUAD:
-            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
+            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, scale_bits, xq, mode,
+                                                   blocks);
         case GX_PROF_DOPPLER:
         case GX_PROF_GAUSSIAN:
-            return launch_prof<GX_PROF_DOPPLER>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
+            return launch_prof<GX_PROF_DOPPLER>(s, ppl, split, items, nitems, jobs, scale_bits, xq, mode,
+                                                blocks);
         default:
             return hipErrorInvalidValue;
     }

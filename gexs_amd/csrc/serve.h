@@ -50,6 +50,7 @@ struct Physics {
     double gauss_hwhm = 0.1;
     int width_conv = 0, pressure_mode = 1;
     int ppl = 4;
+    int split = 1;  // waves per tile
 };
 
 struct Request {

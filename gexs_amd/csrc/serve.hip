@@ -361,7 +361,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         if ((e = gxk::segments(st, d_items, items_total, d_jd, (const LinePos*)base, d_kmax, ph_.profile)) !=
             hipSuccess)
             goto fail;
-        if ((e = gxk::profile_tiles(st, ph_.profile, ph_.ppl, d_items, items_total, d_jobs, d_scale,
+        if ((e = gxk::profile_tiles(st, ph_.profile, ph_.ppl, ph_.split, d_items, items_total, d_jobs, d_scale,
                                     d_counter, 0, 0)) != hipSuccess)
             goto fail;
         size_t hoff = 0;

@@ -150,8 +150,9 @@ class HipManager:
         self._chk(self._L.gexs_line_windows(self._h, int(n), _ptr(nu), _ptr(ib), _ptr(ie)))
         return nu, ib, ie
 
-    def set_tuning(self, points_per_lane=4, order=1):
-        self._chk(self._L.gexs_set_tuning(self._h, int(points_per_lane), int(order)))
+    def set_tuning(self, points_per_lane=4, order=1, waves_per_tile=1):
+        self._chk(self._L.gexs_set_tuning(self._h, int(points_per_lane), int(order),
+                                          int(waves_per_tile)))
 
     # ------------------------------------------------- resident list / batch
     def load_list(self, lst):
@@ -183,13 +184,15 @@ class HipManager:
             arr[i] = _lib.TP(float(t[0]), float(t[1]), float(t[2]) if len(t) > 2 else 0.0)
         grid = _lib.Grid(float(start), float(end), int(npoints), 0)
         best = _lib.Tuning()
-        rep = (_lib.Tuning * 64)()
+        cap = 128
+        rep = (_lib.Tuning * cap)()
         nrep = C.c_int()
         self._chk(self._L.gexs_autotune(self._h, arr, len(tps), C.byref(grid), C.byref(best), rep,
-                                        64, C.byref(nrep)))
-        conv = lambda t: dict(points_per_lane=t.points_per_lane, order=t.order,  # noqa: E731
+                                        cap, C.byref(nrep)))
+        conv = lambda t: dict(points_per_lane=t.points_per_lane,  # noqa: E731
+                              waves_per_tile=t.waves_per_tile, order=t.order,
                               persistent_blocks=t.persistent_blocks, kernel_ms=t.kernel_ms)
-        return conv(best), [conv(rep[i]) for i in range(min(nrep.value, 64))]
+        return conv(best), [conv(rep[i]) for i in range(min(nrep.value, cap))]
 
     def partition(self, temps):
         T = np.ascontiguousarray(temps, dtype=np.float64)

@@ -143,9 +143,10 @@ int gexs_line_windows(gexs_ctx* ctx, int64_t n, double* nu_sorted, int32_t* ib, 
 int gexs_pass_lines(gexs_ctx* ctx, int64_t* first, int64_t* count);
 
 /* Tuning of the FP32C profile kernel: points per lane (1, 2, 4 or 8; a
- * wave covers 64*ppl grid points) and the dispatch order of the tiles
- * (0 = grid order, 1 = heaviest first). */
-int gexs_set_tuning(gexs_ctx* ctx, int points_per_lane, int order);
+ * wave covers 64*ppl grid points), the dispatch order of the tiles (0 = grid
+ * order, 1 = heaviest first, 2 = per-XCD queues of neighbouring tiles) and
+ * the waves that share one tile's lines (1 or 4). */
+int gexs_set_tuning(gexs_ctx* ctx, int points_per_lane, int order, int waves_per_tile);
 
 /* ------------------------------------------------------ resident line list
  * ExoMol-style list held in HBM: a state table and transitions indexing it
@@ -203,16 +204,16 @@ int gexs_run_batch(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* g
 int gexs_set_launch(gexs_ctx* ctx, int persistent_blocks, int use_graph);
 
 /* Autotuner of the FP32C profile kernel: runs the batch once per candidate
- * (points per lane 1, 2, 4, 8 = wave-tile widths of 64..512 points; grid or
- * heaviest-first tile order; one wave per tile or 4 / 8 persistent
- * workgroups per CU), times the profile kernel with HIP events, and keeps
- * the fastest on the ctx.  report (may be NULL) receives up to report_cap
- * candidates; *n_report their count. */
+ * (points per lane 1, 2, 4, 8 = tile widths of 64..512 points; 1 or 4 waves
+ * per tile; grid, heaviest-first or per-XCD tile order; one launch slot per
+ * tile or 4 / 8 persistent workgroups per CU), times the profile kernel with
+ * HIP events, and keeps the fastest on the ctx.  report (may be NULL)
+ * receives up to report_cap candidates; *n_report their count. */
 typedef struct gexs_tuning {
     int32_t points_per_lane;
     int32_t order;
     int32_t persistent_blocks;
-    int32_t reserved;
+    int32_t waves_per_tile;
     double kernel_ms;
 } gexs_tuning;
 int gexs_autotune(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* grid,

@@ -62,6 +62,35 @@ def test_persistent_equals_wave_per_tile(batch_mgr):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("split", [1, 4])
+def test_launch_modes_agree_bitwise(batch_mgr, split):
+    """For one tile width and split, every launch shape and tile order sums
+    each point in the same order."""
+    L, m = batch_mgr
+    tps = table(6)
+    outs = []
+    for order, pb in [(0, 0), (1, 0), (0, 1024), (1, 2048), (2, 1024)]:
+        m.set_tuning(4, order, split)
+        m.set_launch(pb, False)
+        outs.append(m.run_batch(tps, *GRID))
+    m.set_tuning(4, 1, 1)
+    m.set_launch(0, False)
+    for k, o in enumerate(outs[1:]):
+        assert np.array_equal(o, outs[0]), k
+
+
+def test_split_close_to_single_wave(batch_mgr):
+    L, m = batch_mgr
+    tps = table(6)
+    m.set_tuning(4, 1, 1)
+    a = m.run_batch(tps, *GRID)
+    m.set_tuning(4, 1, 4)
+    b = m.run_batch(tps, *GRID)
+    m.set_tuning(4, 1, 1)
+    # only the grouping of the FP32 partial sums differs
+    assert np.max(np.abs(a - b)) / np.max(a) < 1e-6
+
+
 def test_graph_replays_equal_eager(batch_mgr):
     L, m = batch_mgr
     tps = table(12)
@@ -88,7 +117,8 @@ def test_autotune_picks_a_candidate(batch_mgr):
     tps = table(8)
     ref = m.run_batch(tps, *GRID)
     best, cands = m.autotune(tps, *GRID)
-    assert len(cands) == 32
+    # ppl {1,2,4,8} x waves per tile {1,4} x 8 launch shapes
+    assert len(cands) == 64
     assert best["kernel_ms"] == min(c["kernel_ms"] for c in cands)
     print("autotune best", best, flush=True)
     out = m.run_batch(tps, *GRID)

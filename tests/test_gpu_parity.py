@@ -29,11 +29,12 @@ def log(msg):
 
 
 def run_manager(profile, lines, T, P, Q, start, end, N, cutoff, precision="fp32c", mass=18.0153,
-                width_conv=0, pressure_mode=1, hwhm=0.1, ppl=4, order=1, start_idx=0, nwin=None):
+                width_conv=0, pressure_mode=1, hwhm=0.1, ppl=4, order=1, start_idx=0, nwin=None,
+                split=1):
     nu, E, A, g, gam, nex = lines
     dnu = (end - start) / N
     m = HipManager(profile, precision=precision)
-    m.set_tuning(ppl, order)
+    m.set_tuning(ppl, order, split)
     m.InitializeVectors(N, 1)
     m.InitializeConstants(100.0, T, Q, dnu, mass, P, 296.0, P_SCALE, cutoff=cutoff,
                           width_conv=width_conv, pressure_mode=pressure_mode, gauss_hwhm=hwhm)
@@ -77,16 +78,17 @@ def c1_small():
     return L, L.lines()
 
 
-@pytest.mark.parametrize("ppl", [1, 4, 8])
-def test_voigt_fp32c_vs_oracle(O, c1_small, ppl):
+@pytest.mark.parametrize("ppl,split", [(1, 1), (4, 1), (8, 1), (1, 4), (4, 4), (8, 4)])
+def test_voigt_fp32c_vs_oracle(O, c1_small, ppl, split):
     L, lines = c1_small
     T, P = 296.0, 1.0
     Q = O.partition(L.E, L.g, T)
-    log(f"voigt fp32c ppl={ppl} lines={lines[0].shape[0]}")
-    out, st, _ = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0, ppl=ppl)
+    log(f"voigt fp32c ppl={ppl} split={split} lines={lines[0].shape[0]}")
+    out, st, _ = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0, ppl=ppl,
+                             split=split)
     ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0)
     assert st["n_eval"] == ne
-    check(out, ref, 1e-4, 1e-6, f"voigt fp32c ppl={ppl}")
+    check(out, ref, 1e-4, 1e-6, f"voigt fp32c ppl={ppl} split={split}")
 
 
 def test_voigt_fp64_vs_oracle(O, c1_small):
