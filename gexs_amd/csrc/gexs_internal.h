@@ -43,13 +43,15 @@ struct __attribute__((aligned(16))) LinePos {
 // per line, per (T,P): FP32 constants of the profile sum, read by the
 // branch-free loop of the profile kernel.
 // Voigt:       x_j = fma(cf - j, A, B) (= (nu - f_j) / w), y = gammaL / w,
-//              region I:  K*Kg = (Kl*x^2 + Klb) / (x^2 (x^2 + e) + b2),
-//              e = 2y^2 - 1, b = 1/2 + y^2, Kl = Kg*0.5641896*y, Klb = Kl*b,
-//              b2 = b^2, Kg = S / w * scale.
+//              region I:  K*Kg = Kl (s + 1) / (s^2 + H),  s = x^2 + E2,
+//              E2 = y^2 - 1/2, H = 2y^2, Kl = Kg*0.5641896*y, Kg = S / w * scale.
+//              (Re of 0.5641896 t / (1/2 + t^2), t = y - ix: with b = 1/2 + y^2
+//              its denominator x^4 + (2y^2 - 1) x^2 + b^2 equals s^2 + 2y^2 and
+//              its numerator Kl (x^2 + b) equals Kl (s + 1).)
 // Voigt-Quad:  cf, A, B, Kg as Voigt.
 // bin-Gauss:   cf, A, B as Voigt with 1/w = gG, Kg = S * 0.5/dfreq * scale.
 struct __attribute__((aligned(32))) VRec {
-    float cf, A, B, e, Kl, Klb, b2, Kg;
+    float cf, A, B, Kl, E2, H, Kg, pad;
 };
 
 // per line, per (T,P): what the checked loop needs beyond VRec.
@@ -62,8 +64,10 @@ struct __attribute__((aligned(8))) VRecX {
 
 // One wave-tile of one job: grid points [w0, w1) and the lines [lo, hi)
 // whose windows meet them, split into segments
-//   [lo, s1) checked, [s1, s2) region I over the whole tile (no checks),
-//   [s2, s3) checked (line centres near the tile), [s3, s4) region I, [s4, hi) checked.
+//   [lo, s1) window ends inside the tile, [s1, s2) region I over the whole
+//   tile (no checks), [s2, s3) line centres near the tile (checked),
+//   [s3, s4) region I, [s4, hi) window starts inside the tile.
+// s1 = s4 = hi when no line covers the whole tile (every line checked).
 struct __attribute__((aligned(16))) WorkItem {
     int32_t lo, hi;
     int32_t w0, w1;

@@ -77,6 +77,33 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
     return v;
 }
 
+// Workgroup reductions (256 threads): the result is valid in thread 0.  One
+// atomic per workgroup instead of one per wave keeps same-address atomics
+// off the critical path of the per-line passes.
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) v = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    return v;
+}
+
+__device__ __forceinline__ long long block_sum_ll(long long v, long long* sh) {
+    v = wave_sum_ll(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) v = sh[0] + sh[1] + sh[2] + sh[3];
+    return v;
+}
+
+__device__ __forceinline__ int block_max_i(int v, int* sh) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) v = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
+    return v;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- k_line_pos
@@ -132,8 +159,9 @@ __global__ __launch_bounds__(256) void k_line_max(const double* __restrict__ nu,
         double amp = line_amp(p, S, inv_width(p, nu[l]));
         if (isfinite(amp)) m = fmax(m, amp);
     }
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(&scale_bits[k], m);
+    __shared__ double sh[4];
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) atomic_max_pos(&scale_bits[k], m);
 }
 
 // power-of-two scale so that the largest amplitude maps into [0.5, 1)
@@ -190,31 +218,30 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
         r.Kg = (float)amp;
         if (voigt) {
             const double y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
-            const double b = 0.5 + y * y;
-            const double kl = amp * 0.5641896 * y;
-            r.e = (float)(2.0 * y * y - 1.0);
-            r.Kl = (float)kl;
-            r.Klb = (float)(kl * b);
-            r.b2 = (float)(b * b);
+            r.Kl = (float)(amp * 0.5641896 * y);
+            r.E2 = (float)(y * y - 0.5);
+            r.H = (float)(2.0 * y * y);
             x.y = (float)y;
             // |x| >= |c - j| A - |B|; region I needs |x| + y >= 15 (with margin)
             const double kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
             x.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e8 ? 100000000 : (int)ceil(kk));
             if (reaches) km = max(km, x.k15);
         } else {
-            r.e = 0.0f;
             r.Kl = 0.0f;
-            r.Klb = 0.0f;
-            r.b2 = 0.0f;
+            r.E2 = 0.0f;
+            r.H = 0.0f;
             x.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
             x.k15 = 0;
         }
+        r.pad = 0.0f;
         rec[d.rec_off + ii] = r;
         recx[d.recx_off + ii] = x;
     }
-    ne = wave_sum_ll(ne);
-    for (int o = 32; o > 0; o >>= 1) km = max(km, __shfl_xor(km, o));
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ long long sh_ne[4];
+    __shared__ int sh_km[4];
+    ne = block_sum_ll(ne, sh_ne);
+    km = block_max_i(km, sh_km);
+    if (threadIdx.x == 0) {
         if (neval != nullptr && ne != 0) atomicAdd(&neval[k], (unsigned long long)ne);
         if (kmax != nullptr && km > 0) atomicMax(&kmax[k], km);
     }
@@ -253,76 +280,73 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
         int m = (a + b) >> 1;
         if (jp[m].ib >= w1) b = m; else a = m + 1;
     }
+    const int hi = a;
+    // with ib and ie non-decreasing in nu, the lines whose window covers the
+    // whole tile are [lf, rf): lf = first with ie >= w1, rf = first with ib > w0
+    a = lo;
+    b = hi;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (jp[m].ie >= w1) b = m; else a = m + 1;
+    }
+    const int lf = a;
+    a = lo;
+    b = hi;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (jp[m].ib > w0) b = m; else a = m + 1;
+    }
+    const int rf = a;
     WorkItem it;
     it.lo = lo;
-    it.hi = a;
+    it.hi = hi;
     it.w0 = w0;
     it.w1 = w1;
     it.job = job;
-    it.s1 = it.s2 = it.s3 = it.s4 = a;  // all lines checked until k_segments runs
+    it.s1 = lf < rf ? lf : hi;
+    it.s4 = lf < rf ? rf : hi;
+    it.s2 = it.s3 = it.s4;  // [s1, s4) all region I until k_segments runs
     it.pad[0] = it.pad[1] = it.pad[2] = 0;
     const int slot = d.items_off + t;
     items[slot] = it;
     if (cost_key) {
-        const uint32_t c = (uint32_t)max(a - lo, 0);
+        const uint32_t c = (uint32_t)max(hi - lo, 0);
         cost_key[slot] = 0xFFFFFFFFu - c;  // ascending key = heaviest first
         item_idx[slot] = (uint32_t)slot;
     }
 }
 
-// Segment bounds of every work item (WorkItem): with ib and ie
-// non-decreasing in nu, the lines whose window covers the whole tile are
-// [lf, rf), lf = first with ie >= w1, rf = first with ib > w0; among them the
-// lines whose centre c lies within kmax of the tile, [c0, c1), may have
-// points outside Humlicek region I.  kmax = max k15 over the job's lines.
+// Near-centre segment of every work item: among the lines [s1, s4) that cover
+// the whole tile (set by k_tiles), those whose centre c lies within kmax of
+// the tile, [s2, s3), may have points outside Humlicek region I.  kmax = max
+// k15 over the job's lines (T,P-dependent, from k_line_rec).
 __global__ __launch_bounds__(256) void k_segments(WorkItem* __restrict__ items, int nitems,
                                                   const JobDesc* __restrict__ jd,
                                                   const LinePos* __restrict__ pos,
-                                                  const int* __restrict__ kmax, int voigt) {
+                                                  const int* __restrict__ kmax) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nitems) return;
-    WorkItem it = items[t];
-    int s1 = it.hi, s2 = it.hi, s3 = it.hi, s4 = it.hi;
-    if (voigt && it.lo < it.hi) {
+    const WorkItem it = items[t];
+    int s2 = it.s4, s3 = it.s4;
+    if (it.s1 < it.s4) {
         const LinePos* jp = pos + jd[it.job].pos_off;
-        int a = it.lo, b = it.hi;
-        while (a < b) {  // lf
+        const int K = kmax[it.job];
+        const long long cl = (long long)it.w0 - K, ch = (long long)it.w1 + K;
+        int a = it.s1, b = it.s4;
+        while (a < b) {  // first c >= w0 - K
             const int m = (a + b) >> 1;
-            if (jp[m].ie >= it.w1) b = m; else a = m + 1;
+            if ((long long)jp[m].c >= cl) b = m; else a = m + 1;
         }
-        const int lf = a;
-        a = it.lo;
-        b = it.hi;
-        while (a < b) {  // rf
+        s2 = a;
+        b = it.s4;
+        while (a < b) {  // first c >= w1 + K
             const int m = (a + b) >> 1;
-            if (jp[m].ib > it.w0) b = m; else a = m + 1;
+            if ((long long)jp[m].c >= ch) b = m; else a = m + 1;
         }
-        const int rf = a;
-        if (lf < rf) {
-            const int K = kmax[it.job];
-            const long long cl = (long long)it.w0 - K, ch = (long long)it.w1 + K;
-            a = lf;
-            b = rf;
-            while (a < b) {  // c0
-                const int m = (a + b) >> 1;
-                if ((long long)jp[m].c >= cl) b = m; else a = m + 1;
-            }
-            const int c0 = a;
-            b = rf;
-            while (a < b) {  // c1
-                const int m = (a + b) >> 1;
-                if ((long long)jp[m].c >= ch) b = m; else a = m + 1;
-            }
-            s1 = lf;
-            s2 = c0;
-            s3 = a;
-            s4 = rf;
-        }
+        s3 = a;
     }
-    items[t].s1 = s1;
     items[t].s2 = s2;
     items[t].s3 = s3;
-    items[t].s4 = s4;
 }
 
 // tp[k].Q = Q[q_slot] for jobs whose Q comes from a device partition pass
@@ -363,12 +387,13 @@ typedef const VRec* ConstRec;
 typedef const VRecX* ConstRecX;
 #endif
 
-// Region-I value (times the line amplitude) at x:
-// (Kl x^2 + Kl b) / (x^2 (x^2 + 2y^2 - 1) + b^2), b = 1/2 + y^2
-__device__ __forceinline__ float region1(float x, float Kl, float Klb, float e, float b2) {
-    const float x2 = x * x;
-    const float den = fmaf(x2, x2 + e, b2);
-    const float num = fmaf(x2, Kl, Klb);
+// Region-I value (times the line amplitude) at x: Kl (s + 1) / (s^2 + H),
+// s = x^2 + E2.  In region I (|x| + y >= 15) every term is positive: either
+// y^2 >= 1/2 or x^2 > 200, so s has no cancellation.
+__device__ __forceinline__ float region1(float x, float Kl, float E2, float H) {
+    const float s = fmaf(x, x, E2);
+    const float den = fmaf(s, s, H);
+    const float num = fmaf(s, Kl, Kl);
     return num * __builtin_amdgcn_rcpf(den);
 }
 
@@ -397,7 +422,7 @@ template <int PPL>
 __device__ __forceinline__ void region1_line(TileAcc<PPL>& t, const VRec& r) {
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
-        t.part[q] += region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.Klb, r.e, r.b2);
+        t.part[q] += region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
 }
 
 // lines [a, b) whose window covers the tile and whose every point is in
@@ -435,9 +460,21 @@ __device__ __forceinline__ void voigt_fast(TileAcc<PPL>& t, ConstRec rec, int a,
     }
 }
 
-// any line: window masks per lane, region I inline, regions II-IV per lane;
-// the next line's records load while this one is evaluated
-template <int PROF, int PPL>
+// window masks: which window edges may fall inside the tile
+enum { kMaskNone = 0, kMaskHi = 1, kMaskLo = 2, kMaskBoth = 3 };
+
+template <int MASK>
+__device__ __forceinline__ bool in_window(int j, const LinePos& lp) {
+    if (MASK == kMaskNone) return true;
+    if (MASK == kMaskHi) return j < lp.ie;
+    if (MASK == kMaskLo) return j >= lp.ib;
+    return j >= lp.ib && j < lp.ie;
+}
+
+// lines with per-line decisions: window masks per lane (only the edges MASK
+// names), region I inline, regions II-IV per lane; the next line's records
+// load while this one is evaluated
+template <int PROF, int PPL, int MASK>
 __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, ConstRec rec,
                                               ConstRecX recx, int a, int b, int w0, int w1) {
     if (a >= b) return;
@@ -458,22 +495,22 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
             if (kmin >= rx.k15) {
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) {
-                    const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.Klb, r.e, r.b2);
-                    t.part[q] += (t.jj[q] >= lp.ib && t.jj[q] < lp.ie) ? v : 0.0f;
+                    const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
+                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) {
                     const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
-                    float v = region1(x, r.Kl, r.Klb, r.e, r.b2);
+                    float v = region1(x, r.Kl, r.E2, r.H);
                     if (x + rx.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, rx.y);
-                    t.part[q] += (t.jj[q] >= lp.ib && t.jj[q] < lp.ie) ? v : 0.0f;
+                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
                 }
             }
         } else {
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
-                if (t.jj[q] >= lp.ib && t.jj[q] < lp.ie) {
+                if (in_window<MASK>(t.jj[q], lp)) {
                     const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
                     float k;
                     if (PROF == GX_PROF_VOIGT_QUAD) {
@@ -512,13 +549,19 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
     if (PROF == GX_PROF_VOIGT) {
         const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
         const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
-        lines_checked<PROF, PPL>(t, pos, rec, recx, ca, s1, w0, w1);
-        voigt_fast<PPL>(t, rec, s1, s2);
-        lines_checked<PROF, PPL>(t, pos, rec, recx, s2, s3, w0, w1);
-        voigt_fast<PPL>(t, rec, s3, s4);
-        lines_checked<PROF, PPL>(t, pos, rec, recx, s4, cb, w0, w1);
+        if (it.s1 < it.s4) {
+            // [lo, s1): ib <= w0 < ie < w1; [s1, s4): ib <= w0, ie >= w1;
+            // [s4, hi): w0 < ib < w1 <= ie
+            lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
+            voigt_fast<PPL>(t, rec, s1, s2);
+            lines_checked<PROF, PPL, kMaskNone>(t, pos, rec, recx, s2, s3, w0, w1);
+            voigt_fast<PPL>(t, rec, s3, s4);
+            lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
+        } else {
+            lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
+        }
     } else {
-        lines_checked<PROF, PPL>(t, pos, rec, recx, ca, cb, w0, w1);
+        lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
     }
     t.fold();
 }
@@ -670,9 +713,9 @@ __global__ __launch_bounds__(256) void k_line_f64(const double* __restrict__ nu,
         r.inv_w = iw;
         out[i] = r;
     }
-    ne = wave_sum_ll(ne);
-    if (neval != nullptr && (threadIdx.x & 63) == 0 && ne != 0)
-        atomicAdd(neval, (unsigned long long)ne);
+    __shared__ long long sh_ne[4];
+    ne = block_sum_ll(ne, sh_ne);
+    if (neval != nullptr && threadIdx.x == 0 && ne != 0) atomicAdd(neval, (unsigned long long)ne);
 }
 
 template <int PROF>
@@ -814,9 +857,9 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
 
 hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const JobDesc* jd,
                     const LinePos* pos, const int* kmax, int profile) {
-    if (nitems <= 0) return hipSuccess;
+    if (nitems <= 0 || profile != GX_PROF_VOIGT) return hipSuccess;  // only Voigt reads them
     hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 256)), dim3(256), 0, s, items, nitems,
-                       jd, pos, kmax, profile == GX_PROF_VOIGT ? 1 : 0);
+                       jd, pos, kmax);
     return hipGetLastError();
 }
 

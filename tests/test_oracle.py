@@ -106,6 +106,25 @@ def test_humlic_vs_wofz_and_special_cases(O):
     assert O.humlic(-2.3, 0.4) == O.humlic(2.3, 0.4)
 
 
+def test_region1_s_form_equals_humlicek_region1(O):
+    """The profile kernel evaluates Humlicek region I as Kl (s + 1) / (s^2 + 2y^2)
+    with s = x^2 + y^2 - 1/2 and Kl = 0.5641896 y (the 32-byte line record
+    carries Kl, E2 = y^2 - 1/2 and H = 2y^2).  Over region I this equals the
+    oracle's W4 to FP64 rounding; a wrong constant in E2 or H does not."""
+    rng = np.random.default_rng(15)
+    y = 10 ** rng.uniform(-6, 2.5, 4000)
+    x = np.maximum(15.0 - y, 0.0) + rng.uniform(0, 1, 4000) * 10 ** rng.uniform(0, 4, 4000)
+
+    def s_form(x, y, e2, h):
+        s = x * x + e2
+        return 0.5641896 * y * (s + 1.0) / (s * s + h)
+
+    h = np.array([O.humlic(a, b) for a, b in zip(x, y)])
+    assert np.max(np.abs(s_form(x, y, y * y - 0.5, 2 * y * y) - h) / h) < 1e-12
+    for e2, hh in ((y * y + 0.5, 2 * y * y), (y * y - 0.5, y * y)):
+        assert np.max(np.abs(s_form(x, y, e2, hh) - h) / h) > 1e-6
+
+
 def test_humlic_matches_reference_bit_for_bit(O, R):
     rng = np.random.default_rng(6)
     for x, y in zip(rng.uniform(0, 20, 5000), 10 ** rng.uniform(-5, 2, 5000)):
