@@ -47,6 +47,11 @@ def parse():
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0: tuned default)")
     ap.add_argument("--order", type=int, default=1)
     ap.add_argument("--split", type=int, default=1, help="waves per tile (1 or 4)")
+    ap.add_argument("--persistent", type=int, default=0,
+                    help="persistent workgroups of the profile kernel with --ppl (0: one per tile)")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times with a plain copy kernel "
+                         "(the reference point of the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -104,12 +109,28 @@ def traffic_from_profile(workload):
         return json.load(fh).get("hbm_bytes_per_launch")
 
 
+def calib_copy(mib):
+    """A plain copy kernel of known size: torch's device-to-device copy of
+    mib MiB, five times.  Each launch reads and writes mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copied {mib} MiB x 5")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md)"}))
         sys.exit(2)
+    if a.calib_copy:
+        calib_copy(a.calib_copy)
+        return
     from gexs_amd import synth
     from gexs_amd.manager import HipManager
 
@@ -127,6 +148,7 @@ def main():
     m = HipManager(prof)
     if a.ppl:
         m.set_tuning(a.ppl, a.order, a.split)
+        m.set_launch(a.persistent, False)
     m.InitializeVectors(N, steps + warm)
     m.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF, synth.P_SCALE_BAR,
                           cutoff=spec["cutoff"])
@@ -211,10 +233,15 @@ def main():
             "n_eval_per_step": n_eval,
             "xsec_per_s": steps / elapsed,
             "tiles": st["n_tiles"],
-            "tuning": tuning["best"] if tuning else {"points_per_lane": a.ppl or 4},
+            "tuning": tuning["best"] if tuning else {"points_per_lane": a.ppl or 4,
+                                                     "waves_per_tile": a.split, "order": a.order,
+                                                     "persistent_blocks": a.persistent},
         },
         "roofline": {
-            "bound": "valu",
+            # compute-bound: FP32 vector ALU work against the FP32 peak (the
+            # kernel issues no matrix instructions)
+            "bound": "mfma",
+            "compute": "fp32 VALU",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
