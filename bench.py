@@ -59,6 +59,25 @@ def parse():
     return ap.parse_args()
 
 
+class quiet_stdout:
+    """Sends file descriptor 1 to /dev/null (the reference CPU library prints
+    progress there; stdout carries only the JSON line)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+
+    def __exit__(self, *exc):
+        import ctypes
+        sys.stdout.flush()
+        ctypes.CDLL(None).fflush(None)  # C stdio buffers of the library
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.null)
+
+
 def cpu_baseline(lst, spec, T, P, Q, target_s):
     """Reference CPU path on a contiguous nu slice of the list, all host cores,
     repeated until about target_s seconds of CPU work are timed."""
@@ -209,7 +228,8 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("[bench] timing the reference CPU path on a sample")
-        cpu = cpu_baseline(lst, spec, T, P, Q, a.cpu_seconds)
+        with quiet_stdout():
+            cpu = cpu_baseline(lst, spec, T, P, Q, a.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.4e} evals/s on {cpu['cores']} cores")
 
     out = {
