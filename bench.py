@@ -56,6 +56,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-table", action="store_true",
+                    help="skip the batched (T,P) table (10^7 lines, 120 cross-sections)")
+    ap.add_argument("--table-steps", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--serve-rates", default="10,25,50,100,200,500")
     return ap.parse_args()
 
 
@@ -126,6 +131,184 @@ def traffic_from_profile(workload):
         return None
     with open(f) as fh:
         return json.load(fh).get("hbm_bytes_per_launch")
+
+
+_LISTS = {}
+
+
+def big_list(name):
+    from gexs_amd import synth
+    if name not in _LISTS:
+        _LISTS[name] = synth.workload_list(name)
+    return _LISTS[name]
+
+
+def table_bench(a):
+    """Batched (T,P) table: 120 cross-sections (10 T x 12 P) of the resident
+    10^7-line list over 500-15000 cm^-1 at 0.01 cm^-1, Q(T) on the device,
+    one graph-captured launch sequence per batch, results left in HBM.
+    The plain FP64 path runs a 6-point subset of the same table."""
+    import torch
+    from gexs_amd import synth
+    from gexs_amd.manager import HipManager
+    name = "voigt_tp_table_1e7"
+    spec = synth.WORKLOADS[name]
+    start, end, N = spec["grid"]
+    dnu = (end - start) / N
+    lst = big_list(name)
+    tps = synth.tp_table()
+    sub6 = tps[5::20]
+    m = HipManager("voigt")
+    m.InitializeConstants(100.0, 1000.0, 1.0, dnu, spec["mass"], 1.0, synth.T_REF,
+                          synth.P_SCALE_BAR, cutoff=spec["cutoff"])
+    m.load_list(lst)
+    best, _ = m.autotune(sub6, start, end, N)
+    log(f"[bench] table autotune on 6 (T,P): {best}")
+    m.set_launch(best["persistent_blocks"], True)
+    dev = torch.empty(len(tps) * N, dtype=torch.float64, device="cuda")
+    m.run_batch(tps, start, end, N, out=dev.data_ptr())  # captures the graph
+    m.run_batch(tps, start, end, N, out=dev.data_ptr())
+    torch.cuda.synchronize()
+    m.timing(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(a.table_steps):
+        m.run_batch(tps, start, end, N, out=dev.data_ptr())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    k_ms, _, cnt = m.timing()
+    st = m.stats()
+    n_eval = st["n_eval"]
+    value = n_eval * a.table_steps / el
+    log(f"[bench] table: {len(tps)} x-sections in {el * 1e3 / a.table_steps:.1f} ms per batch, "
+        f"{n_eval:.4e} evals, {value:.4e} evals/s, kernel {k_ms / cnt:.1f} ms")
+    m.close()
+    del dev
+    m64 = HipManager("voigt", precision="fp64")
+    m64.InitializeConstants(100.0, 1000.0, 1.0, dnu, spec["mass"], 1.0, synth.T_REF,
+                            synth.P_SCALE_BAR, cutoff=spec["cutoff"])
+    m64.load_list(lst)
+    out64 = np.empty((len(sub6), N))
+    m64.run_batch(sub6[:1], start, end, N, out=out64[:1])
+    t1 = time.perf_counter()
+    m64.run_batch(sub6, start, end, N, out=out64)
+    e64 = time.perf_counter() - t1
+    v64 = m64.stats()["n_eval"] / e64
+    m64.close()
+    log(f"[bench] table FP64 path on 6 (T,P): {v64:.4e} evals/s")
+    return {"workload": f"{name}: {lst.n_trans} transitions / {lst.n_states} states, 120 (T,P) "
+                        f"(T 200-3000 K x P 1e-5-100 bar, log-spaced), grid {start}-{end} cm-1 x "
+                        f"{N} points, cutoff {spec['cutoff']} cm-1",
+            "value": value, "unit": "evals/s", "xsec_per_s": len(tps) * a.table_steps / el,
+            "ms_per_batch": el * 1e3 / a.table_steps, "n_eval_per_batch": n_eval,
+            "kernel_ms_per_batch": k_ms / cnt, "graph": True, "tuning": best,
+            "fp64_gpu_path": {"value": v64, "unit": "evals/s", "sample": "6 of the 120 (T,P)",
+                              "speedup_fp32c_over_fp64": value / v64}}
+
+
+def serve_requests(rng, n):
+    """Requests of the serving workload: T ~ U[200, 3000] K, P log-uniform in
+    [1e-5, 100] bar, a range of width log-uniform in [100, 5000] cm^-1 placed
+    uniformly inside 500-15000 cm^-1, resolution in {0.01, 0.1, 1.0}."""
+    out = []
+    for _ in range(n):
+        T = rng.uniform(200.0, 3000.0)
+        P = 10.0 ** rng.uniform(-5.0, 2.0)
+        w = float(np.exp(rng.uniform(np.log(100.0), np.log(5000.0))))
+        lo = rng.uniform(500.0, 15000.0 - w)
+        res = float(rng.choice([0.01, 0.1, 1.0]))
+        w = round(w / res) * res
+        out.append((T, P, lo, lo + w, res))
+    return out
+
+
+def serve_bench(a):
+    """Poisson arrivals at each rate against the continuous-batching server
+    over the resident 10^7-line list; latency = completion - arrival (host
+    clock).  The one-request-at-a-time plain FP64 path is timed on the same
+    requests and its FIFO queue is replayed on the same arrival times."""
+    from gexs_amd import synth
+    from gexs_amd.manager import HipManager
+    name = "voigt_tp_table_1e7"
+    spec = synth.WORKLOADS[name]
+    lst = big_list(name)
+    rates = [float(r) for r in a.serve_rates.split(",") if r]
+    rng = np.random.default_rng(20240611)
+    m = HipManager("voigt")
+    m.InitializeConstants(100.0, 1000.0, 1.0, 0.01, spec["mass"], 1.0, synth.T_REF,
+                          synth.P_SCALE_BAR, cutoff=spec["cutoff"])
+    m.load_list(lst)
+    m.set_tuning(2, 1, 4)
+    m.serve_start(max_inflight=4, mem_budget=24 << 30)
+    for r in serve_requests(np.random.default_rng(1), 4):  # first-use costs
+        m.wait(m.submit(*r))
+    sweeps = []
+    all_reqs = []
+    for rate in rates:
+        n = int(max(40, min(400, rate * 4.0)))
+        reqs = serve_requests(rng, n)
+        arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        lat = np.zeros(n)
+        pending = {}
+        i = 0
+        t0 = time.perf_counter()
+        while i < n or pending:
+            now = time.perf_counter() - t0
+            while i < n and arr[i] <= now:
+                pending[m.submit(*reqs[i])] = i
+                i += 1
+            for rid in list(pending):
+                r = m.poll(rid)
+                if r["status"] == 0:
+                    k = pending.pop(rid)
+                    lat[k] = time.perf_counter() - t0 - arr[k]
+                    m.release(rid)
+            if i < n and not pending:
+                time.sleep(max(0.0, min(arr[i] - (time.perf_counter() - t0), 0.002)))
+            else:
+                time.sleep(5e-5)
+        span = time.perf_counter() - t0
+        sweeps.append({"rate": rate, "n": n, "req_per_s": n / span,
+                       "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                       "p99_ms": float(np.percentile(lat, 99) * 1e3), "arrivals": arr,
+                       "mean_points": float(np.mean([round((r[3] - r[2]) / r[4]) for r in reqs]))})
+        all_reqs.append(reqs)
+        log(f"[bench] serve {rate:6.1f} req/s offered: {n / span:7.1f} req/s, "
+            f"p50 {sweeps[-1]['p50_ms']:.2f} ms, p99 {sweeps[-1]['p99_ms']:.2f} ms")
+    m.serve_stop()
+    m.close()
+    # one request at a time on the plain FP64 path: service times, then the
+    # FIFO queue on the same arrivals
+    m64 = HipManager("voigt", precision="fp64")
+    m64.InitializeConstants(100.0, 1000.0, 1.0, 0.01, spec["mass"], 1.0, synth.T_REF,
+                            synth.P_SCALE_BAR, cutoff=spec["cutoff"])
+    m64.load_list(lst)
+    r0 = serve_requests(np.random.default_rng(1), 1)[0]
+    m64.run_batch([(r0[0], r0[1])], r0[2], r0[3], int(round((r0[3] - r0[2]) / r0[4])))
+    for sw, reqs in zip(sweeps, all_reqs):
+        svc = []
+        for (T, P, lo, hi, res) in reqs:
+            npt = int(round((hi - lo) / res))
+            t1 = time.perf_counter()
+            m64.run_batch([(T, P)], lo, hi, npt)
+            svc.append(time.perf_counter() - t1)
+        arr = sw.pop("arrivals")
+        fin, lat = 0.0, []
+        for t_a, s in zip(arr, svc):
+            fin = max(fin, t_a) + s
+            lat.append(fin - t_a)
+        sw["fp64_one_at_a_time"] = {"req_per_s": len(reqs) / max(fin, arr[-1]),
+                                    "capacity_req_per_s": len(svc) / sum(svc),
+                                    "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                                    "p99_ms": float(np.percentile(lat, 99) * 1e3)}
+        log(f"[bench] fp64 one-at-a-time at {sw['rate']:6.1f} req/s: "
+            f"p50 {sw['fp64_one_at_a_time']['p50_ms']:.2f} ms, "
+            f"p99 {sw['fp64_one_at_a_time']['p99_ms']:.2f} ms, capacity "
+            f"{sw['fp64_one_at_a_time']['capacity_req_per_s']:.1f} req/s")
+    m64.close()
+    return {"workload": f"{name} list resident; T U[200,3000] K, P log-U[1e-5,100] bar, range "
+                        f"width log-U[100,5000] cm-1 in 500-15000, resolution in "
+                        f"{{0.01, 0.1, 1.0}} cm-1, Poisson arrivals",
+            "max_inflight": 4, "sweep": sweeps}
 
 
 def calib_copy(mib):
@@ -225,6 +408,10 @@ def main():
         log(f"[bench] plain FP64 path {v64:.4e} evals/s ({value / v64:.2f}x slower than FP32C)")
         m64.close()
 
+    m.close()
+    table = None if a.no_table else table_bench(a)
+    served = None if a.no_serve else serve_bench(a)
+
     cpu = None
     if not a.no_cpu:
         log("[bench] timing the reference CPU path on a sample")
@@ -273,6 +460,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "fp64_gpu_path": fp64,
+        "batched": table,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

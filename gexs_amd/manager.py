@@ -201,11 +201,17 @@ class HipManager:
         return Q
 
     def run_batch(self, tps, start, end, npoints, out=None):
-        """Cross-sections at [(T, P[, Q]), ...] over grid [start, end) with npoints."""
+        """Cross-sections at [(T, P[, Q]), ...] over grid [start, end) with npoints.
+        ``out`` is a (len(tps), npoints) float64 array, or an int: the device
+        address of len(tps)*npoints doubles (e.g. a CUDA tensor's data_ptr())."""
         arr = (_lib.TP * len(tps))()
         for i, t in enumerate(tps):
             arr[i] = _lib.TP(float(t[0]), float(t[1]), float(t[2]) if len(t) > 2 else 0.0)
         grid = _lib.Grid(float(start), float(end), int(npoints), 0)
+        if isinstance(out, int):
+            self._chk(self._L.gexs_run_batch(self._h, arr, len(tps), C.byref(grid),
+                                             C.c_void_p(out)))
+            return out
         if out is None:
             out = np.empty((len(tps), int(npoints)), dtype=np.float64)
         self._chk(self._L.gexs_run_batch(self._h, arr, len(tps), C.byref(grid), _ptr(out)))
@@ -240,6 +246,9 @@ class HipManager:
         if release:
             self._chk(self._L.gexs_release(self._h, C.c_uint64(rid)))
         return out
+
+    def release(self, rid):
+        self._chk(self._L.gexs_release(self._h, C.c_uint64(rid)))
 
     def serve_stop(self):
         self._chk(self._L.gexs_serve_stop(self._h))

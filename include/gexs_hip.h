@@ -224,7 +224,7 @@ int gexs_autotune(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* gr
  * footprint as they arrive and run while others are on the device.
  * gexs_serve_start: max_inflight concurrent batches (one stream each), a
  * device arena of mem_budget bytes from which each admitted request takes
- * 48 B per selected line + 8 B per grid point until it completes.  The grid
+ * 56 B per selected line + 8 B per grid point until it completes.  The grid
  * of a request has round((end - start)/dfreq) points with step
  * (end - start)/npoints.  While serving, only submit / poll / wait /
  * release / serve_stop may be called on the ctx (the list is read-only). */
