@@ -24,7 +24,7 @@ EXPORTS = [
     "gexs_transfer_freq", "gexs_transfer_lines", "gexs_transfer_lines_device", "gexs_execute",
     "gexs_transfer_results", "gexs_ready", "gexs_sync", "gexs_last_error", "gexs_last_stats",
     "gexs_timing", "gexs_line_windows", "gexs_pass_lines", "gexs_set_tuning", "gexs_load_list", "gexs_line_count",
-    "gexs_partition", "gexs_run_batch",
+    "gexs_partition", "gexs_run_batch", "gexs_batch_device_out",
     "gexs_set_launch", "gexs_autotune",
     "gexs_serve_start", "gexs_submit", "gexs_poll", "gexs_wait", "gexs_release",
     "gexs_serve_stop",
@@ -121,6 +121,7 @@ def load():
         "gexs_line_count": (i32, [P, C.POINTER(i64)]),
         "gexs_partition": (i32, [P, P, i32, P]),
         "gexs_run_batch": (i32, [P, C.POINTER(TP), i32, C.POINTER(Grid), P]),
+        "gexs_batch_device_out": (i32, [P, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
         "gexs_set_launch": (i32, [P, i32, i32]),
         "gexs_autotune": (i32, [P, C.POINTER(TP), i32, C.POINTER(Grid), C.POINTER(Tuning),
                                 C.POINTER(Tuning), i32, C.POINTER(i32)]),

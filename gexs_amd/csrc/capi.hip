@@ -951,7 +951,7 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, size_t jd_b, size_t job
 }
 
 int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid, double* out) {
-    if (!c || !tp || ntp <= 0 || !grid || !out || grid->npoints <= 0 || !(grid->end > grid->start))
+    if (!c || !tp || ntp <= 0 || !grid || grid->npoints <= 0 || !(grid->end > grid->start))
         return GEXS_EINVAL;
     if (!c->have_k) return c->fail(GEXS_ESTATE, "run_batch before init_constants");
     if (c->server) return c->fail(GEXS_ESTATE, "run_batch while serving");
@@ -1006,7 +1006,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
             if ((rc = run_fp64(c, t, c->batch_out.as<double>() + (size_t)i * np, i == 0)) != GEXS_OK) return rc;
         }
         HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
-        HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
+        if (out) HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         unsigned long long ne = 0;
         HIPCHK(c, hipMemcpy(&ne, c->neval.p, 8, hipMemcpyDeviceToHost));
@@ -1105,7 +1105,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     }
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
     c->last.n_tiles = c->n_items;
-    HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
+    if (out) HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<unsigned long long> ne((size_t)ntp, 0ull);
     HIPCHK(c, hipMemcpy(ne.data(), c->neval.p, ne.size() * 8, hipMemcpyDeviceToHost));
@@ -1116,6 +1116,13 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) == hipSuccess) c->last.kernel_ms = ms;
     if (hipEventElapsedTime(&ms, c->ev_start, c->ev_k1) == hipSuccess) c->last.total_ms = ms;
     c->last_neval_pending = false;
+    return GEXS_OK;
+}
+
+int gexs_batch_device_out(gexs_ctx* c, const double** out, int64_t* count) {
+    if (!c || !out) return GEXS_EINVAL;
+    *out = c->batch_out.as<double>();
+    if (count) *count = (int64_t)(c->batch_out.bytes / 8);
     return GEXS_OK;
 }
 

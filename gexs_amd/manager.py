@@ -194,6 +194,12 @@ class HipManager:
                               persistent_blocks=t.persistent_blocks, kernel_ms=t.kernel_ms)
         return conv(best), [conv(rep[i]) for i in range(min(nrep.value, cap))]
 
+    def batch_device_out(self):
+        """(device address, capacity in doubles) of the last run_batch's results."""
+        p, n = C.c_void_p(), C.c_int64()
+        self._chk(self._L.gexs_batch_device_out(self._h, C.byref(p), C.byref(n)))
+        return p.value, n.value
+
     def partition(self, temps):
         T = np.ascontiguousarray(temps, dtype=np.float64)
         Q = np.empty_like(T)
@@ -202,12 +208,16 @@ class HipManager:
 
     def run_batch(self, tps, start, end, npoints, out=None):
         """Cross-sections at [(T, P[, Q]), ...] over grid [start, end) with npoints.
-        ``out`` is a (len(tps), npoints) float64 array, or an int: the device
-        address of len(tps)*npoints doubles (e.g. a CUDA tensor's data_ptr())."""
+        ``out`` is a (len(tps), npoints) float64 array, an int (the device
+        address of len(tps)*npoints doubles), or "device": the results stay in
+        the library's device buffer (batch_device_out) and None is returned."""
         arr = (_lib.TP * len(tps))()
         for i, t in enumerate(tps):
             arr[i] = _lib.TP(float(t[0]), float(t[1]), float(t[2]) if len(t) > 2 else 0.0)
         grid = _lib.Grid(float(start), float(end), int(npoints), 0)
+        if isinstance(out, str) and out == "device":
+            self._chk(self._L.gexs_run_batch(self._h, arr, len(tps), C.byref(grid), None))
+            return None
         if isinstance(out, int):
             self._chk(self._L.gexs_run_batch(self._h, arr, len(tps), C.byref(grid),
                                              C.c_void_p(out)))

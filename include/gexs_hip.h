@@ -192,10 +192,15 @@ typedef struct gexs_tp {
 /* Cross-sections of the resident list at ntp (T,P) points over one grid, in
  * one launch sequence (partition function, records, one profile kernel over
  * all (T,P) x tiles).  out (host or device pointer) receives ntp*npoints
- * doubles, row-major by (T,P).  Synchronous.  Lines are selected by
- * start - half_width <= nu <= end + half_width. */
+ * doubles, row-major by (T,P); with out == NULL they stay in the ctx's
+ * device buffer (gexs_batch_device_out).  Synchronous.  Lines are selected
+ * by start - half_width <= nu <= end + half_width. */
 int gexs_run_batch(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* grid,
                    double* out);
+/* Device address of the last run_batch's results (ntp*npoints doubles) and
+ * the buffer's capacity in doubles; valid until the next run_batch or
+ * gexs_destroy. */
+int gexs_batch_device_out(gexs_ctx* ctx, const double** out, int64_t* count);
 /* Launch shape of batched passes: persistent_blocks > 0 runs the profile
  * kernel as that many workgroups pulling tiles from a counter (0: one wave
  * per tile); use_graph != 0 captures a batch's device work into a HIP graph

@@ -91,6 +91,23 @@ def test_split_close_to_single_wave(batch_mgr):
     assert np.max(np.abs(a - b)) / np.max(a) < 1e-6
 
 
+def test_device_resident_output(batch_mgr):
+    """out="device" leaves the results in the library's buffer, equal to the
+    host copy of the same batch."""
+    import ctypes as C
+    L, m = batch_mgr
+    tps = table(5)
+    host = m.run_batch(tps, *GRID)
+    assert m.run_batch(tps, *GRID, out="device") is None
+    ptr, cap = m.batch_device_out()
+    assert ptr and cap >= host.size
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime the library is linked to
+    back = np.empty_like(host)
+    assert hip.hipMemcpy(back.ctypes.data_as(C.c_void_p), C.c_void_p(ptr),
+                         C.c_size_t(host.nbytes), 2) == 0  # hipMemcpyDeviceToHost
+    assert np.array_equal(back, host)
+
+
 def test_graph_replays_equal_eager(batch_mgr):
     L, m = batch_mgr
     tps = table(12)
