@@ -2,9 +2,10 @@
 
 The 10^6-line Voigt workload is compared point by point over the whole grid
 (the multi-threaded oracle finishes in seconds).  For the 10^7-line (T,P)
-table the oracle is evaluated at sampled points: 10^4 random grid points, every
-point within 1 cm^-1 of the 100 strongest lines (they set the spectrum peak)
-and three random 10 cm^-1 windows, whose sums stand in for the band strength.
+table and the 10^8-line list the oracle is evaluated at sampled points: 10^4
+random grid points, every point within 1 cm^-1 of the 100 strongest lines
+(they set the spectrum peak) and three random 10 cm^-1 windows, whose sums
+stand in for the band strength.
 """
 import os
 
@@ -60,40 +61,71 @@ def sample_points(rng, N, start, dnu, nu_strong, windows=3):
     return allp, win
 
 
-@pytest.mark.parametrize("k", [0, 58, 119])
-def test_tp_table_1e7_sampled(O, k):
-    spec = synth.WORKLOADS["voigt_tp_table_1e7"]
-    L = synth.workload_list("voigt_tp_table_1e7")
+def sampled_check(O, L, spec, T, P, out, seed):
+    """Oracle at sampled points of the grid; returns (peak-relative error,
+    worst relative error of the three 10 cm^-1 window sums)."""
     start, end, N = spec["grid"]
     dnu = (end - start) / N
-    T, P = synth.tp_table()[k]
-    m = HipManager("voigt")
-    m.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, 296.0, P_SCALE, cutoff=25.0)
-    m.load_list(L)
-    out = m.run_batch([(T, P)], start, end, N)[0]
     nu, E, _, g, _, _ = L.lines()
     A, gam, nex = L.A.astype(float), L.gamma.astype(float), L.nexp.astype(float)
     o = np.argsort(nu, kind="stable")
+    keep = (nu[o] >= start - 100.0) & (nu[o] <= end + 100.0) & (nu[o] != 0)
+    o = o[keep]
     nu, E, A, g, gam, nex = nu[o], E[o], A[o], g[o], gam[o], nex[o]
-    keep = (nu >= start - 100.0) & (nu <= end + 100.0) & (nu != 0)
-    nu, E, A, g, gam, nex = nu[keep], E[keep], A[keep], g[keep], gam[keep], nex[keep]
+    del o, keep
     Q = O.partition(L.E, L.g, T)
     p = O.params("voigt", T=T, P=P, Q=Q, mass=spec["mass"], cutoff=25.0, p_scale=P_SCALE)
     S, _, _ = O.line_params(p, nu, E, A, g, gam, nex)
     inside = (nu > start + 1) & (nu < end - 1)
     strong = nu[inside][np.argsort(S[inside])[-100:]]
-    rng = np.random.default_rng(1000 + k)
+    del S, inside
+    rng = np.random.default_rng(seed)
     pts, wins = sample_points(rng, N, start, dnu, strong)
     ref = O.cross_section_at(p, start, dnu, N, nu, E, A, g, gam, nex, pts, nthreads=NTHREADS)
     got = out[pts]
     err = np.max(np.abs(got - ref)) / ref.max()
-    log(f"T={T:.1f} P={P:.2e}: {pts.shape[0]} points, peak-rel {err:.3e}")
-    assert err <= 1e-4
-    # window sums (band strength over 10 cm^-1 windows)
     pos = {j: i for i, j in enumerate(pts)}
+    wrel = 0.0
     for w in wins:
         r = ref[[pos[j] for j in w]].sum()
-        s = out[w].sum()
         if r > 0:
-            log(f"  window {w[0]}: rel {abs(s - r) / r:.2e}")
-            assert abs(s - r) / r <= 1e-6
+            wrel = max(wrel, abs(out[w].sum() - r) / r)
+    log(f"T={T:.1f} P={P:.2e}: {pts.shape[0]} points, peak-rel {err:.3e}, window sums {wrel:.2e}")
+    return err, wrel
+
+
+def test_stream_1e8_sampled(O):
+    """The 10^8-transition hot list (2e6 states, FP32 A, T = 2000 K, P = 0.1 bar,
+    grid 100-30000 cm^-1 at 0.01): the whole selected list resident in HBM,
+    one pass."""
+    name = "voigt_1e8_stream"
+    spec = synth.WORKLOADS[name]
+    L = synth.workload_list(name)
+    log(f"{name}: {L.n_trans} transitions generated")
+    start, end, N = spec["grid"]
+    T, P = spec["T"], spec["P"]
+    m = HipManager("voigt")
+    m.InitializeConstants(100.0, T, 1.0, (end - start) / N, spec["mass"], P, 296.0, P_SCALE,
+                          cutoff=25.0)
+    m.load_list(L)
+    out = m.run_batch([(T, P)], start, end, N)[0]
+    log(f"N_eval {m.stats()['n_eval']:.4e}, {m.stats()['n_lines']} lines selected")
+    m.close()
+    err, wrel = sampled_check(O, L, spec, T, P, out, 77)
+    assert err <= 1e-4 and wrel <= 1e-6
+
+
+@pytest.mark.parametrize("k", [0, 58, 119])
+def test_tp_table_1e7_sampled(O, k):
+    spec = synth.WORKLOADS["voigt_tp_table_1e7"]
+    L = synth.workload_list("voigt_tp_table_1e7")
+    start, end, N = spec["grid"]
+    T, P = synth.tp_table()[k]
+    m = HipManager("voigt")
+    m.InitializeConstants(100.0, T, 1.0, (end - start) / N, spec["mass"], P, 296.0, P_SCALE,
+                          cutoff=25.0)
+    m.load_list(L)
+    out = m.run_batch([(T, P)], start, end, N)[0]
+    m.close()
+    err, wrel = sampled_check(O, L, spec, T, P, out, 1000 + k)
+    assert err <= 1e-4 and wrel <= 1e-6
