@@ -459,6 +459,40 @@ def main():
         m64.close()
 
     m.close()
+
+    # PCIe-inclusive: the BaseManager boundary hands over host buffers, so
+    # time steps that also move the line arrays in (TransferVectors, which
+    # sorts them on the device) and the grid out (TransferResults)
+    nu_h, E_h, A_h, g_h, gam_h, nex_h = lst.lines()
+    mp = HipManager(prof)
+    if tuning:
+        b = tuning["best"]
+        mp.set_tuning(b["points_per_lane"], b["order"], b["waves_per_tile"])
+        mp.set_launch(b["persistent_blocks"], False)
+    elif a.ppl:
+        mp.set_tuning(a.ppl, a.order, a.split)
+        mp.set_launch(a.persistent, False)
+    mp.InitializeVectors(N, 1)
+    mp.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF, synth.P_SCALE_BAR,
+                           cutoff=spec["cutoff"])
+    freq = start + np.arange(N, dtype=np.float64) * dnu
+    mp.TransferFreq(freq)
+    host_out = np.zeros(N)
+    pcie_steps = 5
+    for k in range(pcie_steps + 1):
+        if k == 1:
+            t2 = time.perf_counter()
+        mp.TransferVectors(nu_h.shape[0], E_h, nu_h, A_h, g_h, gam_h, nex_h)
+        mp.ExecuteCrossSection(N, nu_h.shape[0], 0, T, P, Q)
+        mp.TransferResults(freq, host_out, N, 0, accumulate=False)
+    e2 = time.perf_counter() - t2
+    pcie = {"value": n_eval * pcie_steps / e2, "unit": "evals/s",
+            "ms_per_step": e2 * 1e3 / pcie_steps,
+            "step": "TransferVectors (host FP64 line arrays, device nu sort) + "
+                    "ExecuteCrossSection + TransferResults (host FP64 grid)"}
+    log(f"[bench] PCIe-inclusive step {pcie['ms_per_step']:.2f} ms, {pcie['value']:.4e} evals/s")
+    mp.close()
+
     table = None if a.no_table else table_bench(a)
     served = None if a.no_serve else serve_bench(a)
     stream = stream_bench(a) if a.stream else None
@@ -511,6 +545,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "fp64_gpu_path": fp64,
+        "pcie_inclusive": pcie,
         "batched": table,
         "served": served,
         "stream": stream,
