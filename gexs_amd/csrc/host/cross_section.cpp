@@ -137,21 +137,23 @@ CrossSection::CrossSection(const Input& in, StateReader* reader, BaseManager* ma
 
 CrossSection::~CrossSection() {}
 
+std::unique_ptr<StateReader> MakeReader(const Input& in) {
+    if (in.file_type == FileType::EXOMOL)
+        return std::unique_ptr<StateReader>(new ExomolStateReader(
+            in.state_file, in.broadener_files.empty() ? "" : in.broadener_files[0], in.gamma,
+            in.gamma_n));
+    std::vector<std::pair<double, double>> tab;
+    if (!in.partition_file.empty()) {
+        std::ifstream f(in.partition_file);
+        double t, q;
+        while (f >> t >> q) tab.push_back({t, q});
+    }
+    return std::unique_ptr<StateReader>(new HitranStateReader(in.mixture, in.partition, tab));
+}
+
 void CrossSection::Initialize() {
     if (!reader_) {
-        if (in_.file_type == FileType::EXOMOL) {
-            own_reader_.reset(new ExomolStateReader(
-                in_.state_file, in_.broadener_files.empty() ? "" : in_.broadener_files[0],
-                in_.gamma, in_.gamma_n));
-        } else {
-            std::vector<std::pair<double, double>> tab;
-            if (!in_.partition_file.empty()) {
-                std::ifstream f(in_.partition_file);
-                double t, q;
-                while (f >> t >> q) tab.push_back({t, q});
-            }
-            own_reader_.reset(new HitranStateReader(in_.mixture, in_.partition, tab));
-        }
+        own_reader_ = MakeReader(in_);
         reader_ = own_reader_.get();
     }
     for (auto& tp : tp_) tp.partition = reader_->ComputePartition(tp.temperature);  // :91-98

@@ -26,6 +26,148 @@ FILE* open_or_throw(const std::string& f) {
 }
 }  // namespace
 
+// ------------------------------------------------------------ line sources
+// bzlib 1.0 decompression API (the system libbz2.so.1 is linked; the image
+// has no bzlib.h, so the public declarations are restated here)
+extern "C" {
+struct gx_bz_stream {
+    char* next_in;
+    unsigned int avail_in;
+    unsigned int total_in_lo32, total_in_hi32;
+    char* next_out;
+    unsigned int avail_out;
+    unsigned int total_out_lo32, total_out_hi32;
+    void* state;
+    void* (*bzalloc)(void*, int, int);
+    void (*bzfree)(void*, void*);
+    void* opaque;
+};
+int BZ2_bzDecompressInit(gx_bz_stream* strm, int verbosity, int small);
+int BZ2_bzDecompress(gx_bz_stream* strm);
+int BZ2_bzDecompressEnd(gx_bz_stream* strm);
+}
+namespace {
+constexpr int kBzOk = 0, kBzStreamEnd = 4;
+}
+
+class LineSource {
+   public:
+    virtual ~LineSource() {}
+    // the next line without its newline; false at the end of the file
+    virtual bool next(std::string& line) = 0;
+};
+
+namespace {
+class TextLines : public LineSource {
+   public:
+    explicit TextLines(const std::string& f) : fp_(open_or_throw(f)) {}
+    ~TextLines() override { std::fclose(fp_); }
+    bool next(std::string& line) override {
+        line.clear();
+        char buf[512];
+        while (std::fgets(buf, sizeof buf, fp_)) {
+            const size_t n = std::strlen(buf);
+            if (n && buf[n - 1] == '\n') {
+                line.append(buf, n - 1);
+                return true;
+            }
+            line.append(buf, n);
+        }
+        return !line.empty();
+    }
+
+   private:
+    FILE* fp_;
+};
+
+// bzip2 file -> text lines.  A file may hold several concatenated bzip2
+// streams (parallel compressors write them): at the end of one stream the
+// decoder restarts on the remaining input.
+class Bz2Lines : public LineSource {
+   public:
+    explicit Bz2Lines(const std::string& f) : name_(f), in_(1 << 16), out_(1 << 16) {
+        fp_ = std::fopen(f.c_str(), "rb");
+        if (!fp_) throw std::runtime_error("cannot open " + f);
+        std::memset(&s_, 0, sizeof s_);
+    }
+    ~Bz2Lines() override {
+        if (active_) BZ2_bzDecompressEnd(&s_);
+        std::fclose(fp_);
+    }
+    bool next(std::string& line) override {
+        line.clear();
+        for (;;) {
+            const char* b = out_.data() + opos_;
+            const char* nl = (const char*)std::memchr(b, '\n', olen_ - opos_);
+            if (nl) {
+                line.append(b, (size_t)(nl - b));
+                opos_ = (size_t)(nl - out_.data()) + 1;
+                return true;
+            }
+            line.append(b, olen_ - opos_);
+            opos_ = olen_;
+            if (!refill()) return !line.empty();
+        }
+    }
+
+   private:
+    bool refill() {
+        for (;;) {
+            if (s_.avail_in == 0 && !eof_) {
+                const size_t n = std::fread(in_.data(), 1, in_.size(), fp_);
+                if (n == 0) eof_ = true;
+                s_.next_in = in_.data();
+                s_.avail_in = (unsigned)n;
+            }
+            if (!active_) {
+                if (s_.avail_in == 0 && eof_) return false;
+                char* ni = s_.next_in;
+                const unsigned ai = s_.avail_in;
+                std::memset(&s_, 0, sizeof s_);
+                if (BZ2_bzDecompressInit(&s_, 0, 0) != kBzOk)
+                    throw std::runtime_error("bzip2 decoder init failed for " + name_);
+                s_.next_in = ni;
+                s_.avail_in = ai;
+                active_ = true;
+            }
+            s_.next_out = out_.data();
+            s_.avail_out = (unsigned)out_.size();
+            const int rc = BZ2_bzDecompress(&s_);
+            const size_t produced = out_.size() - s_.avail_out;
+            if (rc == kBzStreamEnd) {
+                char* ni = s_.next_in;
+                const unsigned ai = s_.avail_in;
+                BZ2_bzDecompressEnd(&s_);
+                active_ = false;
+                s_.next_in = ni;
+                s_.avail_in = ai;
+            } else if (rc != kBzOk) {
+                throw std::runtime_error("corrupt bzip2 data in " + name_);
+            } else if (produced == 0 && s_.avail_in == 0 && eof_) {
+                throw std::runtime_error("truncated bzip2 stream in " + name_);
+            }
+            if (produced > 0) {
+                opos_ = 0;
+                olen_ = produced;
+                return true;
+            }
+        }
+    }
+
+    std::string name_;
+    FILE* fp_ = nullptr;
+    gx_bz_stream s_;
+    bool active_ = false, eof_ = false;
+    std::vector<char> in_, out_;
+    size_t opos_ = 0, olen_ = 0;
+};
+
+bool ends_with(const std::string& s, const char* suf) {
+    const size_t n = std::strlen(suf);
+    return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
+}
+}  // namespace
+
 double PartitionSum(const std::vector<double>& E, const std::vector<int>& g, double T) {
     const double beta = kPlanck * kVellgt / (kBoltz * T);
     double q = 0.0;
@@ -78,25 +220,30 @@ ExomolStateReader::ExomolStateReader(const std::string& states_file,
     }
 }
 
+ExomolStateReader::~ExomolStateReader() {}
+
 bool ExomolStateReader::OpenFile(const std::string& filename) {
     CloseFile();
-    fp_ = open_or_throw(filename);
+    if (ends_with(filename, ".bz2"))
+        src_.reset(new Bz2Lines(filename));
+    else
+        src_.reset(new TextLines(filename));
     return true;
 }
 
 bool ExomolStateReader::CloseFile() {
-    if (fp_) std::fclose((FILE*)fp_);
-    fp_ = nullptr;
+    src_.reset();
     return true;
 }
 
 bool ExomolStateReader::ReadNextState(double& nu, double& gns, double& e_i, double& aif,
                                       double& gam, double& n) {
-    if (!fp_) return false;
-    char buf[512];
+    if (!src_) return false;
+    std::string line;
     for (;;) {
-        if (!std::fgets(buf, sizeof buf, (FILE*)fp_)) return false;
-        char* p = buf;
+        if (!src_->next(line)) return false;
+        const char* buf = line.c_str();
+        char* p = (char*)buf;
         char* q = nullptr;
         long idf = std::strtol(p, &q, 10);
         if (q == p) continue;

@@ -67,6 +67,9 @@ struct Input {
 
 Input ParseInput(std::istream& in);
 
+// the reader the input names (ExoMol or HITRAN), as the driver builds it
+std::unique_ptr<StateReader> MakeReader(const Input& in);
+
 class CrossSection {
    public:
     // reader / manager may be supplied (not owned); otherwise they are made

@@ -3,16 +3,20 @@
 // Same interface as the reference's Readers/StateReader.h:6-19: one
 // ReadNextState call yields one line (nu, g', E'', A, gamma_ref, n) and
 // ComputePartition(T) gives Q(T).  Implementations here:
-//   ExomolStateReader  .states table + text .trans + .broad (Readers/Exomol/*)
+//   ExomolStateReader  .states table + .trans (text, or bzip2-compressed when
+//                      the name ends in .bz2) + .broad (Readers/Exomol/*)
 //   HitranStateReader  160-character .par records (Readers/HITRAN/*)
 //   ArrayStateReader   in-memory arrays (tests, embedding)
 #pragma once
 #include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
 
 namespace gexs {
+
+class LineSource;  // text lines of a plain or bzip2-compressed file (readers.cpp)
 
 class StateReader {
    public:
@@ -42,6 +46,11 @@ class ExomolStateReader : public StateReader {
    public:
     ExomolStateReader(const std::string& states_file, const std::string& broadener_file,
                       double default_gamma, double default_n);
+    ~ExomolStateReader() override;
+    // a name ending in ".bz2" is read through bzip2 (concatenated streams
+    // included); the reference reads such files as 37-byte "id_f id_i A"
+    // records (Readers/Exomol/bz2_compression/read_compress_trans.c:48-90),
+    // which are text lines of 36 characters plus the newline
     bool OpenFile(const std::string& filename) override;
     bool CloseFile() override;
     bool ReadNextState(double& nu, double& gns, double& e_i, double& aif, double& gam,
@@ -57,7 +66,7 @@ class ExomolStateReader : public StateReader {
     std::map<int, std::pair<double, double>> a0_;
     std::map<std::pair<int, int>, std::pair<double, double>> a1_;
     double def_gamma_, def_n_;
-    void* fp_ = nullptr;
+    std::unique_ptr<LineSource> src_;
 };
 
 // HITRAN 160-character records; field widths {2,1,12,10,10,5,5,10,4,8,15,15,

@@ -77,6 +77,13 @@ def test_exomol_driver_vs_oracle(O, tmp_path):
         *[f"{t} {p}" for t, p in tps], "END", ""])
     out, band, err = run_cli(cfg, tmp_path)
     assert "selected 10000, chunks 7" in err  # 10000 selected lines / 1500 per chunk
+    # the same transitions bzip2-compressed (two concatenated streams): same bits
+    import bz2
+    raw = (tmp_path / "m.trans").read_bytes()
+    cut = raw.index(b"\n", len(raw) // 2) + 1
+    (tmp_path / "m.trans.bz2").write_bytes(bz2.compress(raw[:cut]) + bz2.compress(raw[cut:]))
+    out_bz, _, _ = run_cli(cfg.replace("\nm.trans\n", "\nm.trans.bz2\n"), tmp_path)
+    assert all(np.array_equal(out_bz[k], out[k]) for k in out)
     nu, E, A, g, _, _ = L.lines()
     gam, nex = gamma_n(L, a0, a1)
     keep = (nu >= -100.0) & (nu <= 10100.0) & (nu != 0)
