@@ -152,8 +152,12 @@ struct gexs_ctx {
     DevArr st_E, st_g, q_T, q_part, q_Q;
 
     // derived per grid / per (T,P)
-    DevArr pos, rec, recx, kmax, lf64, items, items_tmp, cost_key, item_idx, jobs, tp, scale, neval,
-        counter;
+    DevArr pos, rec, recx, lf64, items, items_tmp, cost_key, item_idx;
+    DevArr tab;      // per-pass tables [TPParams | JobDesc | Job] x njobs (one upload)
+    DevArr small;    // per-pass counters [xq | scale | neval | kmax] (one fill)
+    DevArr neval;    // N_eval of the plain FP64 pass
+    DevArr counter;  // bounds of the per-XCD tile queues
+    const void* last_neval = nullptr;  // N_eval of the last pass (device)
     int cus = 256;  // compute units of the device
     PosKey pos_key;
     PosKey items_key;
@@ -171,8 +175,6 @@ struct gexs_ctx {
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;
     bool use_graph = false;
-    void* h_tp = nullptr;
-    size_t h_tp_cap = 0;
     void* h_tab = nullptr;
     size_t h_tab_cap = 0;
     double sel_lo = NAN, sel_hi = NAN;
@@ -182,7 +184,7 @@ struct gexs_ctx {
 
     std::unique_ptr<gxs::Server> server;
     Staging staging;
-    DevArr pos_jd, pos_tp, jd;
+    DevArr pos_jd, pos_tp;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -321,6 +323,45 @@ int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
     return GEXS_OK;
 }
 
+// Per-pass tables of njobs jobs: TPParams, JobDesc and Job arrays at 256-B
+// aligned offsets of c->tab (uploaded with one copy), and the counters the
+// pass clears with one fill: [xq (GX_XCDS u32) | scale | neval | kmax].
+struct PassTables {
+    TPParams* tp;
+    JobDesc* jd;
+    Job* jobs;
+    size_t jd_off, jobs_off, bytes;
+    unsigned int* xq;
+    unsigned long long* scale;
+    unsigned long long* neval;
+    int* kmax;
+    int64_t zero_u64;
+};
+
+size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+
+int pass_tables(gexs_ctx* c, int njobs, PassTables* t) {
+    const size_t tp_b = al256((size_t)njobs * sizeof(TPParams));
+    const size_t jd_b = al256((size_t)njobs * sizeof(JobDesc));
+    const size_t jobs_b = al256((size_t)njobs * sizeof(Job));
+    HIPCHK(c, c->tab.ensure(tp_b + jd_b + jobs_b));
+    HIPCHK(c, c->small.ensure((4 + 3 * (size_t)njobs) * 8));
+    char* b = (char*)c->tab.p;
+    t->jd_off = tp_b;
+    t->jobs_off = tp_b + jd_b;
+    t->bytes = tp_b + jd_b + jobs_b;
+    t->tp = (TPParams*)b;
+    t->jd = (JobDesc*)(b + t->jd_off);
+    t->jobs = (Job*)(b + t->jobs_off);
+    char* s = (char*)c->small.p;
+    t->xq = (unsigned int*)s;
+    t->scale = (unsigned long long*)(s + 32);
+    t->neval = t->scale + njobs;
+    t->kmax = (int*)(t->neval + njobs);
+    t->zero_u64 = 4 + 3 * (int64_t)njobs;
+    return GEXS_OK;
+}
+
 int tiles_per_job(const gexs_ctx* c) {
     const int span = GX_WAVE * c->ppl;
     const int w = c->pos_key.win_hi - c->pos_key.win_lo;
@@ -331,14 +372,15 @@ int tiles_per_job(const gexs_ctx* c) {
 // persistent_blocks > 0 with order 0/1 = one shared queue
 int launch_mode(const gexs_ctx* c) {
     if (c->order == 2) return 2;
-    return c->persistent_blocks > 0 ? 1 : 0;
+    if (c->persistent_blocks > 0) return 1;
+    return c->order == 3 ? 3 : 0;
 }
 int launch_blocks(const gexs_ctx* c) {
     if (c->persistent_blocks > 0) return c->persistent_blocks;
     return 8 * c->cus;
 }
 
-// work items for the njobs jobs of c->jd / c->tp (one grid window); sorted
+// work items for the njobs jobs of the pass tables (one grid window); sorted
 // heaviest first when order == 1; with order == 2 in grid order and split
 // into GX_XCDS contiguous ranges of equal line count for the per-XCD queues
 int ensure_items(gexs_ctx* c, int njobs) {
@@ -355,8 +397,11 @@ int ensure_items(gexs_ctx* c, int njobs) {
     HIPCHK(c, c->item_idx.ensure((size_t)std::max(total, 1) * 4));
     const bool sorted = c->order == 1 && total > 1;
     WorkItem* dst = sorted ? c->items_tmp.as<WorkItem>() : c->items.as<WorkItem>();
-    HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, per,
-                         c->tp.as<TPParams>(), span, dst, sorted ? c->cost_key.as<uint32_t>() : nullptr,
+    PassTables t;
+    int rc;
+    if ((rc = pass_tables(c, njobs, &t)) != GEXS_OK) return rc;
+    HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), t.jd, njobs, per,
+                         t.tp, span, dst, sorted ? c->cost_key.as<uint32_t>() : nullptr,
                          sorted ? c->item_idx.as<uint32_t>() : nullptr));
     if (sorted) {
         HIPCHK(c, gxk::radix_sort_u32(c->stream, c->cost_key.as<uint32_t>(), c->item_idx.as<uint32_t>(),
@@ -383,7 +428,7 @@ int ensure_items(gexs_ctx* c, int njobs) {
         }
         while (q <= GX_XCDS) bounds[q++] = (uint32_t)total;
         bounds[GX_XCDS] = (uint32_t)total;
-        HIPCHK(c, c->staging.upload((char*)c->counter.p + GX_XCDS * 4, bounds, sizeof bounds, c->stream));
+        HIPCHK(c, c->staging.upload(c->counter.p, bounds, sizeof bounds, c->stream));
     }
     c->items_key = key;
     c->items_span = span;
@@ -408,14 +453,10 @@ int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
     return GEXS_OK;
 }
 
-int upload_tp(gexs_ctx* c, const std::vector<TPParams>& tps) {
-    HIPCHK(c, c->tp.ensure(tps.size() * sizeof(TPParams)));
-    HIPCHK(c, c->staging.upload(c->tp.p, tps.data(), tps.size() * sizeof(TPParams), c->stream));
-    return GEXS_OK;
-}
-
 // The FP32C profile pass for njobs (T,P) over the lines and window of
-// c->pos_key; outs[j] is the grid of job j (points indexed from 0).
+// c->pos_key; outs[j] is the grid of job j (points indexed from 0).  Device
+// work: one table upload, one counter fill, k_line_max, k_line_rec,
+// k_segments, the profile kernel.
 int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<double*>& outs,
               bool timed) {
     const int njobs = (int)tps.size();
@@ -423,84 +464,83 @@ int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<d
     const int64_t l0 = c->pos_key.l0;
     hipStream_t s = c->stream;
     int rc;
-    if ((rc = upload_tp(c, tps)) != GEXS_OK) return rc;
-    const int per = tiles_per_job(c);
-    std::vector<JobDesc> jds((size_t)njobs);
-    for (int j = 0; j < njobs; ++j) {
-        jds[j].l0 = l0;
-        jds[j].pos_off = 0;
-        jds[j].rec_off = (int64_t)j * n;
-        jds[j].recx_off = (int64_t)j * n;
-        jds[j].n = n;
-        jds[j].items_off = j * per;
-        jds[j].ntiles = per;
-        jds[j].q_slot = -1;
-    }
-    HIPCHK(c, c->jd.ensure(jds.size() * sizeof(JobDesc)));
-    HIPCHK(c, c->staging.upload(c->jd.p, jds.data(), jds.size() * sizeof(JobDesc), s));
+    PassTables t;
+    if ((rc = pass_tables(c, njobs, &t)) != GEXS_OK) return rc;
     HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRec)));
     HIPCHK(c, c->recx.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRecX)));
-    HIPCHK(c, c->kmax.ensure((size_t)njobs * 8));
-    HIPCHK(c, c->scale.ensure((size_t)njobs * 8));
-    HIPCHK(c, c->neval.ensure((size_t)njobs * 8));
     HIPCHK(c, c->counter.ensure(256));
-    HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->kmax.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 4, 0ull));
+    const int per = tiles_per_job(c);
+    std::vector<char> h(t.bytes, 0);
+    TPParams* htp = (TPParams*)h.data();
+    JobDesc* hjd = (JobDesc*)(h.data() + t.jd_off);
+    Job* hjobs = (Job*)(h.data() + t.jobs_off);
+    for (int j = 0; j < njobs; ++j) {
+        htp[j] = tps[(size_t)j];
+        JobDesc& d = hjd[j];
+        d.l0 = l0;
+        d.pos_off = 0;
+        d.rec_off = (int64_t)j * n;
+        d.recx_off = (int64_t)j * n;
+        d.n = n;
+        d.items_off = j * per;
+        d.ntiles = per;
+        d.q_slot = -1;
+        Job& jb = hjobs[j];
+        jb.pos = c->pos.as<LinePos>();
+        jb.rec = c->rec.as<VRec>() + (size_t)j * n;
+        jb.recx = c->recx.as<VRecX>() + (size_t)j * n;
+        jb.out = outs[(size_t)j];
+        jb.n = n;
+        jb.profile = c->profile;
+        jb.scale_slot = j;
+        jb.accumulate = 1;
+    }
+    HIPCHK(c, c->staging.upload(c->tab.p, h.data(), t.bytes, s));
+    HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)c->small.p, t.zero_u64, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
     HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n, c->tp.as<TPParams>(),
-                            c->scale.as<unsigned long long>()));
+                            c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale));
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            gam, nex, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n,
-                            c->tp.as<TPParams>(), c->scale.as<unsigned long long>(), c->rec.as<VRec>(),
-                            c->recx.as<VRecX>(), c->kmax.as<int>(), c->neval.as<unsigned long long>()));
-    std::vector<Job> jobs((size_t)njobs);
-    for (int j = 0; j < njobs; ++j) {
-        jobs[j].pos = c->pos.as<LinePos>();
-        jobs[j].rec = c->rec.as<VRec>() + (size_t)j * n;
-        jobs[j].recx = c->recx.as<VRecX>() + (size_t)j * n;
-        jobs[j].out = outs[j];
-        jobs[j].n = n;
-        jobs[j].profile = c->profile;
-        jobs[j].scale_slot = j;
-        jobs[j].accumulate = 1;
-    }
-    HIPCHK(c, c->jobs.ensure(jobs.size() * sizeof(Job)));
-    HIPCHK(c, c->staging.upload(c->jobs.p, jobs.data(), jobs.size() * sizeof(Job), s));
+                            gam, nex, c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale,
+                            c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval));
     if ((rc = ensure_items(c, njobs)) != GEXS_OK) return rc;
-    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, c->jd.as<JobDesc>(), c->pos.as<LinePos>(),
-                            c->kmax.as<int>(), c->profile));
+    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, t.jd, c->pos.as<LinePos>(), t.kmax,
+                            c->profile));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
-                                 c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
-                                 c->counter.as<unsigned int>(), launch_mode(c), launch_blocks(c)));
+                                 t.jobs, t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
+                                 launch_blocks(c)));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     c->last.n_tiles = c->n_items;
+    c->last_neval = t.neval;
     return GEXS_OK;
 }
 
-int run_fp64(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
+// reset_neval = false adds this pass's N_eval to the count of the passes
+// before it (a batch of several (T,P))
+int run_fp64(gexs_ctx* c, const TPParams& tp, double* out, bool timed, bool reset_neval = true) {
     const int n = (int)c->pos_key.n;
     const int64_t l0 = c->pos_key.l0;
     hipStream_t s = c->stream;
     int rc;
-    if ((rc = upload_tp(c, std::vector<TPParams>{tp})) != GEXS_OK) return rc;
+    PassTables t;
+    if ((rc = pass_tables(c, 1, &t)) != GEXS_OK) return rc;
+    HIPCHK(c, c->staging.upload(t.tp, &tp, sizeof tp, s));
     HIPCHK(c, c->lf64.ensure((size_t)std::max(n, 1) * sizeof(LineF64)));
     HIPCHK(c, c->neval.ensure(8));
-    HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), 1, 0ull));
+    if (reset_neval) HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), 1, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() + l0 : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() + l0 : nullptr;
     HIPCHK(c, gxk::line_f64(s, c->nu.as<double>() + l0, c->E.as<double>() + l0, c->A.as<double>() + l0,
-                            c->g.as<int32_t>() + l0, gam, nex, n, c->pos.as<LinePos>(), c->tp.as<TPParams>(),
+                            c->g.as<int32_t>() + l0, gam, nex, n, c->pos.as<LinePos>(), t.tp,
                             c->lf64.as<LineF64>(), c->neval.as<unsigned long long>()));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
     HIPCHK(c, gxk::fp64_points(s, c->profile, c->nu.as<double>() + l0, c->pos.as<LinePos>(), c->lf64.as<LineF64>(), n,
                                tp.start, tp.dfreq, tp.win_lo, tp.win_hi, out));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     c->last.n_tiles = 0;
+    c->last_neval = c->neval.p;
     return GEXS_OK;
 }
 
@@ -563,7 +603,6 @@ int gexs_destroy(gexs_ctx* c) {
     gxk::sort_buffers_free(c->sb);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    if (c->h_tp) (void)hipHostFree(c->h_tp);
     if (c->h_tab) (void)hipHostFree(c->h_tab);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -740,7 +779,7 @@ int gexs_last_stats(gexs_ctx* c, gexs_stats* out) {
     if (c->last_neval_pending) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         unsigned long long ne = 0;
-        if (c->neval.p) HIPCHK(c, hipMemcpy(&ne, c->neval.p, 8, hipMemcpyDeviceToHost));
+        if (c->last_neval) HIPCHK(c, hipMemcpy(&ne, c->last_neval, 8, hipMemcpyDeviceToHost));
         c->last.n_eval = (int64_t)ne;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) == hipSuccess) c->last.kernel_ms = ms;
@@ -810,7 +849,7 @@ int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order, int waves_per_t
     if (!c) return GEXS_EINVAL;
     if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4 && points_per_lane != 8)
         return c->fail(GEXS_EINVAL, "points_per_lane must be 1, 2, 4 or 8");
-    if (order < 0 || order > 2) return c->fail(GEXS_EINVAL, "order must be 0, 1 or 2");
+    if (order < 0 || order > 3) return c->fail(GEXS_EINVAL, "order must be 0, 1, 2 or 3");
     if (waves_per_tile != 1 && waves_per_tile != 4) return c->fail(GEXS_EINVAL, "waves_per_tile must be 1 or 4");
     c->ppl = points_per_lane;
     c->order = order;
@@ -911,42 +950,34 @@ static int pinned(gexs_ctx* c, void** p, size_t* cap, size_t want) {
 }
 
 // Device work of one FP32C batch.  Only kernels and copies from the ctx's
-// pinned tables, so it can be captured into a graph: the (T,P) table is read
-// from c->h_tp when the copy executes, the rest of the tables are fixed for a
-// given plan.
-static int enqueue_batch(gexs_ctx* c, int njobs, int nq, size_t jd_b, size_t jobs_b) {
+// pinned tables, so it can be captured into a graph: the tables are read from
+// c->h_tab ([TPParams | JobDesc | Job] in the layout of c->tab, then the T of
+// the device partition sums) when the copies execute.
+static int enqueue_batch(gexs_ctx* c, int njobs, int nq, const PassTables& t) {
     hipStream_t s = c->stream;
     const int n = (int)c->pos_key.n;
-    HIPCHK(c, hipMemcpyAsync(c->tp.p, c->h_tp, (size_t)njobs * sizeof(TPParams), hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(c->jd.p, c->h_tab, (size_t)njobs * sizeof(JobDesc), hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(c->jobs.p, (char*)c->h_tab + jd_b, (size_t)njobs * sizeof(Job),
-                             hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->tab.p, c->h_tab, t.bytes, hipMemcpyHostToDevice, s));
     if (nq > 0) {
         const int nblk = 256;
-        HIPCHK(c, hipMemcpyAsync(c->q_T.p, (char*)c->h_tab + jd_b + jobs_b, (size_t)nq * 8,
+        HIPCHK(c, hipMemcpyAsync(c->q_T.p, (char*)c->h_tab + t.bytes, (size_t)nq * 8,
                                  hipMemcpyHostToDevice, s));
         HIPCHK(c, gxk::partition(s, c->st_E.as<double>(), c->st_g.as<int32_t>(), c->n_states, c->q_T.as<double>(),
                                  nq, c->q_part.as<double>(), nblk, c->q_Q.as<double>()));
-        HIPCHK(c, gxk::take_q(s, c->jd.as<JobDesc>(), njobs, c->q_Q.as<double>(), c->tp.as<TPParams>()));
+        HIPCHK(c, gxk::take_q(s, t.jd, njobs, c->q_Q.as<double>(), t.tp));
     }
-    HIPCHK(c, gxk::fill_u64(s, c->scale.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->kmax.as<unsigned long long>(), njobs, 0ull));
-    HIPCHK(c, gxk::fill_u64(s, c->counter.as<unsigned long long>(), 4, 0ull));
+    HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)c->small.p, t.zero_u64, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
     HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n, c->tp.as<TPParams>(),
-                            c->scale.as<unsigned long long>()));
+                            c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale));
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            gam, nex, c->pos.as<LinePos>(), c->jd.as<JobDesc>(), njobs, n,
-                            c->tp.as<TPParams>(), c->scale.as<unsigned long long>(), c->rec.as<VRec>(),
-                            c->recx.as<VRecX>(), c->kmax.as<int>(), c->neval.as<unsigned long long>()));
-    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, c->jd.as<JobDesc>(), c->pos.as<LinePos>(),
-                            c->kmax.as<int>(), c->profile));
+                            gam, nex, c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale,
+                            c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval));
+    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, t.jd, c->pos.as<LinePos>(), t.kmax,
+                            c->profile));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
-                                 c->jobs.as<Job>(), c->scale.as<unsigned long long>(),
-                                 c->counter.as<unsigned int>(), launch_mode(c), launch_blocks(c)));
+                                 t.jobs, t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
+                                 launch_blocks(c)));
     return GEXS_OK;
 }
 
@@ -1003,14 +1034,15 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
         HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
         for (int i = 0; i < ntp; ++i) {
             TPParams t = make_tp(c, tp[i].T, tp[i].P, Qs[i], grid->start, dfreq, 0, grid->npoints);
-            if ((rc = run_fp64(c, t, c->batch_out.as<double>() + (size_t)i * np, i == 0)) != GEXS_OK) return rc;
+            if ((rc = run_fp64(c, t, c->batch_out.as<double>() + (size_t)i * np, i == 0, i == 0)) != GEXS_OK)
+                return rc;
         }
         HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
         if (out) HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         unsigned long long ne = 0;
         HIPCHK(c, hipMemcpy(&ne, c->neval.p, 8, hipMemcpyDeviceToHost));
-        c->last.n_eval = (int64_t)ne * ntp;  // the per-job count is the same for every job
+        c->last.n_eval = (int64_t)ne;  // summed over the batch
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) == hipSuccess) c->last.kernel_ms = ms;
         if (hipEventElapsedTime(&ms, c->ev_start, c->ev_k1) == hipSuccess) c->last.total_ms = ms;
@@ -1019,22 +1051,15 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     }
     // ---- FP32C: host tables in pinned memory
     const int per = tiles_per_job(c);
-    const size_t jd_b = ((size_t)ntp * sizeof(JobDesc) + 255) / 256 * 256;
-    const size_t jobs_b = ((size_t)ntp * sizeof(Job) + 255) / 256 * 256;
-    if ((rc = pinned(c, &c->h_tp, &c->h_tp_cap, (size_t)ntp * sizeof(TPParams))) != GEXS_OK) return rc;
-    if ((rc = pinned(c, &c->h_tab, &c->h_tab_cap, jd_b + jobs_b + (size_t)ntp * 8)) != GEXS_OK) return rc;
-    TPParams* htp = (TPParams*)c->h_tp;
-    JobDesc* hjd = (JobDesc*)c->h_tab;
-    Job* hjobs = (Job*)((char*)c->h_tab + jd_b);
-    double* hT = (double*)((char*)c->h_tab + jd_b + jobs_b);
-    HIPCHK(c, c->tp.ensure((size_t)ntp * sizeof(TPParams)));
-    HIPCHK(c, c->jd.ensure((size_t)ntp * sizeof(JobDesc)));
-    HIPCHK(c, c->jobs.ensure((size_t)ntp * sizeof(Job)));
+    PassTables t;
+    if ((rc = pass_tables(c, ntp, &t)) != GEXS_OK) return rc;
+    if ((rc = pinned(c, &c->h_tab, &c->h_tab_cap, t.bytes + (size_t)ntp * 8)) != GEXS_OK) return rc;
+    TPParams* htp = (TPParams*)c->h_tab;
+    JobDesc* hjd = (JobDesc*)((char*)c->h_tab + t.jd_off);
+    Job* hjobs = (Job*)((char*)c->h_tab + t.jobs_off);
+    double* hT = (double*)((char*)c->h_tab + t.bytes);
     HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * ntp * sizeof(VRec)));
     HIPCHK(c, c->recx.ensure((size_t)std::max(n, 1) * ntp * sizeof(VRecX)));
-    HIPCHK(c, c->kmax.ensure((size_t)ntp * 8));
-    HIPCHK(c, c->scale.ensure((size_t)ntp * 8));
-    HIPCHK(c, c->neval.ensure((size_t)ntp * 8));
     HIPCHK(c, c->counter.ensure(256));
     HIPCHK(c, c->q_T.ensure((size_t)ntp * 8));
     HIPCHK(c, c->q_part.ensure((size_t)ntp * 256 * 8));
@@ -1066,26 +1091,32 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
         jb.accumulate = 0;  // every point of every grid is written once
         hjobs[j] = jb;
     }
-    // the tiles kernel reads window and job tables from the device copies
-    HIPCHK(c, hipMemcpyAsync(c->tp.p, c->h_tp, (size_t)ntp * sizeof(TPParams), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->jd.p, c->h_tab, (size_t)ntp * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
+    // the tiles kernel reads window and job tables from the device copy
+    HIPCHK(c, hipMemcpyAsync(c->tab.p, c->h_tab, t.bytes, hipMemcpyHostToDevice, c->stream));
     if ((rc = ensure_items(c, ntp)) != GEXS_OK) return rc;
     if ((rc = next_events(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
     if (c->use_graph) {
-        char key[256];
-        std::snprintf(key, sizeof key, "%d|%d|%llu|%d|%d|%d|%p|%p|%p|%lld|%lld|%d|%d", ntp, qs,
-                      (unsigned long long)c->gen, c->ppl, c->order, c->persistent_blocks, c->rec.p,
-                      c->batch_out.p, c->items.p, (long long)l0, (long long)n, grid->npoints, per);
-        const std::string k2 = std::string(key) + "|" + std::to_string(grid->start) + "|" +
-                               std::to_string(grid->end) + "|" + std::to_string(cutoff);
+        // everything the captured work depends on: launch shape, sizes and
+        // every device address it reads or writes (a buffer that grew since
+        // the capture has moved)
+        char key[512];
+        std::snprintf(key, sizeof key,
+                      "%d|%d|%llu|%d|%d|%d|%d|%lld|%lld|%d|%d|%a|%a|%a|%p|%p|%p|%p|%p|%p|%p|%p|%p|"
+                      "%p|%p|%p|%p|%p",
+                      ntp, qs, (unsigned long long)c->gen, c->ppl, c->split, c->order,
+                      c->persistent_blocks, (long long)l0, (long long)n, grid->npoints, per,
+                      grid->start, grid->end, cutoff, c->rec.p, c->recx.p, c->batch_out.p,
+                      c->items.p, c->pos.p, c->tab.p, c->small.p, c->counter.p, c->h_tab,
+                      c->st_E.p, c->st_g.p, c->nu.p, c->q_T.p, c->q_part.p);
+        const std::string k2 = std::string(key) + "|" + std::to_string((uintptr_t)c->q_Q.p);
         if (!c->graph_exec || c->graph_key != k2) {
             if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
             if (c->graph) (void)hipGraphDestroy(c->graph);
             c->graph_exec = nullptr;
             c->graph = nullptr;
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_batch(c, ntp, qs, jd_b, jobs_b);
+            rc = enqueue_batch(c, ntp, qs, t);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (rc != GEXS_OK) {
@@ -1101,14 +1132,14 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
         HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     } else {
         HIPCHK(c, hipEventRecord(c->ev_k0, c->stream));
-        if ((rc = enqueue_batch(c, ntp, qs, jd_b, jobs_b)) != GEXS_OK) return rc;
+        if ((rc = enqueue_batch(c, ntp, qs, t)) != GEXS_OK) return rc;
     }
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
     c->last.n_tiles = c->n_items;
     if (out) HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<unsigned long long> ne((size_t)ntp, 0ull);
-    HIPCHK(c, hipMemcpy(ne.data(), c->neval.p, ne.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ne.data(), t.neval, ne.size() * 8, hipMemcpyDeviceToHost));
     int64_t tot = 0;
     for (auto v : ne) tot += (int64_t)v;
     c->last.n_eval = tot;
@@ -1153,9 +1184,10 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
     int nrep = 0;
     for (int pi = 0; pi < 4; ++pi)
         for (int split = 1; split <= 4; split += 3)
-        for (int order = 0; order < 3; ++order)
+        for (int order = 0; order < 4; ++order)
             for (int pb = 0; pb < 3; ++pb) {
                 if (order == 2 && pb == 0) continue;  // per-XCD queues are persistent
+                if (order == 3 && pb != 0) continue;  // dispatch order applies to launch slots
                 c->ppl = ppls[pi];
                 c->split = split;
                 c->order = order;

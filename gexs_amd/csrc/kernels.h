@@ -29,12 +29,13 @@ hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, 
 hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx, int n,
                          WorkItem* out);
 // split 1: one wave per item; split 4: the four waves of a workgroup share an
-// item's lines.  mode 0: one launch slot per item; 1: `blocks` persistent
-// workgroups on one counter (xq[0]); 2: `blocks` persistent workgroups on
-// per-XCD queues (xq = 8 counters then 9 item bounds)
+// item's lines.  mode 0 / 3: one launch slot per item (XCD-contiguous /
+// dispatch order); 1: `blocks` persistent workgroups on one counter (xq[0]);
+// 2: `blocks` persistent workgroups on per-XCD queues (xq = 8 counters, zero
+// at launch; xqb = 9 item bounds)
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,
-                         unsigned int* xq, int mode, int blocks);
+                         unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
 hipError_t line_f64(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, int n,
                     const LinePos* pos, const TPParams* tp, LineF64* out,

@@ -624,21 +624,24 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }  // namespace
 
 // mode 0: one item per wave (SPLIT 1) or per workgroup (SPLIT 4), blocks in
-// XCD-contiguous order; 1: persistent, items pulled from one counter; 2:
+// XCD-contiguous order; 3: as 0 in dispatch order (consecutive items on
+// different XCDs); 1: persistent, items pulled from one counter; 2:
 // persistent, items pulled from the queue of the workgroup's XCD (items in
 // grid order, split into GX_XCDS cost-balanced contiguous ranges), then from
-// the other queues.  xq = [GX_XCDS counters | GX_XCDS + 1 bounds].
+// the other queues.  xq = GX_XCDS counters (zero at launch), xqb = the
+// GX_XCDS + 1 queue bounds.
 template <int PROF, int PPL, int SPLIT>
 __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
                                                          const unsigned long long* __restrict__ scale_bits,
-                                                         unsigned int* __restrict__ xq, int mode) {
+                                                         unsigned int* __restrict__ xq,
+                                                         const unsigned int* __restrict__ xqb, int mode) {
     __shared__ double red[SPLIT > 1 ? kSplit * GX_WAVE * PPL : 1];
     __shared__ int grab;
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    if (mode == 0) {
+    const int lb = mode == 3 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+    if (mode == 0 || mode == 3) {
         if (SPLIT == 1) {
             const int idx = lb * kSplit + wv;
             if (idx >= nitems) return;
@@ -658,8 +661,8 @@ __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restr
     const int home = mode == 2 ? (int)(blockIdx.x % GX_XCDS) : 0;
     for (int k = 0; k < nq; ++k) {
         const int q = (home + k) % nq;
-        const int beg = mode == 2 ? (int)xq[GX_XCDS + q] : 0;
-        const int end = mode == 2 ? (int)xq[GX_XCDS + q + 1] : nitems;
+        const int beg = mode == 2 ? (int)xqb[q] : 0;
+        const int end = mode == 2 ? (int)xqb[q + 1] : nitems;
         for (;;) {
             int idx;
             if (SPLIT == 1) {
@@ -889,25 +892,26 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
 template <int PROF, int PPL, int SPLIT>
 static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitems,
                                  const Job* jobs, const unsigned long long* scale_bits,
-                                 unsigned int* xq, int mode, int blocks) {
+                                 unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     if (nitems <= 0) return hipSuccess;
     const int ipb = SPLIT == 1 ? GX_WG / GX_WAVE : 1;  // items per block
-    const int nb = mode == 0 ? blocks_for(nitems, ipb)
-                             : std::max(1, std::min(blocks, blocks_for(nitems, ipb)));
+    const bool slots = mode == 0 || mode == 3;  // one launch slot per item
+    const int nb = slots ? blocks_for(nitems, ipb)
+                         : std::max(1, std::min(blocks, blocks_for(nitems, ipb)));
     hipLaunchKernelGGL((k_profile_tiles<PROF, PPL, SPLIT>), dim3(nb), dim3(GX_WG), 0, s, items,
-                       nitems, jobs, scale_bits, xq, mode);
+                       nitems, jobs, scale_bits, xq, xqb, mode);
     return hipGetLastError();
 }
 
 template <int PROF, int SPLIT>
 static hipError_t launch_ppl(hipStream_t s, int ppl, const WorkItem* items, int nitems,
                              const Job* jobs, const unsigned long long* scale_bits,
-                             unsigned int* xq, int mode, int blocks) {
+                             unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     switch (ppl) {
-        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
-        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
-        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
-        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
         default: return hipErrorInvalidValue;
     }
 }
@@ -915,26 +919,27 @@ static hipError_t launch_ppl(hipStream_t s, int ppl, const WorkItem* items, int 
 template <int PROF>
 static hipError_t launch_prof(hipStream_t s, int ppl, int split, const WorkItem* items,
                               int nitems, const Job* jobs, const unsigned long long* scale_bits,
-                              unsigned int* xq, int mode, int blocks) {
-    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
+                              unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
+    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
     if (split == kSplit)
-        return launch_ppl<PROF, kSplit>(s, ppl, items, nitems, jobs, scale_bits, xq, mode, blocks);
+        return launch_ppl<PROF, kSplit>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
     return hipErrorInvalidValue;
 }
 
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,
-                         unsigned int* xq, int mode, int blocks) {
+                         unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     switch (profile) {
         case GX_PROF_VOIGT:
-            return launch_prof<GX_PROF_VOIGT>(s, ppl, split, items, nitems, jobs, scale_bits, xq, mode, blocks);
+            return launch_prof<GX_PROF_VOIGT>(s, ppl, split, items, nitems, jobs, scale_bits, xq, xqb, mode,
+                                              blocks);
         case GX_PROF_VOIGT_QUAD:
-            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, scale_bits, xq, mode,
-                                                   blocks);
+            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, scale_bits, xq, xqb,
+                                                   mode, blocks);
         case GX_PROF_DOPPLER:
         case GX_PROF_GAUSSIAN:
-            return launch_prof<GX_PROF_DOPPLER>(s, ppl, split, items, nitems, jobs, scale_bits, xq, mode,
-                                                blocks);
+            return launch_prof<GX_PROF_DOPPLER>(s, ppl, split, items, nitems, jobs, scale_bits, xq, xqb,
+                                                mode, blocks);
         default:
             return hipErrorInvalidValue;
     }

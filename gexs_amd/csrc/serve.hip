@@ -290,10 +290,11 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     const size_t jobs_b = align_up(jobs.size() * sizeof(Job));
     const size_t t_b = align_up(Ts.size() * 8);
     const size_t tab_b = tp_b + jd_b + jobs_b + t_b;
-    // small arrays: scale, neval (kMaxBatch u64 each), counter, Q, partials
-    const size_t sm_scale = 0, sm_neval = kMaxBatch * 8, sm_counter = 2 * kMaxBatch * 8,
-                 sm_q = sm_counter + 256, sm_kmax = sm_q + kMaxBatch * 8,
-                 sm_part = sm_kmax + kMaxBatch * 8, sm_b = sm_part + (size_t)kMaxBatch * kQBlocks * 8;
+    // small arrays: scale, neval, kmax (kMaxBatch u64 each; cleared by one
+    // fill), counter, Q, partials
+    const size_t sm_scale = 0, sm_neval = kMaxBatch * 8, sm_kmax = 2 * kMaxBatch * 8,
+                 sm_counter = 3 * kMaxBatch * 8, sm_q = sm_counter + 256,
+                 sm_part = sm_q + kMaxBatch * 8, sm_b = sm_part + (size_t)kMaxBatch * kQBlocks * 8;
     hipError_t e = hipSuccess;
     size_t dummy_cap = 0;
     if (!s.small && (e = grow(&s.small, &dummy_cap, sm_b)) != hipSuccess) goto fail;
@@ -344,13 +345,11 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             hipSuccess)
             goto fail;
         if ((e = gxk::take_q(st, d_jd, nreq, d_Q, d_tp)) != hipSuccess) goto fail;
-        if ((e = gxk::fill_u64(st, d_scale, nreq, 0ull)) != hipSuccess) goto fail;
-        if ((e = gxk::fill_u64(st, d_neval, nreq, 0ull)) != hipSuccess) goto fail;
+        if ((e = gxk::fill_u64(st, d_scale, 3 * kMaxBatch, 0ull)) != hipSuccess) goto fail;  // + neval, kmax
         if ((e = gxk::line_pos(st, v_.nu, d_jd, nreq, max_n, d_tp, (LinePos*)base)) != hipSuccess) goto fail;
         if ((e = gxk::line_max(st, v_.nu, v_.E, v_.A, v_.g, (const LinePos*)base, d_jd, nreq, max_n, d_tp,
                                d_scale)) != hipSuccess)
             goto fail;
-        if ((e = gxk::fill_u64(st, (unsigned long long*)d_kmax, (nreq + 1) / 2, 0ull)) != hipSuccess) goto fail;
         if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, (const LinePos*)base, d_jd,
                                nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval)) !=
             hipSuccess)
@@ -362,7 +361,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             hipSuccess)
             goto fail;
         if ((e = gxk::profile_tiles(st, ph_.profile, ph_.ppl, ph_.split, d_items, items_total, d_jobs, d_scale,
-                                    d_counter, 0, 0)) != hipSuccess)
+                                    d_counter, nullptr, 0, 0)) != hipSuccess)
             goto fail;
         size_t hoff = 0;
         for (int k = 0; k < nreq; ++k) {

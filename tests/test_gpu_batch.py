@@ -91,6 +91,29 @@ def test_split_close_to_single_wave(batch_mgr):
     assert np.max(np.abs(a - b)) / np.max(a) < 1e-6
 
 
+def test_graph_follows_tuning_and_buffer_changes(batch_mgr):
+    """A captured batch is replayed only while everything it was captured
+    with still holds: changing the split, or growing the buffers with a
+    larger batch in between, makes the next graph run capture afresh."""
+    L, m = batch_mgr
+    tps = table(4)
+    m.set_launch(1024, False)
+    eager = {}
+    for split in (1, 4):
+        m.set_tuning(4, 1, split)
+        eager[split] = m.run_batch(tps, *GRID)
+    m.set_launch(1024, True)
+    for split in (1, 4, 1):
+        m.set_tuning(4, 1, split)
+        assert np.array_equal(m.run_batch(tps, *GRID), eager[split]), split
+    m.set_launch(1024, False)
+    m.run_batch(table(12), *GRID)  # grows the per-pass tables
+    m.set_launch(1024, True)
+    assert np.array_equal(m.run_batch(tps, *GRID), eager[1])
+    m.set_tuning(4, 1, 1)
+    m.set_launch(0, False)
+
+
 def test_device_resident_output(batch_mgr):
     """out="device" leaves the results in the library's buffer, equal to the
     host copy of the same batch."""
