@@ -157,8 +157,8 @@ def test_autotune_picks_a_candidate(batch_mgr):
     tps = table(8)
     ref = m.run_batch(tps, *GRID)
     best, cands = m.autotune(tps, *GRID)
-    # ppl {1,2,4,8} x waves per tile {1,4} x 8 launch shapes
-    assert len(cands) == 64
+    # ppl {1,2,4,8} x waves per tile {1,4} x 9 launch shapes
+    assert len(cands) == 72
     assert best["kernel_ms"] == min(c["kernel_ms"] for c in cands)
     print("autotune best", best, flush=True)
     out = m.run_batch(tps, *GRID)
