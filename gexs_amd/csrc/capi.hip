@@ -369,11 +369,13 @@ int tiles_per_job(const gexs_ctx* c) {
 }
 
 // profile-kernel launch shape: order 2 = per-XCD queues (persistent);
-// persistent_blocks > 0 with order 0/1 = one shared queue
+// persistent_blocks > 0 with order 0/1 = one shared queue; otherwise one
+// launch slot per tile, XCD-contiguous for order 0 and in dispatch order
+// (consecutive tiles on different XCDs) for heaviest-first and order 3
 int launch_mode(const gexs_ctx* c) {
     if (c->order == 2) return 2;
     if (c->persistent_blocks > 0) return 1;
-    return c->order == 3 ? 3 : 0;
+    return c->order == 0 ? 0 : 3;
 }
 int launch_blocks(const gexs_ctx* c) {
     if (c->persistent_blocks > 0) return c->persistent_blocks;
