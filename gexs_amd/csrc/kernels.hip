@@ -471,9 +471,22 @@ __device__ __forceinline__ bool in_window(int j, const LinePos& lp) {
     return j >= lp.ib && j < lp.ie;
 }
 
+// how many of the 64 points j0 .. j0+63 of one slice lie in the window:
+// 0 none (the slice is skipped), 1 some (masked), 2 all (unmasked).  j0 and
+// the window are wave-uniform, so this is a scalar branch.
+template <int MASK>
+__device__ __forceinline__ int slice_in_window(int j0, const LinePos& lp) {
+    if (MASK == kMaskNone) return 2;
+    const bool none = (MASK != kMaskLo && j0 >= lp.ie) || (MASK != kMaskHi && j0 + GX_WAVE <= lp.ib);
+    if (none) return 0;
+    const bool all = (MASK == kMaskHi || j0 >= lp.ib) && (MASK == kMaskLo || j0 + GX_WAVE <= lp.ie);
+    return all ? 2 : 1;
+}
+
 // lines with per-line decisions: window masks per lane (only the edges MASK
-// names), region I inline, regions II-IV per lane; the next line's records
-// load while this one is evaluated
+// names; slices wholly outside the window are skipped, slices wholly inside
+// are unmasked), region I inline, regions II-IV per lane; the next line's
+// records load while this one is evaluated
 template <int PROF, int PPL, int MASK>
 __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, ConstRec rec,
                                               ConstRecX recx, int a, int b, int w0, int w1) {
@@ -495,22 +508,34 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
             if (kmin >= rx.k15) {
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) {
+                    const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+                    if (st == 0) continue;
                     const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
-                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
+                    if (st == 2)
+                        t.part[q] += v;
+                    else
+                        t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) {
+                    const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+                    if (st == 0) continue;
                     const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
                     float v = region1(x, r.Kl, r.E2, r.H);
                     if (x + rx.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, rx.y);
-                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
+                    if (st == 2)
+                        t.part[q] += v;
+                    else
+                        t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
                 }
             }
         } else {
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
-                if (in_window<MASK>(t.jj[q], lp)) {
+                const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+                if (st == 0) continue;
+                if (st == 2 || in_window<MASK>(t.jj[q], lp)) {
                     const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
                     float k;
                     if (PROF == GX_PROF_VOIGT_QUAD) {
