@@ -287,7 +287,7 @@ def serve_bench(a):
     m.InitializeConstants(100.0, 1000.0, 1.0, 0.01, spec["mass"], 1.0, synth.T_REF,
                           synth.P_SCALE_BAR, cutoff=spec["cutoff"])
     m.load_list(lst)
-    m.set_tuning(2, 1, 4)
+    m.set_tuning(4, 3, 4)  # the shape the autotuner picks on the single-(T,P) workload
     m.serve_start(max_inflight=4, mem_budget=24 << 30)
     for r in serve_requests(np.random.default_rng(1), 4):  # first-use costs
         m.wait(m.submit(*r))
@@ -298,6 +298,8 @@ def serve_bench(a):
         reqs = serve_requests(rng, n)
         arr = np.cumsum(rng.exponential(1.0 / rate, n))
         lat = np.zeros(n)
+        srv = np.zeros(n)  # submit -> completion as the server saw it
+        dev = np.zeros(n)  # device time of the request's batch
         pending = {}
         i = 0
         t0 = time.perf_counter()
@@ -311,6 +313,8 @@ def serve_bench(a):
                 if r["status"] == 0:
                     k = pending.pop(rid)
                     lat[k] = time.perf_counter() - t0 - arr[k]
+                    srv[k] = r["latency_ms"]
+                    dev[k] = r["device_ms"]
                     m.release(rid)
             if i < n and not pending:
                 time.sleep(max(0.0, min(arr[i] - (time.perf_counter() - t0), 0.002)))
@@ -320,10 +324,15 @@ def serve_bench(a):
         sweeps.append({"rate": rate, "n": n, "req_per_s": n / span,
                        "p50_ms": float(np.percentile(lat, 50) * 1e3),
                        "p99_ms": float(np.percentile(lat, 99) * 1e3), "arrivals": arr,
+                       "server_p50_ms": float(np.percentile(srv, 50)),
+                       "server_p99_ms": float(np.percentile(srv, 99)),
+                       "device_p50_ms": float(np.percentile(dev, 50)),
                        "mean_points": float(np.mean([round((r[3] - r[2]) / r[4]) for r in reqs]))})
         all_reqs.append(reqs)
         log(f"[bench] serve {rate:6.1f} req/s offered: {n / span:7.1f} req/s, "
-            f"p50 {sweeps[-1]['p50_ms']:.2f} ms, p99 {sweeps[-1]['p99_ms']:.2f} ms")
+            f"p50 {sweeps[-1]['p50_ms']:.2f} ms, p99 {sweeps[-1]['p99_ms']:.2f} ms (server "
+            f"{sweeps[-1]['server_p50_ms']:.2f} / {sweeps[-1]['server_p99_ms']:.2f} ms, device p50 "
+            f"{sweeps[-1]['device_p50_ms']:.2f} ms)")
     m.serve_stop()
     m.close()
     # one request at a time on the plain FP64 path: service times, then the

@@ -361,7 +361,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             hipSuccess)
             goto fail;
         if ((e = gxk::profile_tiles(st, ph_.profile, ph_.ppl, ph_.split, d_items, items_total, d_jobs, d_scale,
-                                    d_counter, nullptr, 0, 0)) != hipSuccess)
+                                    d_counter, nullptr, 3, 0)) != hipSuccess)  // slots in dispatch order
             goto fail;
         size_t hoff = 0;
         for (int k = 0; k < nreq; ++k) {
@@ -408,7 +408,13 @@ void Server::loop() {
             }
             if (batch.empty()) break;
             std::string err;
-            if (launch(*s, batch, &err) != 0) {
+            // the launch (host tables, copies, kernel launches) runs without
+            // the lock so that submit and poll do not wait for it; slots and
+            // the arena's regions of this batch belong to this thread
+            g.unlock();
+            const int lrc = launch(*s, batch, &err);
+            g.lock();
+            if (lrc != 0) {
                 for (Req* q : batch) {
                     q->res.status = -3;
                     arena_.free(q->off, q->foot);
