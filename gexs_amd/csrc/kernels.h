@@ -22,9 +22,10 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
 // segment bounds of every work item (after line_rec has produced kmax per job)
 hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const JobDesc* jd,
                     const LinePos* pos, const int* kmax, int profile);
+// jobs job0 .. job0 + njobs - 1 of jd / tp, all with tiles of span points
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
-                 uint32_t* item_idx);
+                 uint32_t* item_idx, int job0 = 0);
 hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, TPParams* tp);
 hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx, int n,
                          WorkItem* out);

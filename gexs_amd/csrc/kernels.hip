@@ -248,16 +248,16 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
 }
 
 // ---------------------------------------------------------------- k_tiles
-// One work item per wave-tile: lines whose window meets [w0, w1) form the
+// One work item per tile: lines whose window meets [w0, w1) form the
 // contiguous range [lo, hi) because ib and ie are non-decreasing in nu.
-// blockIdx.y = job.
+// blockIdx.y = job - job0 (jobs job0 .. job0 + gridDim.y - 1 share the span).
 __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
                                                const JobDesc* __restrict__ jd,
                                                const TPParams* __restrict__ tp, int span,
-                                               WorkItem* __restrict__ items,
+                                               int job0, WorkItem* __restrict__ items,
                                                uint32_t* __restrict__ cost_key,
                                                uint32_t* __restrict__ item_idx) {
-    const int job = blockIdx.y;
+    const int job = job0 + blockIdx.y;
     const JobDesc d = jd[job];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.ntiles) return;
@@ -893,10 +893,10 @@ hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const JobDesc* j
 
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
-                 uint32_t* item_idx) {
+                 uint32_t* item_idx, int job0) {
     if (njobs <= 0 || max_tiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tiles, dim3(blocks_for(max_tiles, 256), njobs), dim3(256), 0, s, pos, jd,
-                       tp, span, items, cost_key, item_idx);
+                       tp, span, job0, items, cost_key, item_idx);
     return hipGetLastError();
 }
 

@@ -5,10 +5,11 @@
 // server's device arena, groups the admitted ones into a batch and launches
 // the batch on a free stream slot, so batches overlap on the device.  A batch
 // is one launch sequence: partition function per request, line windows,
-// FP32 records, wave-tile items and one profile-kernel launch over all of the
-// batch's tiles, then the copy-out.  Every request owns its positions,
-// records, tiles and output, and reductions never cross requests, so a
-// result is bit-for-bit the same alone and in any mix.
+// FP32 records, tiles and one profile-kernel launch per tile width over the
+// batch's tiles, then the copy-out.  Each request's tile width follows from
+// its own grid size; every request owns its positions, records, tiles and
+// output, and reductions never cross requests, so a result is bit-for-bit the
+// same alone and in any mix.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -98,6 +99,7 @@ class Server {
         uint64_t id;
         Request r;
         int npoints = 0, ntiles = 0;
+        int ppl = 1;  // tile width 64*ppl points, from the request's own size
         int64_t l0 = 0, n = 0;
         size_t foot = 0, off = 0;
         std::chrono::steady_clock::time_point arrival;
@@ -113,7 +115,7 @@ class Server {
         size_t tab_cap = 0;
         void* items = nullptr;  // work items
         size_t items_cap = 0;
-        void* small = nullptr;  // scale | neval | counter | Q | partition partials
+        void* small = nullptr;  // scale | neval | kmax | counter | Q | partition partials
         double* host_out = nullptr;  // pinned copy-out
         size_t host_cap = 0;
         void* stage = nullptr;  // pinned staging of the tables
@@ -121,6 +123,7 @@ class Server {
         Server* owner = nullptr;
     };
     void loop();
+    int request_ppl(int npoints) const;
     int launch(Slot& s, std::vector<Req*>& batch, std::string* err);
     void finish(Slot& s);
     size_t footprint(const Req& q) const;

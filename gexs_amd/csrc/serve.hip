@@ -153,8 +153,8 @@ int Server::submit(const Request& r, uint64_t* id, std::string* err) {
     std::unique_ptr<Req> q(new Req());
     q->r = r;
     q->npoints = (int)npf;
-    const int span = GX_WAVE * ph_.ppl;
-    q->ntiles = (q->npoints + span - 1) / span;
+    q->ppl = request_ppl(q->npoints);
+    q->ntiles = (q->npoints + GX_WAVE * q->ppl - 1) / (GX_WAVE * q->ppl);
     // line selection, Profiles/BaseProfile.cpp:152-157
     const double lo = r.start - ph_.half_width, hi = r.end + ph_.half_width;
     auto a = std::lower_bound(v_.nu_host.begin(), v_.nu_host.end(), lo);
@@ -225,10 +225,23 @@ void Server::finish(Slot& s) {
     s.busy = false;
 }
 
+// Tile width of a request: the widest tile up to the configured one that
+// still gives the device's SIMDs a few waves each (16 K waves); small
+// requests take narrow tiles, so their lines spread over more waves.  It
+// depends on the request alone, so a result is the same in any batch.
+int Server::request_ppl(int npoints) const {
+    const int64_t want_waves = 16 * 1024;
+    for (int ppl = ph_.ppl; ppl > 1; ppl /= 2)
+        if ((int64_t)npoints / (GX_WAVE * ppl) * ph_.split >= want_waves) return ppl;
+    return 1;
+}
+
 int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     const int nreq = (int)batch.size();
     hipStream_t st = s.stream;
     char* base = arena_.base();
+    // requests grouped by tile width (one tiles / profile launch per group)
+    std::stable_sort(batch.begin(), batch.end(), [](const Req* a, const Req* b) { return a->ppl > b->ppl; });
     // ---- host tables; each request's arena region holds its positions,
     // records and output grid
     std::vector<TPParams> tps((size_t)nreq);
@@ -354,15 +367,29 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
                                nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval)) !=
             hipSuccess)
             goto fail;
-        if ((e = gxk::tiles(st, (const LinePos*)base, d_jd, nreq, max_tiles, d_tp, GX_WAVE * ph_.ppl,
-                            d_items, nullptr, nullptr)) != hipSuccess)
-            goto fail;
+        for (int k0 = 0; k0 < nreq;) {  // tiles per tile-width group
+            int k1 = k0, mt = 0;
+            while (k1 < nreq && batch[(size_t)k1]->ppl == batch[(size_t)k0]->ppl)
+                mt = std::max(mt, batch[(size_t)k1++]->ntiles);
+            if ((e = gxk::tiles(st, (const LinePos*)base, d_jd, k1 - k0, mt, d_tp,
+                                GX_WAVE * batch[(size_t)k0]->ppl, d_items, nullptr, nullptr, k0)) !=
+                hipSuccess)
+                goto fail;
+            k0 = k1;
+        }
         if ((e = gxk::segments(st, d_items, items_total, d_jd, (const LinePos*)base, d_kmax, ph_.profile)) !=
             hipSuccess)
             goto fail;
-        if ((e = gxk::profile_tiles(st, ph_.profile, ph_.ppl, ph_.split, d_items, items_total, d_jobs, d_scale,
-                                    d_counter, nullptr, 3, 0)) != hipSuccess)  // slots in dispatch order
-            goto fail;
+        for (int k0 = 0; k0 < nreq;) {  // one profile launch per group, slots in dispatch order
+            int k1 = k0, i1 = jds[(size_t)k0].items_off;
+            while (k1 < nreq && batch[(size_t)k1]->ppl == batch[(size_t)k0]->ppl)
+                i1 += batch[(size_t)k1++]->ntiles;
+            const int i0 = jds[(size_t)k0].items_off;
+            if ((e = gxk::profile_tiles(st, ph_.profile, batch[(size_t)k0]->ppl, ph_.split, d_items + i0,
+                                        i1 - i0, d_jobs, d_scale, d_counter, nullptr, 3, 0)) != hipSuccess)
+                goto fail;
+            k0 = k1;
+        }
         size_t hoff = 0;
         for (int k = 0; k < nreq; ++k) {
             const size_t np = (size_t)batch[(size_t)k]->npoints;
