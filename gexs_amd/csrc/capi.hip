@@ -852,7 +852,8 @@ int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order, int waves_per_t
     if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4 && points_per_lane != 8)
         return c->fail(GEXS_EINVAL, "points_per_lane must be 1, 2, 4 or 8");
     if (order < 0 || order > 3) return c->fail(GEXS_EINVAL, "order must be 0, 1, 2 or 3");
-    if (waves_per_tile != 1 && waves_per_tile != 4) return c->fail(GEXS_EINVAL, "waves_per_tile must be 1 or 4");
+    if (waves_per_tile != 1 && waves_per_tile != 4 && waves_per_tile != 16)
+        return c->fail(GEXS_EINVAL, "waves_per_tile must be 1, 4 or 16");
     c->ppl = points_per_lane;
     c->order = order;
     c->split = waves_per_tile;
@@ -1185,7 +1186,7 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
     top.kernel_ms = 1e300;
     int nrep = 0;
     for (int pi = 0; pi < 4; ++pi)
-        for (int split = 1; split <= 4; split += 3)
+        for (int split = 1; split <= 16; split *= 4)
         for (int order = 0; order < 4; ++order)
             for (int pb = 0; pb < 3; ++pb) {
                 if (order == 2 && pb == 0) continue;  // per-XCD queues are persistent

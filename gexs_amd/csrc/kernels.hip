@@ -611,27 +611,31 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, doub
     }
 }
 
-constexpr int kSplit = GX_WG / GX_WAVE;  // waves of a workgroup
+constexpr int kWavesPerBlock = GX_WG / GX_WAVE;  // SPLIT 1: items per workgroup
 
-// the kSplit waves of the workgroup take contiguous quarters of the item's
+// workgroup size of a split: 4 waves (SPLIT 1 and 4) or 16 waves (SPLIT 16)
+template <int SPLIT>
+constexpr int block_threads() { return SPLIT > 4 ? SPLIT * GX_WAVE : GX_WG; }
+
+// the SPLIT waves of the workgroup take contiguous shares of the item's
 // lines; their FP64 partial sums are added in wave order through LDS
-template <int PROF, int PPL>
+template <int PROF, int PPL, int SPLIT>
 __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb,
                                                double inv_scale, int lane, int wv,
                                                double* red) {
     TileAcc<PPL> t;
     const int n = max(it.hi - it.lo, 0);
-    const int ca = it.lo + (int)(((long long)n * wv) / kSplit);
-    const int cb = it.lo + (int)(((long long)n * (wv + 1)) / kSplit);
+    const int ca = it.lo + (int)(((long long)n * wv) / SPLIT);
+    const int cb = it.lo + (int)(((long long)n * (wv + 1)) / SPLIT);
     sum_lines<PROF, PPL>(t, it, jb, lane, ca, cb);
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
         red[(size_t)wv * GX_WAVE * PPL + GX_WAVE * q + lane] = (double)t.sum[q] - (double)t.cmp[q];
     __syncthreads();
-    for (int k = threadIdx.x; k < GX_WAVE * PPL; k += GX_WG) {
+    for (int k = threadIdx.x; k < GX_WAVE * PPL; k += block_threads<SPLIT>()) {
         double v = red[k];
 #pragma unroll
-        for (int w = 1; w < kSplit; ++w) v += red[(size_t)w * GX_WAVE * PPL + k];
+        for (int w = 1; w < SPLIT; ++w) v += red[(size_t)w * GX_WAVE * PPL + k];
         const int j = it.w0 + k;
         if (j < it.w1) store_point(jb, j, v * inv_scale);
     }
@@ -648,7 +652,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 }  // namespace
 
-// mode 0: one item per wave (SPLIT 1) or per workgroup (SPLIT 4), blocks in
+// mode 0: one item per wave (SPLIT 1) or per workgroup (SPLIT 4, 16), blocks in
 // XCD-contiguous order; 3: as 0 in dispatch order (consecutive items on
 // different XCDs); 1: persistent, items pulled from one counter; 2:
 // persistent, items pulled from the queue of the workgroup's XCD (items in
@@ -656,19 +660,19 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // the other queues.  xq = GX_XCDS counters (zero at launch), xqb = the
 // GX_XCDS + 1 queue bounds.
 template <int PROF, int PPL, int SPLIT>
-__global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restrict__ items,
+__global__ __launch_bounds__(block_threads<SPLIT>()) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
                                                          const unsigned long long* __restrict__ scale_bits,
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
-    __shared__ double red[SPLIT > 1 ? kSplit * GX_WAVE * PPL : 1];
+    __shared__ double red[SPLIT > 1 ? SPLIT * GX_WAVE * PPL : 1];
     __shared__ int grab;
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = mode == 3 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (mode == 0 || mode == 3) {
         if (SPLIT == 1) {
-            const int idx = lb * kSplit + wv;
+            const int idx = lb * kWavesPerBlock + wv;
             if (idx >= nitems) return;
             const WorkItem it = items[idx];
             const Job jb = jobs[it.job];
@@ -677,8 +681,8 @@ __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restr
             if (lb >= nitems) return;
             const WorkItem it = items[lb];
             const Job jb = jobs[it.job];
-            run_item_split<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane, wv,
-                                      red);
+            run_item_split<PROF, PPL, SPLIT>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane,
+                                             wv, red);
         }
         return;
     }
@@ -706,8 +710,8 @@ __global__ __launch_bounds__(GX_WG) void k_profile_tiles(const WorkItem* __restr
             if (SPLIT == 1)
                 run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
             else
-                run_item_split<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane,
-                                          wv, red);
+                run_item_split<PROF, PPL, SPLIT>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]),
+                                                 lane, wv, red);
         }
     }
 }
@@ -919,11 +923,11 @@ static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitem
                                  const Job* jobs, const unsigned long long* scale_bits,
                                  unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     if (nitems <= 0) return hipSuccess;
-    const int ipb = SPLIT == 1 ? GX_WG / GX_WAVE : 1;  // items per block
+    const int ipb = SPLIT == 1 ? kWavesPerBlock : 1;  // items per block
     const bool slots = mode == 0 || mode == 3;  // one launch slot per item
     const int nb = slots ? blocks_for(nitems, ipb)
                          : std::max(1, std::min(blocks, blocks_for(nitems, ipb)));
-    hipLaunchKernelGGL((k_profile_tiles<PROF, PPL, SPLIT>), dim3(nb), dim3(GX_WG), 0, s, items,
+    hipLaunchKernelGGL((k_profile_tiles<PROF, PPL, SPLIT>), dim3(nb), dim3(block_threads<SPLIT>()), 0, s, items,
                        nitems, jobs, scale_bits, xq, xqb, mode);
     return hipGetLastError();
 }
@@ -946,8 +950,9 @@ static hipError_t launch_prof(hipStream_t s, int ppl, int split, const WorkItem*
                               int nitems, const Job* jobs, const unsigned long long* scale_bits,
                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
-    if (split == kSplit)
-        return launch_ppl<PROF, kSplit>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+    if (split == 4) return launch_ppl<PROF, 4>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+    if (split == 16)
+        return launch_ppl<PROF, 16>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
     return hipErrorInvalidValue;
 }
 

@@ -99,7 +99,8 @@ class Server {
         uint64_t id;
         Request r;
         int npoints = 0, ntiles = 0;
-        int ppl = 1;  // tile width 64*ppl points, from the request's own size
+        int ppl = 1;    // tile width 64*ppl points and
+        int split = 4;  // waves per tile, both from the request's own size
         int64_t l0 = 0, n = 0;
         size_t foot = 0, off = 0;
         std::chrono::steady_clock::time_point arrival;
@@ -123,7 +124,7 @@ class Server {
         Server* owner = nullptr;
     };
     void loop();
-    int request_ppl(int npoints) const;
+    void request_shape(Req& q) const;
     int launch(Slot& s, std::vector<Req*>& batch, std::string* err);
     void finish(Slot& s);
     size_t footprint(const Req& q) const;

@@ -153,8 +153,7 @@ int Server::submit(const Request& r, uint64_t* id, std::string* err) {
     std::unique_ptr<Req> q(new Req());
     q->r = r;
     q->npoints = (int)npf;
-    q->ppl = request_ppl(q->npoints);
-    q->ntiles = (q->npoints + GX_WAVE * q->ppl - 1) / (GX_WAVE * q->ppl);
+    request_shape(*q);
     // line selection, Profiles/BaseProfile.cpp:152-157
     const double lo = r.start - ph_.half_width, hi = r.end + ph_.half_width;
     auto a = std::lower_bound(v_.nu_host.begin(), v_.nu_host.end(), lo);
@@ -225,23 +224,32 @@ void Server::finish(Slot& s) {
     s.busy = false;
 }
 
-// Tile width of a request: the widest tile up to the configured one that
-// still gives the device's SIMDs a few waves each (16 K waves); small
-// requests take narrow tiles, so their lines spread over more waves.  It
-// depends on the request alone, so a result is the same in any batch.
-int Server::request_ppl(int npoints) const {
+// Launch shape of a request: the widest tile up to the configured one that
+// still gives the device's SIMDs a few waves each (16 K waves), and 16 waves
+// per tile when even 64-point tiles give fewer; small requests thus spread
+// their lines over more waves.  The shape depends on the request alone, so a
+// result is the same in any batch.
+void Server::request_shape(Req& q) const {
     const int64_t want_waves = 16 * 1024;
+    const int split = std::max(ph_.split, 1);
+    q.ppl = 1;
     for (int ppl = ph_.ppl; ppl > 1; ppl /= 2)
-        if ((int64_t)npoints / (GX_WAVE * ppl) * ph_.split >= want_waves) return ppl;
-    return 1;
+        if ((int64_t)q.npoints / (GX_WAVE * ppl) * split >= want_waves) {
+            q.ppl = ppl;
+            break;
+        }
+    q.ntiles = (q.npoints + GX_WAVE * q.ppl - 1) / (GX_WAVE * q.ppl);
+    q.split = (int64_t)q.ntiles * split >= want_waves ? split : 16;
 }
 
 int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     const int nreq = (int)batch.size();
     hipStream_t st = s.stream;
     char* base = arena_.base();
-    // requests grouped by tile width (one tiles / profile launch per group)
-    std::stable_sort(batch.begin(), batch.end(), [](const Req* a, const Req* b) { return a->ppl > b->ppl; });
+    // requests grouped by launch shape (one tiles / profile launch per group)
+    auto shape = [](const Req* q) { return q->ppl * 64 + q->split; };
+    std::stable_sort(batch.begin(), batch.end(),
+                     [&](const Req* a, const Req* b) { return shape(a) > shape(b); });
     // ---- host tables; each request's arena region holds its positions,
     // records and output grid
     std::vector<TPParams> tps((size_t)nreq);
@@ -369,7 +377,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             goto fail;
         for (int k0 = 0; k0 < nreq;) {  // tiles per tile-width group
             int k1 = k0, mt = 0;
-            while (k1 < nreq && batch[(size_t)k1]->ppl == batch[(size_t)k0]->ppl)
+            while (k1 < nreq && shape(batch[(size_t)k1]) == shape(batch[(size_t)k0]))
                 mt = std::max(mt, batch[(size_t)k1++]->ntiles);
             if ((e = gxk::tiles(st, (const LinePos*)base, d_jd, k1 - k0, mt, d_tp,
                                 GX_WAVE * batch[(size_t)k0]->ppl, d_items, nullptr, nullptr, k0)) !=
@@ -382,10 +390,11 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             goto fail;
         for (int k0 = 0; k0 < nreq;) {  // one profile launch per group, slots in dispatch order
             int k1 = k0, i1 = jds[(size_t)k0].items_off;
-            while (k1 < nreq && batch[(size_t)k1]->ppl == batch[(size_t)k0]->ppl)
+            while (k1 < nreq && shape(batch[(size_t)k1]) == shape(batch[(size_t)k0]))
                 i1 += batch[(size_t)k1++]->ntiles;
             const int i0 = jds[(size_t)k0].items_off;
-            if ((e = gxk::profile_tiles(st, ph_.profile, batch[(size_t)k0]->ppl, ph_.split, d_items + i0,
+            if ((e = gxk::profile_tiles(st, ph_.profile, batch[(size_t)k0]->ppl, batch[(size_t)k0]->split,
+                                        d_items + i0,
                                         i1 - i0, d_jobs, d_scale, d_counter, nullptr, 3, 0)) != hipSuccess)
                 goto fail;
             k0 = k1;

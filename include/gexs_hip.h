@@ -145,7 +145,7 @@ int gexs_pass_lines(gexs_ctx* ctx, int64_t* first, int64_t* count);
 /* Tuning of the FP32C profile kernel: points per lane (1, 2, 4 or 8; a
  * wave covers 64*ppl grid points), the dispatch order of the tiles (0 = grid
  * order, 1 = heaviest first, 2 = per-XCD queues of neighbouring tiles) and
- * the waves that share one tile's lines (1 or 4). */
+ * the waves that share one tile's lines (1, 4 or 16). */
 int gexs_set_tuning(gexs_ctx* ctx, int points_per_lane, int order, int waves_per_tile);
 
 /* ------------------------------------------------------ resident line list
@@ -209,9 +209,9 @@ int gexs_batch_device_out(gexs_ctx* ctx, const double** out, int64_t* count);
 int gexs_set_launch(gexs_ctx* ctx, int persistent_blocks, int use_graph);
 
 /* Autotuner of the FP32C profile kernel: runs the batch once per candidate
- * (points per lane 1, 2, 4, 8 = tile widths of 64..512 points; 1 or 4 waves
- * per tile; grid, heaviest-first or per-XCD tile order; one launch slot per
- * tile or 4 / 8 persistent workgroups per CU), times the profile kernel with
+ * (points per lane 1, 2, 4, 8 = tile widths of 64..512 points; 1, 4 or 16
+ * waves per tile; grid, heaviest-first or per-XCD tile order; one launch slot
+ * per tile or 4 / 8 persistent workgroups per CU), times the profile kernel with
  * HIP events, and keeps the fastest on the ctx.  report (may be NULL)
  * receives up to report_cap candidates; *n_report their count. */
 typedef struct gexs_tuning {

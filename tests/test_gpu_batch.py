@@ -62,14 +62,14 @@ def test_persistent_equals_wave_per_tile(batch_mgr):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("split", [1, 4])
+@pytest.mark.parametrize("split", [1, 4, 16])
 def test_launch_modes_agree_bitwise(batch_mgr, split):
     """For one tile width and split, every launch shape and tile order sums
     each point in the same order."""
     L, m = batch_mgr
     tps = table(6)
     outs = []
-    for order, pb in [(0, 0), (1, 0), (0, 1024), (1, 2048), (2, 1024)]:
+    for order, pb in [(0, 0), (1, 0), (3, 0), (0, 1024), (1, 2048), (2, 1024)]:
         m.set_tuning(4, order, split)
         m.set_launch(pb, False)
         outs.append(m.run_batch(tps, *GRID))
@@ -157,8 +157,8 @@ def test_autotune_picks_a_candidate(batch_mgr):
     tps = table(8)
     ref = m.run_batch(tps, *GRID)
     best, cands = m.autotune(tps, *GRID)
-    # ppl {1,2,4,8} x waves per tile {1,4} x 9 launch shapes
-    assert len(cands) == 72
+    # ppl {1,2,4,8} x waves per tile {1,4,16} x 9 launch shapes
+    assert len(cands) == 108
     assert best["kernel_ms"] == min(c["kernel_ms"] for c in cands)
     print("autotune best", best, flush=True)
     out = m.run_batch(tps, *GRID)

@@ -78,7 +78,8 @@ def c1_small():
     return L, L.lines()
 
 
-@pytest.mark.parametrize("ppl,split", [(1, 1), (4, 1), (8, 1), (1, 4), (4, 4), (8, 4)])
+@pytest.mark.parametrize("ppl,split", [(1, 1), (4, 1), (8, 1), (1, 4), (4, 4), (8, 4), (1, 16),
+                                       (4, 16)])
 def test_voigt_fp32c_vs_oracle(O, c1_small, ppl, split):
     L, lines = c1_small
     T, P = 296.0, 1.0
