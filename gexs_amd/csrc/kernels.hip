@@ -377,14 +377,23 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 // the constant address space lets the wave-uniform loads below issue as
 // scalar loads (s_load) into SGPRs, so per-line decisions are scalar branches.
 // (The host pass of this single-source file only parses device code.)
+// The batched loop below reads them with vector loads through global-space
+// pointers (plain pointers would be flat accesses, whose waits cannot be
+// counted per load).
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) LinePos* ConstPos;
 typedef const __attribute__((address_space(4))) VRec* ConstRec;
 typedef const __attribute__((address_space(4))) VRecX* ConstRecX;
+typedef const __attribute__((address_space(1))) LinePos* GlobPos;
+typedef const __attribute__((address_space(1))) VRec* GlobRec;
+typedef const __attribute__((address_space(1))) VRecX* GlobRecX;
 #else
 typedef const LinePos* ConstPos;
 typedef const VRec* ConstRec;
 typedef const VRecX* ConstRecX;
+typedef const LinePos* GlobPos;
+typedef const VRec* GlobRec;
+typedef const VRecX* GlobRecX;
 #endif
 
 // Region-I value (times the line amplitude) at x: Kl (s + 1) / (s^2 + H),
@@ -487,6 +496,63 @@ __device__ __forceinline__ int slice_in_window(int j0, const LinePos& lp) {
 // names; slices wholly outside the window are skipped, slices wholly inside
 // are unmasked), region I inline, regions II-IV per lane; the next line's
 // records load while this one is evaluated
+// one line with per-line decisions: window masks per lane (only the edges
+// MASK names; slices wholly outside the window are skipped, slices wholly
+// inside are unmasked), region I inline, regions II-IV per lane
+template <int PROF, int PPL, int MASK>
+__device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp, const VRec& r,
+                                             const VRecX& rx, int w0, int w1) {
+    if (PROF == GX_PROF_VOIGT) {
+        const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
+        if (kmin >= rx.k15) {
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+                if (st == 0) continue;
+                const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
+                if (st == 2)
+                    t.part[q] += v;
+                else
+                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+                if (st == 0) continue;
+                const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
+                float v = region1(x, r.Kl, r.E2, r.H);
+                if (x + rx.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, rx.y);
+                if (st == 2)
+                    t.part[q] += v;
+                else
+                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) {
+            const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+            if (st == 0) continue;
+            if (st == 2 || in_window<MASK>(t.jj[q], lp)) {
+                const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
+                float k;
+                if (PROF == GX_PROF_VOIGT_QUAD) {
+                    const float rr = x * x + rx.y * rx.y;
+                    k = rr < 100.0f ? (float)gx::voigt916_f64((double)x, (double)rx.y)
+                                    : gx::gh11_f32(x, rx.y);
+                } else {
+                    k = gx::bin_erf_f32(x, rx.y);
+                }
+                t.part[q] = fmaf(r.Kg, k, t.part[q]);
+            }
+        }
+    }
+    t.tick(1);
+}
+
+// lines [a, b) through checked_line, records by scalar loads; the next
+// line's records load while this one is evaluated
 template <int PROF, int PPL, int MASK>
 __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, ConstRec rec,
                                               ConstRecX recx, int a, int b, int w0, int w1) {
@@ -503,53 +569,76 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
             rn = rec[i + 1];
             rxn = recx[i + 1];
         }
-        if (PROF == GX_PROF_VOIGT) {
-            const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
-            if (kmin >= rx.k15) {
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
-                    if (st == 0) continue;
-                    const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
-                    if (st == 2)
-                        t.part[q] += v;
-                    else
-                        t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
-                    if (st == 0) continue;
-                    const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
-                    float v = region1(x, r.Kl, r.E2, r.H);
-                    if (x + rx.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, rx.y);
-                    if (st == 2)
-                        t.part[q] += v;
-                    else
-                        t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < PPL; ++q) {
-                const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
-                if (st == 0) continue;
-                if (st == 2 || in_window<MASK>(t.jj[q], lp)) {
-                    const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
-                    float k;
-                    if (PROF == GX_PROF_VOIGT_QUAD) {
-                        const float rr = x * x + rx.y * rx.y;
-                        k = rr < 100.0f ? (float)gx::voigt916_f64((double)x, (double)rx.y)
-                                        : gx::gh11_f32(x, rx.y);
-                    } else {
-                        k = gx::bin_erf_f32(x, rx.y);
-                    }
-                    t.part[q] = fmaf(r.Kg, k, t.part[q]);
-                }
+        checked_line<PROF, PPL, MASK>(t, lp, r, rx, w0, w1);
+    }
+}
+
+__device__ __forceinline__ float bcast(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ int bcast(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+
+// One point per lane (64-point tiles): a line's evaluation is a handful of
+// VALU instructions, too few to cover a scalar load's latency when few waves
+// share a SIMD.  Here lane k loads the records of line base + k with vector
+// loads (64 lines per batch, the next batch in flight while this one is
+// evaluated) and each line is broadcast to the wave with readlane.
+// FAST: the lines of a region-I segment (no decisions, only VRec is read).
+template <int PROF, int MASK, bool FAST>
+__device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRec rec,
+                                              GlobRecX recx, int a, int b, int w0, int w1,
+                                              int lane) {
+    if (a >= b) return;
+    LinePos pc{}, pn{};
+    VRec rc, rn;
+    VRecX xc{}, xn{};
+    {
+        const int li = min(a + lane, b - 1);
+        rc = rec[li];
+        if (!FAST) {
+            pc = pos[li];
+            xc = recx[li];
+        }
+    }
+    for (int base = a; base < b; base += GX_WAVE) {
+        const int nb = min(GX_WAVE, b - base);
+        {
+            // unconditional (the last batch reloads line b - 1), so every
+            // iteration has the same loads in flight and the wait before the
+            // broadcasts covers only the current batch
+            const int li = min(base + GX_WAVE + lane, b - 1);
+            rn = rec[li];
+            if (!FAST) {
+                pn = pos[li];
+                xn = recx[li];
             }
         }
-        t.tick(1);
+        for (int k = 0; k < nb; ++k) {
+            VRec r;
+            r.cf = bcast(rc.cf, k);
+            r.A = bcast(rc.A, k);
+            r.B = bcast(rc.B, k);
+            r.Kl = bcast(rc.Kl, k);
+            r.E2 = bcast(rc.E2, k);
+            r.H = bcast(rc.H, k);
+            if (FAST) {
+                t.part[0] += region1(fmaf(r.cf - t.jf[0], r.A, r.B), r.Kl, r.E2, r.H);
+                t.tick(1);
+            } else {
+                r.Kg = bcast(rc.Kg, k);
+                LinePos lp;
+                lp.ib = bcast(pc.ib, k);
+                lp.ie = bcast(pc.ie, k);
+                lp.c = bcast(pc.c, k);
+                VRecX rx;
+                rx.y = bcast(xc.y, k);
+                rx.k15 = bcast(xc.k15, k);
+                checked_line<PROF, 1, MASK>(t, lp, r, rx, w0, w1);
+            }
+        }
+        rc = rn;
+        pc = pn;
+        xc = xn;
     }
 }
 
@@ -571,7 +660,23 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
     }
     t.n = 0;
     const int w0 = it.w0, w1 = it.w1;
-    if (PROF == GX_PROF_VOIGT) {
+    if constexpr (PPL == 1) {
+        // records by vector loads, broadcast per line
+        const GlobPos gpos = (GlobPos)jb.pos;
+        const GlobRec grec = (GlobRec)jb.rec;
+        const GlobRecX grecx = (GlobRecX)jb.recx;
+        if (PROF == GX_PROF_VOIGT && it.s1 < it.s4) {
+            const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
+            const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
+            lines_batched<PROF, kMaskHi, false>(t, gpos, grec, grecx, ca, s1, w0, w1, lane);
+            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s1, s2, w0, w1, lane);
+            lines_batched<PROF, kMaskNone, false>(t, gpos, grec, grecx, s2, s3, w0, w1, lane);
+            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s3, s4, w0, w1, lane);
+            lines_batched<PROF, kMaskLo, false>(t, gpos, grec, grecx, s4, cb, w0, w1, lane);
+        } else {
+            lines_batched<PROF, kMaskBoth, false>(t, gpos, grec, grecx, ca, cb, w0, w1, lane);
+        }
+    } else if (PROF == GX_PROF_VOIGT) {
         const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
         const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
         if (it.s1 < it.s4) {
