@@ -225,12 +225,12 @@ void Server::finish(Slot& s) {
 }
 
 // Launch shape of a request: the widest tile up to the configured one that
-// still gives the device's SIMDs a few waves each (16 K waves), and 16 waves
-// per tile when even 64-point tiles give fewer; small requests thus spread
-// their lines over more waves.  The shape depends on the request alone, so a
+// still gives every SIMD of the device a wave (8 K waves), and 16 waves per
+// tile when even 64-point tiles give fewer; small requests thus spread their
+// lines over more waves.  The shape depends on the request alone, so a
 // result is the same in any batch.
 void Server::request_shape(Req& q) const {
-    const int64_t want_waves = 16 * 1024;
+    const int64_t want_waves = 8 * 1024;
     const int split = std::max(ph_.split, 1);
     q.ppl = 1;
     for (int ppl = ph_.ppl; ppl > 1; ppl /= 2)
