@@ -306,7 +306,22 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.s1 = lf < rf ? lf : hi;
     it.s4 = lf < rf ? rf : hi;
     it.s2 = it.s3 = it.s4;  // [s1, s4) all region I until k_segments runs
-    it.pad[0] = it.pad[1] = it.pad[2] = 0;
+    // far lines: centre c < w0 - span (before f1) or c >= w1 + span (from f2);
+    // c is non-decreasing in nu
+    a = it.s1;
+    b = it.s4;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if ((long long)jp[m].c >= (long long)w0 - span) b = m; else a = m + 1;
+    }
+    it.f1 = a;
+    b = it.s4;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if ((long long)jp[m].c >= (long long)w1 + span) b = m; else a = m + 1;
+    }
+    it.f2 = a;
+    it.pad = 0;
     const int slot = d.items_off + t;
     items[slot] = it;
     if (cost_key) {
@@ -465,6 +480,51 @@ __device__ __forceinline__ void voigt_fast(TileAcc<PPL>& t, ConstRec rec, int a,
     }
     for (; i < b; ++i) {
         region1_line(t, rec[i]);
+        t.tick(1);
+    }
+}
+
+// lines [a, b) of a region-I segment whose centre is a tile width or more
+// away from the tile: per line X0 = fma(c - w0, A, B) (c - w0 exact), per
+// point x = X0 - l*A with l = j - w0 (nl = -l).  For these lines |X0| <= 2|x|
+// at every point of the tile, so x keeps FP32 accuracy (within 3 ulp); it
+// saves one instruction per point against voigt_fast.
+template <int PPL>
+__device__ __forceinline__ void far_line(TileAcc<PPL>& t, const VRec& r, float w0f,
+                                         const float* nl) {
+    const float X0 = fmaf(r.cf - w0f, r.A, r.B);
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) t.part[q] += region1(fmaf(nl[q], r.A, X0), r.Kl, r.E2, r.H);
+}
+
+template <int PPL>
+__device__ __forceinline__ void voigt_far(TileAcc<PPL>& t, ConstRec rec, int a, int b, float w0f,
+                                          const float* nl) {
+    int i = a;
+    if (b - a >= 8) {
+        VRec c0 = rec[i], c1 = rec[i + 1], c2 = rec[i + 2], c3 = rec[i + 3];
+        for (; i + 8 <= b; i += 4) {
+            const VRec n0 = rec[i + 4], n1 = rec[i + 5], n2 = rec[i + 6], n3 = rec[i + 7];
+            __builtin_amdgcn_sched_barrier(0);
+            far_line(t, c0, w0f, nl);
+            far_line(t, c1, w0f, nl);
+            far_line(t, c2, w0f, nl);
+            far_line(t, c3, w0f, nl);
+            t.tick(4);
+            c0 = n0;
+            c1 = n1;
+            c2 = n2;
+            c3 = n3;
+        }
+        far_line(t, c0, w0f, nl);
+        far_line(t, c1, w0f, nl);
+        far_line(t, c2, w0f, nl);
+        far_line(t, c3, w0f, nl);
+        t.tick(4);
+        i += 4;
+    }
+    for (; i < b; ++i) {
+        far_line(t, rec[i], w0f, nl);
         t.tick(1);
     }
 }
@@ -681,11 +741,20 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
         if (it.s1 < it.s4) {
             // [lo, s1): ib <= w0 < ie < w1; [s1, s4): ib <= w0, ie >= w1;
-            // [s4, hi): w0 < ib < w1 <= ie
+            // [s4, hi): w0 < ib < w1 <= ie.  In [s1, s2) the lines before f1
+            // and in [s3, s4) those from f2 on are far from the tile.
+            const int fl = min(max(min(it.f1, it.s2), ca), cb);
+            const int fr = min(max(max(it.f2, it.s3), ca), cb);
+            float nl[PPL];
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) nl[q] = -(float)(lane + GX_WAVE * q);
+            const float w0f = (float)w0;
             lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
-            voigt_fast<PPL>(t, rec, s1, s2);
+            voigt_far<PPL>(t, rec, s1, fl, w0f, nl);
+            voigt_fast<PPL>(t, rec, fl, s2);
             lines_checked<PROF, PPL, kMaskNone>(t, pos, rec, recx, s2, s3, w0, w1);
-            voigt_fast<PPL>(t, rec, s3, s4);
+            voigt_fast<PPL>(t, rec, s3, fr);
+            voigt_far<PPL>(t, rec, fr, s4, w0f, nl);
             lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
         } else {
             lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
