@@ -157,7 +157,21 @@ struct gexs_ctx {
     DevArr small;    // per-pass counters [xq | scale | neval | kmax] (one fill)
     DevArr neval;    // N_eval of the plain FP64 pass
     DevArr counter;  // bounds of the per-XCD tile queues
+    DevArr item_tab; // window and job tables of the tiles kernel
     const void* last_neval = nullptr;  // N_eval of the last pass (device)
+
+    // execute path: the per-line pre-pass of a pass runs on `pre` into one of
+    // two buffer sets while the profile kernel of the previous pass runs on
+    // `stream` from the other set
+    hipStream_t pre = nullptr;
+    struct PassBuf {
+        DevArr tab, small, rec, recx, items;
+        hipEvent_t pre_done = nullptr, prof_done = nullptr;
+        bool used = false;
+    };
+    PassBuf pb[2];
+    int pbuf = 0;
+    hipEvent_t geom = nullptr;  // on `stream` after positions and tiles were (re)built
     int cus = 256;  // compute units of the device
     PosKey pos_key;
     PosKey items_key;
@@ -320,6 +334,7 @@ int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
                             c->pos_tp.as<TPParams>(), c->pos.as<LinePos>()));
     c->pos_key = key;
     c->items_key = PosKey();  // tiles depend on positions
+    HIPCHK(c, hipEventRecord(c->geom, c->stream));
     return GEXS_OK;
 }
 
@@ -340,20 +355,20 @@ struct PassTables {
 
 size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 
-int pass_tables(gexs_ctx* c, int njobs, PassTables* t) {
+int pass_tables_in(gexs_ctx* c, DevArr& tab, DevArr& small, int njobs, PassTables* t) {
     const size_t tp_b = al256((size_t)njobs * sizeof(TPParams));
     const size_t jd_b = al256((size_t)njobs * sizeof(JobDesc));
     const size_t jobs_b = al256((size_t)njobs * sizeof(Job));
-    HIPCHK(c, c->tab.ensure(tp_b + jd_b + jobs_b));
-    HIPCHK(c, c->small.ensure((4 + 3 * (size_t)njobs) * 8));
-    char* b = (char*)c->tab.p;
+    HIPCHK(c, tab.ensure(tp_b + jd_b + jobs_b));
+    HIPCHK(c, small.ensure((4 + 3 * (size_t)njobs) * 8));
+    char* b = (char*)tab.p;
     t->jd_off = tp_b;
     t->jobs_off = tp_b + jd_b;
     t->bytes = tp_b + jd_b + jobs_b;
     t->tp = (TPParams*)b;
     t->jd = (JobDesc*)(b + t->jd_off);
     t->jobs = (Job*)(b + t->jobs_off);
-    char* s = (char*)c->small.p;
+    char* s = (char*)small.p;
     t->xq = (unsigned int*)s;
     t->scale = (unsigned long long*)(s + 32);
     t->neval = t->scale + njobs;
@@ -361,6 +376,8 @@ int pass_tables(gexs_ctx* c, int njobs, PassTables* t) {
     t->zero_u64 = 4 + 3 * (int64_t)njobs;
     return GEXS_OK;
 }
+
+int pass_tables(gexs_ctx* c, int njobs, PassTables* t) { return pass_tables_in(c, c->tab, c->small, njobs, t); }
 
 int tiles_per_job(const gexs_ctx* c) {
     const int span = GX_WAVE * c->ppl;
@@ -382,10 +399,12 @@ int launch_blocks(const gexs_ctx* c) {
     return 8 * c->cus;
 }
 
-// work items for the njobs jobs of the pass tables (one grid window); sorted
+// work items for njobs jobs over one grid window, from the jobs' window and
+// job tables (host copies, uploaded when the items are rebuilt); sorted
 // heaviest first when order == 1; with order == 2 in grid order and split
-// into GX_XCDS contiguous ranges of equal line count for the per-XCD queues
-int ensure_items(gexs_ctx* c, int njobs) {
+// into GX_XCDS contiguous ranges of equal line count for the per-XCD queues.
+// Rebuilds run on c->stream and record c->geom.
+int ensure_items(gexs_ctx* c, int njobs, const TPParams* htp, const JobDesc* hjd) {
     const int span = GX_WAVE * c->ppl;
     const PosKey key = c->pos_key;
     if (key == c->items_key && c->items_span == span && c->items_order == c->order &&
@@ -399,11 +418,13 @@ int ensure_items(gexs_ctx* c, int njobs) {
     HIPCHK(c, c->item_idx.ensure((size_t)std::max(total, 1) * 4));
     const bool sorted = c->order == 1 && total > 1;
     WorkItem* dst = sorted ? c->items_tmp.as<WorkItem>() : c->items.as<WorkItem>();
-    PassTables t;
-    int rc;
-    if ((rc = pass_tables(c, njobs, &t)) != GEXS_OK) return rc;
-    HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), t.jd, njobs, per,
-                         t.tp, span, dst, sorted ? c->cost_key.as<uint32_t>() : nullptr,
+    const size_t tp_b = al256((size_t)njobs * sizeof(TPParams));
+    HIPCHK(c, c->item_tab.ensure(tp_b + (size_t)njobs * sizeof(JobDesc)));
+    HIPCHK(c, c->staging.upload(c->item_tab.p, htp, (size_t)njobs * sizeof(TPParams), c->stream));
+    HIPCHK(c, c->staging.upload((char*)c->item_tab.p + tp_b, hjd, (size_t)njobs * sizeof(JobDesc), c->stream));
+    HIPCHK(c, gxk::tiles(c->stream, c->pos.as<LinePos>(), (const JobDesc*)((char*)c->item_tab.p + tp_b), njobs,
+                         per, c->item_tab.as<TPParams>(), span, dst,
+                         sorted ? c->cost_key.as<uint32_t>() : nullptr,
                          sorted ? c->item_idx.as<uint32_t>() : nullptr));
     if (sorted) {
         HIPCHK(c, gxk::radix_sort_u32(c->stream, c->cost_key.as<uint32_t>(), c->item_idx.as<uint32_t>(),
@@ -437,6 +458,7 @@ int ensure_items(gexs_ctx* c, int njobs) {
     c->items_order = c->order;
     c->items_njobs = njobs;
     c->n_items = total;
+    HIPCHK(c, hipEventRecord(c->geom, c->stream));
     return GEXS_OK;
 }
 
@@ -455,65 +477,77 @@ int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
     return GEXS_OK;
 }
 
-// The FP32C profile pass for njobs (T,P) over the lines and window of
-// c->pos_key; outs[j] is the grid of job j (points indexed from 0).  Device
-// work: one table upload, one counter fill, k_line_max, k_line_rec,
-// k_segments, the profile kernel.
-int run_fp32c(gexs_ctx* c, const std::vector<TPParams>& tps, const std::vector<double*>& outs,
-              bool timed) {
-    const int njobs = (int)tps.size();
+// The FP32C pass of one execute over the lines and window of c->pos_key into
+// out.  The per-line pre-pass (table upload, counter fill, k_line_max,
+// k_line_rec, k_segments) runs on c->pre into one of two buffer sets, after
+// the profile kernel that last used that set; the profile kernel runs on
+// c->stream after it.  The pre-pass of the next execute thus overlaps this
+// pass's profile kernel; every pass computes exactly what it would alone.
+int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     const int n = (int)c->pos_key.n;
     const int64_t l0 = c->pos_key.l0;
-    hipStream_t s = c->stream;
     int rc;
-    PassTables t;
-    if ((rc = pass_tables(c, njobs, &t)) != GEXS_OK) return rc;
-    HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRec)));
-    HIPCHK(c, c->recx.ensure((size_t)std::max(n, 1) * njobs * sizeof(VRecX)));
-    HIPCHK(c, c->counter.ensure(256));
     const int per = tiles_per_job(c);
+    JobDesc d{};
+    d.l0 = l0;
+    d.pos_off = 0;
+    d.rec_off = 0;
+    d.recx_off = 0;
+    d.n = n;
+    d.items_off = 0;
+    d.ntiles = per;
+    d.q_slot = -1;
+    if ((rc = ensure_items(c, 1, &tp, &d)) != GEXS_OK) return rc;
+    gexs_ctx::PassBuf& B = c->pb[c->pbuf];
+    c->pbuf ^= 1;
+    PassTables t;
+    if ((rc = pass_tables_in(c, B.tab, B.small, 1, &t)) != GEXS_OK) return rc;
+    HIPCHK(c, B.rec.ensure((size_t)std::max(n, 1) * sizeof(VRec)));
+    HIPCHK(c, B.recx.ensure((size_t)std::max(n, 1) * sizeof(VRecX)));
+    HIPCHK(c, B.items.ensure((size_t)std::max(c->n_items, 1) * sizeof(WorkItem)));
+    HIPCHK(c, c->counter.ensure(256));
     std::vector<char> h(t.bytes, 0);
-    TPParams* htp = (TPParams*)h.data();
-    JobDesc* hjd = (JobDesc*)(h.data() + t.jd_off);
-    Job* hjobs = (Job*)(h.data() + t.jobs_off);
-    for (int j = 0; j < njobs; ++j) {
-        htp[j] = tps[(size_t)j];
-        JobDesc& d = hjd[j];
-        d.l0 = l0;
-        d.pos_off = 0;
-        d.rec_off = (int64_t)j * n;
-        d.recx_off = (int64_t)j * n;
-        d.n = n;
-        d.items_off = j * per;
-        d.ntiles = per;
-        d.q_slot = -1;
-        Job& jb = hjobs[j];
-        jb.pos = c->pos.as<LinePos>();
-        jb.rec = c->rec.as<VRec>() + (size_t)j * n;
-        jb.recx = c->recx.as<VRecX>() + (size_t)j * n;
-        jb.out = outs[(size_t)j];
-        jb.n = n;
-        jb.profile = c->profile;
-        jb.scale_slot = j;
-        jb.accumulate = 1;
-    }
-    HIPCHK(c, c->staging.upload(c->tab.p, h.data(), t.bytes, s));
-    HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)c->small.p, t.zero_u64, 0ull));
+    *(TPParams*)h.data() = tp;
+    *(JobDesc*)(h.data() + t.jd_off) = d;
+    Job& jb = *(Job*)(h.data() + t.jobs_off);
+    jb.pos = c->pos.as<LinePos>();
+    jb.rec = B.rec.as<VRec>();
+    jb.recx = B.recx.as<VRecX>();
+    jb.out = out;
+    jb.n = n;
+    jb.profile = c->profile;
+    jb.scale_slot = 0;
+    jb.accumulate = 1;
+    hipStream_t P = c->pre;
+    HIPCHK(c, hipStreamWaitEvent(P, c->geom, 0));  // positions and tiles
+    if (B.used) HIPCHK(c, hipStreamWaitEvent(P, B.prof_done, 0));  // buffer set free
+    HIPCHK(c, hipEventRecord(c->ev_start, P));
+    HIPCHK(c, c->staging.upload(B.tab.p, h.data(), t.bytes, P));
+    HIPCHK(c, gxk::fill_u64(P, (unsigned long long*)B.small.p, t.zero_u64, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale));
-    HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            gam, nex, c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale,
-                            c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval));
-    if ((rc = ensure_items(c, njobs)) != GEXS_OK) return rc;
-    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, t.jd, c->pos.as<LinePos>(), t.kmax,
-                            c->profile));
+    HIPCHK(c, gxk::line_max(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale));
+    HIPCHK(c, gxk::line_rec(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
+                            gam, nex, c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale, B.rec.as<VRec>(),
+                            B.recx.as<VRecX>(), t.kmax, t.neval));
+    // Voigt items carry this pass's near-centre bounds (a copy per buffer
+    // set); the other profiles read the shared items unchanged
+    const bool voigt = c->profile == GEXS_VOIGT;
+    if (voigt)
+        HIPCHK(c, gxk::segments(P, c->items.as<WorkItem>(), B.items.as<WorkItem>(), c->n_items, t.jd,
+                                c->pos.as<LinePos>(), t.kmax, c->profile));
+    HIPCHK(c, hipEventRecord(B.pre_done, P));
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipStreamWaitEvent(s, B.pre_done, 0));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
-                                 t.jobs, t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split,
+                                 voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>(), c->n_items, t.jobs,
+                                 t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
                                  launch_blocks(c)));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
+    HIPCHK(c, hipEventRecord(B.prof_done, s));
+    B.used = true;
     c->last.n_tiles = c->n_items;
     c->last_neval = t.neval;
     return GEXS_OK;
@@ -566,9 +600,16 @@ int gexs_create(int device, int profile, gexs_ctx** out) {
     if (!c) return GEXS_ENOMEM;
     c->device = device;
     c->profile = profile;
-    if (set_device(c) != GEXS_OK || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        next_events(c) != GEXS_OK) {
-        delete c;
+    bool ok = set_device(c) == GEXS_OK &&
+              hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->pre, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->geom, hipEventDisableTiming) == hipSuccess &&
+              next_events(c) == GEXS_OK;
+    for (auto& b : c->pb)
+        ok = ok && hipEventCreateWithFlags(&b.pre_done, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&b.prof_done, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        gexs_destroy(c);
         return GEXS_EHIP;
     }
     c->tcount = 0;
@@ -607,6 +648,13 @@ int gexs_destroy(gexs_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->h_tab) (void)hipHostFree(c->h_tab);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+    if (c->pre) (void)hipStreamSynchronize(c->pre);
+    for (auto& b : c->pb) {
+        if (b.pre_done) (void)hipEventDestroy(b.pre_done);
+        if (b.prof_done) (void)hipEventDestroy(b.prof_done);
+    }
+    if (c->geom) (void)hipEventDestroy(c->geom);
+    if (c->pre) (void)hipStreamDestroy(c->pre);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GEXS_OK;
@@ -737,11 +785,12 @@ int gexs_execute(gexs_ctx* c, int npoints, int64_t nlines, int start_idx, double
     TPParams tp = make_tp(c, T, P, Q, c->start, dfreq, win_lo, win_hi);
     if ((rc = ensure_pos(c, tp, 0, c->n)) != GEXS_OK) return rc;
     if ((rc = next_events(c)) != GEXS_OK) return rc;
-    HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
-    if (c->k.precision == GEXS_FP64)
+    if (c->k.precision == GEXS_FP64) {
+        HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
         rc = run_fp64(c, tp, out, true);
-    else
-        rc = run_fp32c(c, std::vector<TPParams>{tp}, std::vector<double*>{out}, true);
+    } else {
+        rc = run_fp32c(c, tp, out, true);  // records ev_start on the pre-pass stream
+    }
     c->last_neval_pending = true;
     return rc;
 }
@@ -976,7 +1025,8 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, const PassTables& t) {
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale,
                             c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval));
-    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->n_items, t.jd, c->pos.as<LinePos>(), t.kmax,
+    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->items.as<WorkItem>(), c->n_items, t.jd,
+                            c->pos.as<LinePos>(), t.kmax,
                             c->profile));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
                                  t.jobs, t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
@@ -1096,7 +1146,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     }
     // the tiles kernel reads window and job tables from the device copy
     HIPCHK(c, hipMemcpyAsync(c->tab.p, c->h_tab, t.bytes, hipMemcpyHostToDevice, c->stream));
-    if ((rc = ensure_items(c, ntp)) != GEXS_OK) return rc;
+    if ((rc = ensure_items(c, ntp, htp, hjd)) != GEXS_OK) return rc;
     if ((rc = next_events(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
     if (c->use_graph) {

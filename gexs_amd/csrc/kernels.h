@@ -19,9 +19,10 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     const unsigned long long* scale_bits, VRec* rec, VRecX* recx, int* kmax,
                     unsigned long long* neval);
-// segment bounds of every work item (after line_rec has produced kmax per job)
-hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const JobDesc* jd,
-                    const LinePos* pos, const int* kmax, int profile);
+// near-centre bounds of every work item (after line_rec has produced kmax per
+// job): out[i] = in[i] with s2, s3 set (out may equal in); Voigt only
+hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,
+                    const JobDesc* jd, const LinePos* pos, const int* kmax, int profile);
 // jobs job0 .. job0 + njobs - 1 of jd / tp, all with tiles of span points
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,

@@ -335,13 +335,14 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
 // the whole tile (set by k_tiles), those whose centre c lies within kmax of
 // the tile, [s2, s3), may have points outside Humlicek region I.  kmax = max
 // k15 over the job's lines (T,P-dependent, from k_line_rec).
-__global__ __launch_bounds__(256) void k_segments(WorkItem* __restrict__ items, int nitems,
+// out may equal in (in place) or be a per-pass copy.
+__global__ __launch_bounds__(256) void k_segments(const WorkItem* in, WorkItem* out, int nitems,
                                                   const JobDesc* __restrict__ jd,
                                                   const LinePos* __restrict__ pos,
                                                   const int* __restrict__ kmax) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nitems) return;
-    const WorkItem it = items[t];
+    WorkItem it = in[t];
     int s2 = it.s4, s3 = it.s4;
     if (it.s1 < it.s4) {
         const LinePos* jp = pos + jd[it.job].pos_off;
@@ -360,8 +361,9 @@ __global__ __launch_bounds__(256) void k_segments(WorkItem* __restrict__ items, 
         }
         s3 = a;
     }
-    items[t].s2 = s2;
-    items[t].s3 = s3;
+    it.s2 = s2;
+    it.s3 = s3;
+    out[t] = it;
 }
 
 // tp[k].Q = Q[q_slot] for jobs whose Q comes from a device partition pass
@@ -1061,10 +1063,10 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
     return hipGetLastError();
 }
 
-hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const JobDesc* jd,
-                    const LinePos* pos, const int* kmax, int profile) {
+hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,
+                    const JobDesc* jd, const LinePos* pos, const int* kmax, int profile) {
     if (nitems <= 0 || profile != GX_PROF_VOIGT) return hipSuccess;  // only Voigt reads them
-    hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 256)), dim3(256), 0, s, items, nitems,
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 256)), dim3(256), 0, s, in, out, nitems,
                        jd, pos, kmax);
     return hipGetLastError();
 }

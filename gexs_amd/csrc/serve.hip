@@ -385,7 +385,8 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
                 goto fail;
             k0 = k1;
         }
-        if ((e = gxk::segments(st, d_items, items_total, d_jd, (const LinePos*)base, d_kmax, ph_.profile)) !=
+        if ((e = gxk::segments(st, d_items, d_items, items_total, d_jd, (const LinePos*)base, d_kmax,
+                               ph_.profile)) !=
             hipSuccess)
             goto fail;
         for (int k0 = 0; k0 < nreq;) {  // one profile launch per group, slots in dispatch order
