@@ -109,3 +109,50 @@ def test_truncated_and_corrupt_bzip2_fail_loudly(files):
     r = dump(d, ["bad.trans.bz2"], expect_ok=False)
     assert r.returncode == 1
     assert "bzip2" in r.stderr or "unreadable transition record" in r.stderr
+
+
+def test_hitran_160_char_records(tmp_path):
+    """HITRAN .par records (HITRANStateReader.cpp:137-250): fields by the
+    fixed widths 2,1,12,10,10,5,5,10,4,8,... (:3-18), gamma = g_air*mix +
+    g_self*(1-mix) (P is applied later, once), and a record with A = 0 gets
+    its A from the 296 K intensity with g' = 1 (:222-233).  The A-from-S
+    records are checked by the inverse relation: the line strength that A
+    gives at 296 K with Q(296) equals the record's S."""
+    from tests.test_gpu_driver import par_record
+    rng = np.random.default_rng(5)
+    n = 300
+    nu = np.sort(rng.uniform(500.0, 4000.0, n))
+    S = 10 ** rng.uniform(-25, -19, n)
+    A = 10 ** rng.uniform(-3, 1, n)
+    A[::3] = 0.0
+    rows = zip(nu, S, A, rng.uniform(0.05, 0.09, n), rng.uniform(0.2, 0.4, n),
+               rng.uniform(0.0, 3000.0, n), rng.uniform(0.5, 0.8, n),
+               rng.integers(1, 30, n).astype(float))
+    recs = [par_record(*r) for r in rows]
+    # a blank line between records is not a record
+    (tmp_path / "x.par").write_text("\n".join(recs[:100]) + "\n\n" + "\n".join(recs[100:]) + "\n")
+    mix, q296 = 0.7, 174.5
+    cfg = "\n".join(["HITRAN", "TRANS-FILES", "x.par", "END", "PROFILE VOIGT", "N-POINTS 1000",
+                     "NU-RANGE 400.0 4100.0", f"HITRAN-MIXTURE-AIR {mix}", f"PARTITION {q296}",
+                     ""])
+    r = subprocess.run([CLI, "--lines"], input=cfg, capture_output=True, text=True,
+                       cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in ln.split()] for ln in r.stdout.splitlines()])
+    assert got.shape == (n, 6)
+    f = lambda a, b: np.array([float(rec[a:b]) for rec in recs])
+    nu_r, S_r, A_r, ga, gs, E_r, n_r, g_r = (f(3, 15), f(15, 25), f(25, 35), f(35, 40),
+                                             f(40, 45), f(45, 55), f(55, 59), f(146, 153))
+    assert np.array_equal(got[:, 0], nu_r)
+    assert np.array_equal(got[:, 2], E_r)
+    assert np.array_equal(got[:, 4], ga * mix + gs * (1.0 - mix))
+    assert np.array_equal(got[:, 5], n_r)
+    has_a = A_r != 0.0
+    assert np.array_equal(got[has_a, 3], A_r[has_a])
+    assert np.array_equal(got[has_a, 1], g_r[has_a])
+    assert np.all(got[~has_a, 1] == 1.0)
+    beta = 6.6260693e-27 * 2.99792458e10 / (1.380658e-16 * 296.0)
+    Aa, nn, EE = got[~has_a, 3], nu_r[~has_a], E_r[~has_a]
+    S_back = Aa * np.exp(-beta * EE) * (1 - np.exp(-beta * nn)) / (
+        8 * np.pi * 2.99792458e10 * nn * nn * q296)
+    assert np.max(np.abs(S_back / S_r[~has_a] - 1)) < 1e-13
