@@ -444,90 +444,78 @@ struct TileAcc {
     }
 };
 
+// lines [a, b) whose window covers the tile and whose every point is in
+// Humlicek region I: no per-line decisions.  Per batch of 64 lines, lane k
+// loads line a + k's (cf, A, B) with one vector load and stages
+// X0q = (cf - (w0 + 64 q)) A + B, the line's x at lane 0 of slice q, for every
+// slice in the wave's LDS block xs (64 x PPL floats); the batch's lines then
+// read their X0q with one broadcast LDS read each and take A, Kl, E2, H by
+// scalar loads, four lines at a time, the next four loading while these are
+// evaluated.  Per point x = X0q - lane A: one FMA, no per-line vector work.
+// |X0q - x| <= 63 A, so x keeps FP32 accuracy except where |x| << 63 A; in
+// region I there |x| + y >= 15 makes y >= 15 - |x|, and K's relative slope in
+// x, below 2|x| / (x^2 + y^2), keeps the error of K under 1e-6.
 template <int PPL>
-__device__ __forceinline__ void region1_line(TileAcc<PPL>& t, const VRec& r) {
+__device__ __forceinline__ void staged_group(TileAcc<PPL>& t, const float* xs, const VRec& r,
+                                             float nl) {
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
-        t.part[q] += region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
+        t.part[q] += region1(fmaf(nl, r.A, xs[q]), r.Kl, r.E2, r.H);
 }
 
-// lines [a, b) whose window covers the tile and whose every point is in
-// Humlicek region I: no per-line decisions.  Records come in groups of four
-// (128 B of scalar loads), the next group loading while this one is evaluated.
 template <int PPL>
-__device__ __forceinline__ void voigt_fast(TileAcc<PPL>& t, ConstRec rec, int a, int b) {
-    int i = a;
-    if (b - a >= 8) {
-        VRec c0 = rec[i], c1 = rec[i + 1], c2 = rec[i + 2], c3 = rec[i + 3];
-        for (; i + 8 <= b; i += 4) {
-            const VRec n0 = rec[i + 4], n1 = rec[i + 5], n2 = rec[i + 6], n3 = rec[i + 7];
-            // keep the loads of the next group ahead of this group's arithmetic
-            __builtin_amdgcn_sched_barrier(0);
-            region1_line(t, c0);
-            region1_line(t, c1);
-            region1_line(t, c2);
-            region1_line(t, c3);
-            t.tick(4);
-            c0 = n0;
-            c1 = n1;
-            c2 = n2;
-            c3 = n3;
+__device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, ConstRec rec, int a,
+                                             int b, int w0, int lane, float* xs) {
+    if (a >= b) return;
+    const float nl = -(float)lane;
+    float cfn, An, Bn;
+    {
+        const int li = min(a + lane, b - 1);
+        cfn = grec[li].cf;
+        An = grec[li].A;
+        Bn = grec[li].B;
+    }
+    for (int base = a; base < b; base += GX_WAVE) {
+        const float cf = cfn, A = An, B = Bn;
+        {
+            // unconditional (the last batch reloads line b - 1): every
+            // batch has the same loads in flight
+            const int li = min(base + GX_WAVE + lane, b - 1);
+            cfn = grec[li].cf;
+            An = grec[li].A;
+            Bn = grec[li].B;
         }
-        region1_line(t, c0);
-        region1_line(t, c1);
-        region1_line(t, c2);
-        region1_line(t, c3);
-        t.tick(4);
-        i += 4;
-    }
-    for (; i < b; ++i) {
-        region1_line(t, rec[i]);
-        t.tick(1);
-    }
-}
-
-// lines [a, b) of a region-I segment whose centre is a tile width or more
-// away from the tile: per line X0 = fma(c - w0, A, B) (c - w0 exact), per
-// point x = X0 - l*A with l = j - w0 (nl = -l).  For these lines |X0| <= 2|x|
-// at every point of the tile, so x keeps FP32 accuracy (within 3 ulp); it
-// saves one instruction per point against voigt_fast.
-template <int PPL>
-__device__ __forceinline__ void far_line(TileAcc<PPL>& t, const VRec& r, float w0f,
-                                         const float* nl) {
-    const float X0 = fmaf(r.cf - w0f, r.A, r.B);
 #pragma unroll
-    for (int q = 0; q < PPL; ++q) t.part[q] += region1(fmaf(nl[q], r.A, X0), r.Kl, r.E2, r.H);
-}
-
-template <int PPL>
-__device__ __forceinline__ void voigt_far(TileAcc<PPL>& t, ConstRec rec, int a, int b, float w0f,
-                                          const float* nl) {
-    int i = a;
-    if (b - a >= 8) {
-        VRec c0 = rec[i], c1 = rec[i + 1], c2 = rec[i + 2], c3 = rec[i + 3];
-        for (; i + 8 <= b; i += 4) {
-            const VRec n0 = rec[i + 4], n1 = rec[i + 5], n2 = rec[i + 6], n3 = rec[i + 7];
-            __builtin_amdgcn_sched_barrier(0);
-            far_line(t, c0, w0f, nl);
-            far_line(t, c1, w0f, nl);
-            far_line(t, c2, w0f, nl);
-            far_line(t, c3, w0f, nl);
+        for (int q = 0; q < PPL; ++q) xs[lane * PPL + q] = fmaf(cf - (float)(w0 + GX_WAVE * q), A, B);
+        const int nb = min(GX_WAVE, b - base);
+        ConstRec rb = rec + base;
+        int k = 0;
+        if (nb >= 8) {
+            VRec c0 = rb[0], c1 = rb[1], c2 = rb[2], c3 = rb[3];
+            for (; k + 8 <= nb; k += 4) {
+                const VRec n0 = rb[k + 4], n1 = rb[k + 5], n2 = rb[k + 6], n3 = rb[k + 7];
+                __builtin_amdgcn_sched_barrier(0);
+                staged_group(t, xs + (k + 0) * PPL, c0, nl);
+                staged_group(t, xs + (k + 1) * PPL, c1, nl);
+                staged_group(t, xs + (k + 2) * PPL, c2, nl);
+                staged_group(t, xs + (k + 3) * PPL, c3, nl);
+                t.tick(4);
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+            }
+            staged_group(t, xs + (k + 0) * PPL, c0, nl);
+            staged_group(t, xs + (k + 1) * PPL, c1, nl);
+            staged_group(t, xs + (k + 2) * PPL, c2, nl);
+            staged_group(t, xs + (k + 3) * PPL, c3, nl);
             t.tick(4);
-            c0 = n0;
-            c1 = n1;
-            c2 = n2;
-            c3 = n3;
+            k += 4;
         }
-        far_line(t, c0, w0f, nl);
-        far_line(t, c1, w0f, nl);
-        far_line(t, c2, w0f, nl);
-        far_line(t, c3, w0f, nl);
-        t.tick(4);
-        i += 4;
-    }
-    for (; i < b; ++i) {
-        far_line(t, rec[i], w0f, nl);
-        t.tick(1);
+        for (; k < nb; ++k) {
+            staged_group(t, xs + k * PPL, rb[k], nl);
+            t.tick(1);
+        }
     }
 }
 
@@ -708,7 +696,7 @@ __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRe
 // the 64*PPL points of the tile; lines stay in nu order across the segments
 template <int PROF, int PPL>
 __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, const Job& jb,
-                                          int lane, int ca, int cb) {
+                                          int lane, int ca, int cb, float* xs) {
     const ConstPos pos = (ConstPos)jb.pos;
     const ConstRec rec = (ConstRec)jb.rec;
     const ConstRecX recx = (ConstRecX)jb.recx;
@@ -745,18 +733,11 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             // [lo, s1): ib <= w0 < ie < w1; [s1, s4): ib <= w0, ie >= w1;
             // [s4, hi): w0 < ib < w1 <= ie.  In [s1, s2) the lines before f1
             // and in [s3, s4) those from f2 on are far from the tile.
-            const int fl = min(max(min(it.f1, it.s2), ca), cb);
-            const int fr = min(max(max(it.f2, it.s3), ca), cb);
-            float nl[PPL];
-#pragma unroll
-            for (int q = 0; q < PPL; ++q) nl[q] = -(float)(lane + GX_WAVE * q);
-            const float w0f = (float)w0;
+            const GlobRec grec = (GlobRec)jb.rec;
             lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
-            voigt_far<PPL>(t, rec, s1, fl, w0f, nl);
-            voigt_fast<PPL>(t, rec, fl, s2);
+            voigt_staged<PPL>(t, grec, rec, s1, s2, w0, lane, xs);
             lines_checked<PROF, PPL, kMaskNone>(t, pos, rec, recx, s2, s3, w0, w1);
-            voigt_fast<PPL>(t, rec, s3, fr);
-            voigt_far<PPL>(t, rec, fr, s4, w0f, nl);
+            voigt_staged<PPL>(t, grec, rec, s3, s4, w0, lane, xs);
             lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
         } else {
             lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
@@ -777,9 +758,9 @@ __device__ __forceinline__ void store_point(const Job& jb, int j, double v) {
 // one wave evaluates the whole item
 template <int PROF, int PPL>
 __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
-                                         int lane) {
+                                         int lane, float* xs) {
     TileAcc<PPL> t;
-    sum_lines<PROF, PPL>(t, it, jb, lane, it.lo, it.hi);
+    sum_lines<PROF, PPL>(t, it, jb, lane, it.lo, it.hi, xs);
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
         const double v = ((double)t.sum[q] - (double)t.cmp[q]) * inv_scale;
@@ -798,12 +779,12 @@ constexpr int block_threads() { return SPLIT > 4 ? SPLIT * GX_WAVE : GX_WG; }
 template <int PROF, int PPL, int SPLIT>
 __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb,
                                                double inv_scale, int lane, int wv,
-                                               double* red) {
+                                               double* red, float* xs) {
     TileAcc<PPL> t;
     const int n = max(it.hi - it.lo, 0);
     const int ca = it.lo + (int)(((long long)n * wv) / SPLIT);
     const int cb = it.lo + (int)(((long long)n * (wv + 1)) / SPLIT);
-    sum_lines<PROF, PPL>(t, it, jb, lane, ca, cb);
+    sum_lines<PROF, PPL>(t, it, jb, lane, ca, cb, xs);
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
         red[(size_t)wv * GX_WAVE * PPL + GX_WAVE * q + lane] = (double)t.sum[q] - (double)t.cmp[q];
@@ -842,9 +823,12 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) void k_profile_tiles(const 
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
     __shared__ double red[SPLIT > 1 ? SPLIT * GX_WAVE * PPL : 1];
+    // per wave: the staged X0 of one batch of 64 region-I lines
+    __shared__ float xs_all[PPL > 1 ? block_threads<SPLIT>() * PPL : 1];
     __shared__ int grab;
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* xs = xs_all + (PPL > 1 ? wv * GX_WAVE * PPL : 0);
     const int lb = mode == 3 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (mode == 0 || mode == 3) {
         if (SPLIT == 1) {
@@ -852,13 +836,13 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) void k_profile_tiles(const 
             if (idx >= nitems) return;
             const WorkItem it = items[idx];
             const Job jb = jobs[it.job];
-            run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+            run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane, xs);
         } else {
             if (lb >= nitems) return;
             const WorkItem it = items[lb];
             const Job jb = jobs[it.job];
             run_item_split<PROF, PPL, SPLIT>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane,
-                                             wv, red);
+                                             wv, red, xs);
         }
         return;
     }
@@ -884,10 +868,10 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) void k_profile_tiles(const 
             const WorkItem it = items[idx];
             const Job jb = jobs[it.job];
             if (SPLIT == 1)
-                run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane);
+                run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane, xs);
             else
                 run_item_split<PROF, PPL, SPLIT>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]),
-                                                 lane, wv, red);
+                                                 lane, wv, red, xs);
         }
     }
 }
