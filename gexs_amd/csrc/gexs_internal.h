@@ -68,15 +68,12 @@ struct __attribute__((aligned(8))) VRecX {
 //   tile (no checks), [s2, s3) line centres near the tile (checked),
 //   [s3, s4) region I, [s4, hi) window starts inside the tile.
 // s1 = s4 = hi when no line covers the whole tile (every line checked).
-// Among [s1, s4), the lines before f1 have their centre a tile width or more
-// below the tile and those from f2 on a tile width or more above it ("far").
 struct __attribute__((aligned(16))) WorkItem {
     int32_t lo, hi;
     int32_t w0, w1;
     int32_t job;
     int32_t s1, s2, s3, s4;
-    int32_t f1, f2;
-    int32_t pad;
+    int32_t pad[3];
 };
 
 // one cross-section

@@ -306,22 +306,7 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.s1 = lf < rf ? lf : hi;
     it.s4 = lf < rf ? rf : hi;
     it.s2 = it.s3 = it.s4;  // [s1, s4) all region I until k_segments runs
-    // far lines: centre c < w0 - span (before f1) or c >= w1 + span (from f2);
-    // c is non-decreasing in nu
-    a = it.s1;
-    b = it.s4;
-    while (a < b) {
-        const int m = (a + b) >> 1;
-        if ((long long)jp[m].c >= (long long)w0 - span) b = m; else a = m + 1;
-    }
-    it.f1 = a;
-    b = it.s4;
-    while (a < b) {
-        const int m = (a + b) >> 1;
-        if ((long long)jp[m].c >= (long long)w1 + span) b = m; else a = m + 1;
-    }
-    it.f2 = a;
-    it.pad = 0;
+    it.pad[0] = it.pad[1] = it.pad[2] = 0;
     const int slot = d.items_off + t;
     items[slot] = it;
     if (cost_key) {
@@ -444,81 +429,6 @@ struct TileAcc {
     }
 };
 
-// lines [a, b) whose window covers the tile and whose every point is in
-// Humlicek region I: no per-line decisions.  Per batch of 64 lines, lane k
-// loads line a + k's (cf, A, B) with one vector load and stages
-// X0q = (cf - (w0 + 64 q)) A + B, the line's x at lane 0 of slice q, for every
-// slice in the wave's LDS block xs (64 x PPL floats); the batch's lines then
-// read their X0q with one broadcast LDS read each and take A, Kl, E2, H by
-// scalar loads, four lines at a time, the next four loading while these are
-// evaluated.  Per point x = X0q - lane A: one FMA, no per-line vector work.
-// |X0q - x| <= 63 A, so x keeps FP32 accuracy except where |x| << 63 A; in
-// region I there |x| + y >= 15 makes y >= 15 - |x|, and K's relative slope in
-// x, below 2|x| / (x^2 + y^2), keeps the error of K under 1e-6.
-template <int PPL>
-__device__ __forceinline__ void staged_group(TileAcc<PPL>& t, const float* xs, const VRec& r,
-                                             float nl) {
-#pragma unroll
-    for (int q = 0; q < PPL; ++q)
-        t.part[q] += region1(fmaf(nl, r.A, xs[q]), r.Kl, r.E2, r.H);
-}
-
-template <int PPL>
-__device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, ConstRec rec, int a,
-                                             int b, int w0, int lane, float* xs) {
-    if (a >= b) return;
-    const float nl = -(float)lane;
-    float cfn, An, Bn;
-    {
-        const int li = min(a + lane, b - 1);
-        cfn = grec[li].cf;
-        An = grec[li].A;
-        Bn = grec[li].B;
-    }
-    for (int base = a; base < b; base += GX_WAVE) {
-        const float cf = cfn, A = An, B = Bn;
-        {
-            // unconditional (the last batch reloads line b - 1): every
-            // batch has the same loads in flight
-            const int li = min(base + GX_WAVE + lane, b - 1);
-            cfn = grec[li].cf;
-            An = grec[li].A;
-            Bn = grec[li].B;
-        }
-#pragma unroll
-        for (int q = 0; q < PPL; ++q) xs[lane * PPL + q] = fmaf(cf - (float)(w0 + GX_WAVE * q), A, B);
-        const int nb = min(GX_WAVE, b - base);
-        ConstRec rb = rec + base;
-        int k = 0;
-        if (nb >= 8) {
-            VRec c0 = rb[0], c1 = rb[1], c2 = rb[2], c3 = rb[3];
-            for (; k + 8 <= nb; k += 4) {
-                const VRec n0 = rb[k + 4], n1 = rb[k + 5], n2 = rb[k + 6], n3 = rb[k + 7];
-                __builtin_amdgcn_sched_barrier(0);
-                staged_group(t, xs + (k + 0) * PPL, c0, nl);
-                staged_group(t, xs + (k + 1) * PPL, c1, nl);
-                staged_group(t, xs + (k + 2) * PPL, c2, nl);
-                staged_group(t, xs + (k + 3) * PPL, c3, nl);
-                t.tick(4);
-                c0 = n0;
-                c1 = n1;
-                c2 = n2;
-                c3 = n3;
-            }
-            staged_group(t, xs + (k + 0) * PPL, c0, nl);
-            staged_group(t, xs + (k + 1) * PPL, c1, nl);
-            staged_group(t, xs + (k + 2) * PPL, c2, nl);
-            staged_group(t, xs + (k + 3) * PPL, c3, nl);
-            t.tick(4);
-            k += 4;
-        }
-        for (; k < nb; ++k) {
-            staged_group(t, xs + k * PPL, rb[k], nl);
-            t.tick(1);
-        }
-    }
-}
-
 // window masks: which window edges may fall inside the tile
 enum { kMaskNone = 0, kMaskHi = 1, kMaskLo = 2, kMaskBoth = 3 };
 
@@ -540,6 +450,110 @@ __device__ __forceinline__ int slice_in_window(int j0, const LinePos& lp) {
     if (none) return 0;
     const bool all = (MASK == kMaskHi || j0 >= lp.ib) && (MASK == kMaskLo || j0 + GX_WAVE <= lp.ie);
     return all ? 2 : 1;
+}
+
+// lines [a, b) whose every point in the tile is in Humlicek region I.  Per
+// batch of 64 lines, lane k loads line a + k's (cf, A, B) with one vector
+// load and stages X0q = (cf - (w0 + 64 q)) A + B, the line's x at lane 0 of
+// slice q, for every slice in the wave's LDS block xs (64 x PPL floats); the
+// batch's lines then read their X0q with one broadcast LDS read each and take
+// A, Kl, E2, H by scalar loads.  Per point x = X0q - lane A: one FMA, no
+// per-line vector work.  |X0q - x| <= 63 A, so x keeps FP32 accuracy except
+// where |x| << 63 A; in region I there |x| + y >= 15 makes y >= 15 - |x|,
+// and K's relative slope in x, below 2|x| / (x^2 + y^2), keeps the error of
+// K under 1e-6.
+// MASK kMaskNone: the windows cover the tile; four lines at a time, the next
+// four loading while these are evaluated; one Kahan fold per batch.
+// kMaskHi / kMaskLo: the windows end / start inside the tile; slices outside
+// the window are skipped, the one the edge cuts is masked per lane.
+template <int PPL>
+__device__ __forceinline__ void staged_group(TileAcc<PPL>& t, const float* xs, const VRec& r,
+                                             float nl) {
+#pragma unroll
+    for (int q = 0; q < PPL; ++q)
+        t.part[q] += region1(fmaf(nl, r.A, xs[q]), r.Kl, r.E2, r.H);
+}
+
+template <int PPL, int MASK>
+__device__ __forceinline__ void staged_masked(TileAcc<PPL>& t, const float* xs, const VRec& r,
+                                              const LinePos& lp, float nl, int w0) {
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+        if (st == 0) continue;
+        const float v = region1(fmaf(nl, r.A, xs[q]), r.Kl, r.E2, r.H);
+        t.part[q] += (st == 2 || in_window<MASK>(t.jj[q], lp)) ? v : 0.0f;
+    }
+    t.tick(1);
+}
+
+template <int PPL, int MASK>
+__device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, ConstRec rec,
+                                             ConstPos pos, int a, int b, int w0, int lane,
+                                             float* xs) {
+    if (a >= b) return;
+    if (MASK == kMaskNone && t.n > 0) t.fold();
+    const float nl = -(float)lane;
+    float cfn, An, Bn;
+    {
+        const int li = min(a + lane, b - 1);
+        cfn = grec[li].cf;
+        An = grec[li].A;
+        Bn = grec[li].B;
+    }
+    for (int base = a; base < b; base += GX_WAVE) {
+        const float cf = cfn, A = An, B = Bn;
+        {
+            // unconditional (the last batch reloads line b - 1): every
+            // batch has the same loads in flight
+            const int li = min(base + GX_WAVE + lane, b - 1);
+            cfn = grec[li].cf;
+            An = grec[li].A;
+            Bn = grec[li].B;
+        }
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) xs[lane * PPL + q] = fmaf(cf - (float)(w0 + GX_WAVE * q), A, B);
+        const int nb = min(GX_WAVE, b - base);
+        const ConstRec rb = rec + base;
+        if (MASK != kMaskNone) {
+            const ConstPos pb = pos + base;
+            LinePos pn = pb[0];
+            VRec rn = rb[0];
+            for (int k = 0; k < nb; ++k) {
+                const LinePos lp = pn;
+                const VRec r = rn;
+                if (k + 1 < nb) {
+                    pn = pb[k + 1];
+                    rn = rb[k + 1];
+                }
+                staged_masked<PPL, MASK>(t, xs + k * PPL, r, lp, nl, w0);
+            }
+            continue;
+        }
+        int k = 0;
+        if (nb >= 8) {
+            VRec c0 = rb[0], c1 = rb[1], c2 = rb[2], c3 = rb[3];
+            for (; k + 8 <= nb; k += 4) {
+                const VRec n0 = rb[k + 4], n1 = rb[k + 5], n2 = rb[k + 6], n3 = rb[k + 7];
+                __builtin_amdgcn_sched_barrier(0);
+                staged_group(t, xs + (k + 0) * PPL, c0, nl);
+                staged_group(t, xs + (k + 1) * PPL, c1, nl);
+                staged_group(t, xs + (k + 2) * PPL, c2, nl);
+                staged_group(t, xs + (k + 3) * PPL, c3, nl);
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+            }
+            staged_group(t, xs + (k + 0) * PPL, c0, nl);
+            staged_group(t, xs + (k + 1) * PPL, c1, nl);
+            staged_group(t, xs + (k + 2) * PPL, c2, nl);
+            staged_group(t, xs + (k + 3) * PPL, c3, nl);
+            k += 4;
+        }
+        for (; k < nb; ++k) staged_group(t, xs + k * PPL, rb[k], nl);
+        t.fold();
+    }
 }
 
 // lines with per-line decisions: window masks per lane (only the edges MASK
@@ -731,14 +745,22 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
         if (it.s1 < it.s4) {
             // [lo, s1): ib <= w0 < ie < w1; [s1, s4): ib <= w0, ie >= w1;
-            // [s4, hi): w0 < ib < w1 <= ie.  In [s1, s2) the lines before f1
-            // and in [s3, s4) those from f2 on are far from the tile.
+            // [s4, hi): w0 < ib < w1 <= ie.
+            // ν order makes c non-decreasing: when the first whole-tile line
+            // is far below the tile (s2 > s1) so are all of [lo, s1); when
+            // the last one is far above (s3 < s4) so are all of [s4, hi)
             const GlobRec grec = (GlobRec)jb.rec;
-            lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
-            voigt_staged<PPL>(t, grec, rec, s1, s2, w0, lane, xs);
+            if (it.s2 > it.s1)
+                voigt_staged<PPL, kMaskHi>(t, grec, rec, pos, ca, s1, w0, lane, xs);
+            else
+                lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
+            voigt_staged<PPL, kMaskNone>(t, grec, rec, pos, s1, s2, w0, lane, xs);
             lines_checked<PROF, PPL, kMaskNone>(t, pos, rec, recx, s2, s3, w0, w1);
-            voigt_staged<PPL>(t, grec, rec, s3, s4, w0, lane, xs);
-            lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
+            voigt_staged<PPL, kMaskNone>(t, grec, rec, pos, s3, s4, w0, lane, xs);
+            if (it.s3 < it.s4)
+                voigt_staged<PPL, kMaskLo>(t, grec, rec, pos, s4, cb, w0, lane, xs);
+            else
+                lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
         } else {
             lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
         }
@@ -816,8 +838,12 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // grid order, split into GX_XCDS cost-balanced contiguous ranges), then from
 // the other queues.  xq = GX_XCDS counters (zero at launch), xqb = the
 // GX_XCDS + 1 queue bounds.
+// amdgpu_waves_per_eu(7): at 4 points per lane the kernel takes 86 VGPRs
+// unconstrained (5 waves per SIMD); held to 72 (7 waves) it spills two values
+// outside its loops and runs 4 % faster, the extra waves hiding the scalar
+// and LDS load latency of the line loops.  Wider tiles keep their registers.
 template <int PROF, int PPL, int SPLIT>
-__global__ __launch_bounds__(block_threads<SPLIT>()) void k_profile_tiles(const WorkItem* __restrict__ items,
+__global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves_per_eu(7))) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
                                                          const unsigned long long* __restrict__ scale_bits,
                                                          unsigned int* __restrict__ xq,
