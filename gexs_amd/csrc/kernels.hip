@@ -563,7 +563,9 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
 // one line with per-line decisions: window masks per lane (only the edges
 // MASK names; slices wholly outside the window are skipped, slices wholly
 // inside are unmasked), region I inline, regions II-IV per lane
-template <int PROF, int PPL, int MASK>
+// DEFER (Voigt): points outside region I add nothing here; their W4 values
+// come from batch_w4.
+template <int PROF, int PPL, int MASK, bool DEFER = false>
 __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp, const VRec& r,
                                              const VRecX& rx, int w0, int w1) {
     if (PROF == GX_PROF_VOIGT) {
@@ -586,7 +588,7 @@ __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp,
                 if (st == 0) continue;
                 const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
                 float v = region1(x, r.Kl, r.E2, r.H);
-                if (x + rx.y < 15.0f) v = r.Kg * gx::w4_f32_inner(x, rx.y);
+                if (x + rx.y < 15.0f) v = DEFER ? 0.0f : r.Kg * gx::w4_f32_inner(x, rx.y);
                 if (st == 2)
                     t.part[q] += v;
                 else
@@ -642,6 +644,71 @@ __device__ __forceinline__ float bcast(float v, int k) {
 }
 __device__ __forceinline__ int bcast(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
 
+// W4 values of one batch of lines at one point per lane (64-point tiles),
+// compacted: a line's points outside Humlicek region I (|x| + y < 15, all
+// within k15 of its centre) are few and adjacent, so evaluating W4 line by
+// line leaves most lanes idle.  Lane k holds line k's records (rc, pc, xc);
+// its candidates are the points of the tile and of its window with
+// |c - j| < k15.  A wave scan numbers the candidates, each lane takes one
+// (binary search in the scan, held in the wave's 64-int LDS block incl),
+// repeats the region test of checked_line bit for bit and evaluates W4;
+// the values then go to the lanes owning the points line by line in batch
+// order (ds_bpermute), so each point sums them in a fixed order.
+template <int MASK>
+__device__ __forceinline__ void batch_w4(TileAcc<1>& t, const VRec& rc, const LinePos& pc,
+                                         const VRecX& xc, int nb, int w0, int w1, int lane,
+                                         int* incl) {
+    int n = 0, jl = 0;
+    if (lane < nb && xc.k15 > 0) {
+        jl = max(pc.c - xc.k15 + 1, w0);
+        int jh = min(pc.c + xc.k15, w1);
+        if (MASK == kMaskLo || MASK == kMaskBoth) jl = max(jl, pc.ib);
+        if (MASK == kMaskHi || MASK == kMaskBoth) jh = min(jh, pc.ie);
+        n = max(jh - jl, 0);
+    }
+    int sc = n;
+#pragma unroll
+    for (int d = 1; d < GX_WAVE; d <<= 1) {
+        const int u = __shfl_up(sc, d);
+        if (lane >= d) sc += u;
+    }
+    const int total = __builtin_amdgcn_readlane(sc, GX_WAVE - 1);
+    if (total == 0) return;
+    incl[lane] = sc;
+    const int ex = sc - n;  // the first candidate of this lane's line
+    for (int p0 = 0; p0 < total; p0 += GX_WAVE) {
+        const int pp = p0 + lane;
+        int k = 0;  // the line of candidate pp: smallest k with incl[k] > pp
+        if (pp < total) {
+#pragma unroll
+            for (int step = GX_WAVE / 2; step > 0; step >>= 1)
+                if (incl[k + step - 1] <= pp) k += step;
+        }
+        const float cf = __shfl(rc.cf, k), A = __shfl(rc.A, k), B = __shfl(rc.B, k);
+        const float Kg = __shfl(rc.Kg, k), y = __shfl(xc.y, k);
+        const int jlk = __shfl(jl, k), exk = __shfl(ex, k);
+        float v = 0.0f;
+        if (pp < total) {
+            const int j = jlk + (pp - exk);
+            const float x = fabsf(fmaf(cf - (float)j, A, B));
+            if (x + y < 15.0f) v = Kg * gx::w4_f32_inner(x, y);
+        }
+        const int last = min(total - p0, GX_WAVE) - 1;
+        const int k0 = __builtin_amdgcn_readlane(k, 0), k1 = __builtin_amdgcn_readlane(k, last);
+        for (int kk = k0; kk <= k1; ++kk) {
+            const int jq = __builtin_amdgcn_readlane(jl, kk);
+            const int eq = __builtin_amdgcn_readlane(ex, kk);
+            const int nq = __builtin_amdgcn_readlane(n, kk);
+            const int d = t.jj[0] - jq;    // this lane's point among line kk's candidates
+            const int L = eq + d - p0;     // the lane holding that candidate in this chunk
+            const bool mine = d >= 0 && d < nq && L >= 0 && L < GX_WAVE;
+            const float got = __shfl(v, L & (GX_WAVE - 1));
+            t.part[0] += mine ? got : 0.0f;
+            t.tick(1);
+        }
+    }
+}
+
 // One point per lane (64-point tiles): a line's evaluation is a handful of
 // VALU instructions, too few to cover a scalar load's latency when few waves
 // share a SIMD.  Here lane k loads the records of line base + k with vector
@@ -651,7 +718,8 @@ __device__ __forceinline__ int bcast(int v, int k) { return __builtin_amdgcn_rea
 template <int PROF, int MASK, bool FAST>
 __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRec rec,
                                               GlobRecX recx, int a, int b, int w0, int w1,
-                                              int lane) {
+                                              int lane, int* incl) {
+    constexpr bool kDefer = PROF == GX_PROF_VOIGT && !FAST;
     if (a >= b) return;
     LinePos pc{}, pn{};
     VRec rc, rn;
@@ -697,9 +765,10 @@ __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRe
                 VRecX rx;
                 rx.y = bcast(xc.y, k);
                 rx.k15 = bcast(xc.k15, k);
-                checked_line<PROF, 1, MASK>(t, lp, r, rx, w0, w1);
+                checked_line<PROF, 1, MASK, kDefer>(t, lp, r, rx, w0, w1);
             }
         }
+        if (kDefer) batch_w4<MASK>(t, rc, pc, xc, nb, w0, w1, lane, incl);
         rc = rn;
         pc = pn;
         xc = xn;
@@ -725,20 +794,22 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
     t.n = 0;
     const int w0 = it.w0, w1 = it.w1;
     if constexpr (PPL == 1) {
-        // records by vector loads, broadcast per line
+        // records by vector loads, broadcast per line; the wave's LDS block
+        // holds the W4 candidate scan of batch_w4
+        int* incl = reinterpret_cast<int*>(xs);
         const GlobPos gpos = (GlobPos)jb.pos;
         const GlobRec grec = (GlobRec)jb.rec;
         const GlobRecX grecx = (GlobRecX)jb.recx;
         if (PROF == GX_PROF_VOIGT && it.s1 < it.s4) {
             const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
             const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
-            lines_batched<PROF, kMaskHi, false>(t, gpos, grec, grecx, ca, s1, w0, w1, lane);
-            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s1, s2, w0, w1, lane);
-            lines_batched<PROF, kMaskNone, false>(t, gpos, grec, grecx, s2, s3, w0, w1, lane);
-            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s3, s4, w0, w1, lane);
-            lines_batched<PROF, kMaskLo, false>(t, gpos, grec, grecx, s4, cb, w0, w1, lane);
+            lines_batched<PROF, kMaskHi, false>(t, gpos, grec, grecx, ca, s1, w0, w1, lane, incl);
+            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s1, s2, w0, w1, lane, incl);
+            lines_batched<PROF, kMaskNone, false>(t, gpos, grec, grecx, s2, s3, w0, w1, lane, incl);
+            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s3, s4, w0, w1, lane, incl);
+            lines_batched<PROF, kMaskLo, false>(t, gpos, grec, grecx, s4, cb, w0, w1, lane, incl);
         } else {
-            lines_batched<PROF, kMaskBoth, false>(t, gpos, grec, grecx, ca, cb, w0, w1, lane);
+            lines_batched<PROF, kMaskBoth, false>(t, gpos, grec, grecx, ca, cb, w0, w1, lane, incl);
         }
     } else if (PROF == GX_PROF_VOIGT) {
         const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
@@ -849,12 +920,13 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
     __shared__ double red[SPLIT > 1 ? SPLIT * GX_WAVE * PPL : 1];
-    // per wave: the staged X0 of one batch of 64 region-I lines
-    __shared__ float xs_all[PPL > 1 ? block_threads<SPLIT>() * PPL : 1];
+    // per wave, 64 x PPL words: the staged X0 of one batch of 64 region-I
+    // lines, or at PPL 1 the W4 candidate scan of one batch
+    __shared__ float xs_all[block_threads<SPLIT>() * PPL];
     __shared__ int grab;
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float* xs = xs_all + (PPL > 1 ? wv * GX_WAVE * PPL : 0);
+    float* xs = xs_all + wv * GX_WAVE * PPL;
     const int lb = mode == 3 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (mode == 0 || mode == 3) {
         if (SPLIT == 1) {
