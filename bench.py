@@ -54,6 +54,7 @@ def parse():
                          "(the reference point of the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-probe", action="store_true", help="skip the on-device VALU rate probe")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-table", action="store_true",
@@ -383,6 +384,38 @@ def calib_copy(mib):
     log(f"[bench] copied {mib} MiB x 5")
 
 
+def valu_probe():
+    """On-device VALU rates (gexs_amd/libgexs_peak.so): independent FP32 FMA
+    chains (the vector rate this build can reach, packed FP32 being off),
+    v_rcp_f32 chains, and the profile kernel's region-I sequence (six VALU
+    instructions per point).  8192 workgroups of 256 threads, 1024 steps of
+    16 chains per thread, 5 timed launches each."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "gexs_amd", "libgexs_peak.so"))
+    lib.gexs_peak_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_float)]
+    blocks, iters, reps = 8192, 1024, 5
+    steps = float(blocks) * 256 * iters * 16  # chain steps (lane-instructions) per launch
+    ms = {}
+    for kind, name in ((0, "fma"), (1, "rcp"), (2, "region1")):
+        t = ctypes.c_float()
+        rc = lib.gexs_peak_run(kind, blocks, iters, reps, ctypes.byref(t))
+        if rc != 0:
+            raise RuntimeError(f"VALU probe {name} failed (hip error {rc})")
+        ms[name] = t.value
+    fma_rate = steps / (ms["fma"] * 1e-3)  # lane-FMAs/s
+    res = {"fp32_fma_tflops": 2.0 * fma_rate / 1e12,
+           "rcp_issue_cycles_per_fma": ms["rcp"] / ms["fma"],
+           "region1_evals_per_s": steps / (ms["region1"] * 1e-3),
+           "region1_issue_cycles_per_eval": ms["region1"] / ms["fma"],
+           "probe": "8192 x 256 threads x 1024 steps x 16 independent chains"}
+    log(f"[bench] VALU probe: FP32 FMA {res['fp32_fma_tflops']:.1f} TFLOP/s, rcp "
+        f"{res['rcp_issue_cycles_per_fma']:.2f} FMA slots, region-I sequence "
+        f"{res['region1_evals_per_s']:.4e} evals/s ({res['region1_issue_cycles_per_eval']:.2f} "
+        f"FMA slots per evaluation)")
+    return res
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -506,6 +539,8 @@ def main():
     served = None if a.no_serve else serve_bench(a)
     stream = stream_bench(a) if a.stream else None
 
+    probe = None if a.no_probe else valu_probe()
+
     cpu = None
     if not a.no_cpu:
         log("[bench] timing the reference CPU path on a sample")
@@ -548,7 +583,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "traffic": traffic_from_profile(a.workload),
-            "kernel": "k_profile_tiles<VOIGT>",
+            "kernel": f"k_profile_tiles<{prof.upper()}>",
+            "valu_probe": probe,
             "kernel_ms_avg": k_avg,
             "flop_per_eval": flops,
         },

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libgexs_hip.so")
+PEAK_LIB = os.path.join(HERE, "libgexs_peak.so")
 
 SOURCES = ["kernels.hip", "sort.hip", "serve.hip", "capi.hip"]
 HOST_SOURCES = ["HipManager.cpp", "host/readers.cpp", "host/cross_section.cpp"]
@@ -83,6 +84,15 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("build of gexs_xsec failed")
     os.replace(CLI + ".tmp", CLI)
+    # the VALU rate probes bench.py runs beside the profile kernel
+    obj = os.path.join(bdir, "peak.o")
+    for cmd in ([cc, *flags(), "-c", os.path.join(CSRC, "peak.hip"), "-o", obj],
+                [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", PEAK_LIB + ".tmp", obj]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("build of libgexs_peak.so failed")
+    os.replace(PEAK_LIB + ".tmp", PEAK_LIB)
     return LIB
 
 

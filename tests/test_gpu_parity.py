@@ -182,6 +182,23 @@ def test_voigt_pressure_regimes(O, T, P):
     check(out, ref, 1e-4, 1e-6, f"voigt T={T} P={P}")
 
 
+@pytest.mark.parametrize("dnu,ppl,split", [(1.0, 1, 1), (1.0, 1, 16), (0.1, 1, 4), (1.0, 4, 1)])
+@pytest.mark.parametrize("T,P", [(2500.0, 1e-4), (900.0, 50.0)])
+def test_voigt_coarse_grids(O, T, P, dnu, ppl, split):
+    # windows of +-25 points at 1 cm^-1 are shorter than a tile, so every line
+    # is masked at both edges, and a line has one or two points outside
+    # Humlicek region I: the compacted W4 path of one-point-per-lane tiles
+    L = synth.workload_list("voigt_tp_table_1e7", scale=0.002)
+    lines = L.lines()
+    Q = O.partition(L.E, L.g, T)
+    N = int(round(14500.0 / dnu))
+    out, st, _ = run_manager("voigt", lines, T, P, Q, 500.0, 15000.0, N, 25.0, mass=28.0,
+                             ppl=ppl, split=split)
+    ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 500.0, 15000.0, N, 25.0, mass=28.0)
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-4, 1e-6, f"voigt T={T} P={P} dnu={dnu} ppl={ppl} split={split}")
+
+
 def test_reference_cpu_conventions(O, c1_small):
     # the reference CPU path: no pressure in gammaL, cutoff min(25P, 100),
     # 1/w = 1/(nu*dpwcoef); and the textbook width convention
