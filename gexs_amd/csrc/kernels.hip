@@ -709,28 +709,26 @@ __device__ __forceinline__ void batch_w4(TileAcc<1>& t, const VRec& rc, const Li
     }
 }
 
-// One point per lane (64-point tiles): a line's evaluation is a handful of
-// VALU instructions, too few to cover a scalar load's latency when few waves
-// share a SIMD.  Here lane k loads the records of line base + k with vector
-// loads (64 lines per batch, the next batch in flight while this one is
-// evaluated) and each line is broadcast to the wave with readlane.
-// FAST: the lines of a region-I segment (no decisions, only VRec is read).
-template <int PROF, int MASK, bool FAST>
+// One point per lane (64-point tiles), lines with per-line decisions: a
+// line's evaluation is a handful of VALU instructions, too few to cover a
+// scalar load's latency when few waves share a SIMD.  Here lane k loads the
+// records of line base + k with vector loads (64 lines per batch, the next
+// batch in flight while this one is evaluated) and each line is broadcast to
+// the wave with readlane.  Voigt's W4 points are left to batch_w4.
+template <int PROF, int MASK>
 __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRec rec,
                                               GlobRecX recx, int a, int b, int w0, int w1,
                                               int lane, int* incl) {
-    constexpr bool kDefer = PROF == GX_PROF_VOIGT && !FAST;
+    constexpr bool kDefer = PROF == GX_PROF_VOIGT;
     if (a >= b) return;
-    LinePos pc{}, pn{};
+    LinePos pc, pn;
     VRec rc, rn;
-    VRecX xc{}, xn{};
+    VRecX xc, xn;
     {
         const int li = min(a + lane, b - 1);
         rc = rec[li];
-        if (!FAST) {
-            pc = pos[li];
-            xc = recx[li];
-        }
+        pc = pos[li];
+        xc = recx[li];
     }
     for (int base = a; base < b; base += GX_WAVE) {
         const int nb = min(GX_WAVE, b - base);
@@ -740,10 +738,8 @@ __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRe
             // broadcasts covers only the current batch
             const int li = min(base + GX_WAVE + lane, b - 1);
             rn = rec[li];
-            if (!FAST) {
-                pn = pos[li];
-                xn = recx[li];
-            }
+            pn = pos[li];
+            xn = recx[li];
         }
         for (int k = 0; k < nb; ++k) {
             VRec r;
@@ -753,20 +749,15 @@ __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRe
             r.Kl = bcast(rc.Kl, k);
             r.E2 = bcast(rc.E2, k);
             r.H = bcast(rc.H, k);
-            if (FAST) {
-                t.part[0] += region1(fmaf(r.cf - t.jf[0], r.A, r.B), r.Kl, r.E2, r.H);
-                t.tick(1);
-            } else {
-                r.Kg = bcast(rc.Kg, k);
-                LinePos lp;
-                lp.ib = bcast(pc.ib, k);
-                lp.ie = bcast(pc.ie, k);
-                lp.c = bcast(pc.c, k);
-                VRecX rx;
-                rx.y = bcast(xc.y, k);
-                rx.k15 = bcast(xc.k15, k);
-                checked_line<PROF, 1, MASK, kDefer>(t, lp, r, rx, w0, w1);
-            }
+            r.Kg = bcast(rc.Kg, k);
+            LinePos lp;
+            lp.ib = bcast(pc.ib, k);
+            lp.ie = bcast(pc.ie, k);
+            lp.c = bcast(pc.c, k);
+            VRecX rx;
+            rx.y = bcast(xc.y, k);
+            rx.k15 = bcast(xc.k15, k);
+            checked_line<PROF, 1, MASK, kDefer>(t, lp, r, rx, w0, w1);
         }
         if (kDefer) batch_w4<MASK>(t, rc, pc, xc, nb, w0, w1, lane, incl);
         rc = rn;
@@ -794,8 +785,9 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
     t.n = 0;
     const int w0 = it.w0, w1 = it.w1;
     if constexpr (PPL == 1) {
-        // records by vector loads, broadcast per line; the wave's LDS block
-        // holds the W4 candidate scan of batch_w4
+        // region-I segments staged, the others by vector loads broadcast
+        // per line; the wave's LDS block holds the staged X0 or the W4
+        // candidate scan of batch_w4
         int* incl = reinterpret_cast<int*>(xs);
         const GlobPos gpos = (GlobPos)jb.pos;
         const GlobRec grec = (GlobRec)jb.rec;
@@ -803,13 +795,23 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         if (PROF == GX_PROF_VOIGT && it.s1 < it.s4) {
             const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
             const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
-            lines_batched<PROF, kMaskHi, false>(t, gpos, grec, grecx, ca, s1, w0, w1, lane, incl);
-            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s1, s2, w0, w1, lane, incl);
-            lines_batched<PROF, kMaskNone, false>(t, gpos, grec, grecx, s2, s3, w0, w1, lane, incl);
-            lines_batched<PROF, kMaskNone, true>(t, gpos, grec, grecx, s3, s4, w0, w1, lane, incl);
-            lines_batched<PROF, kMaskLo, false>(t, gpos, grec, grecx, s4, cb, w0, w1, lane, incl);
+            // the region-I segments through the staged loop (6 VALU per
+            // line against 14 with broadcasts), the rest batched
+            const ConstRec rec = (ConstRec)jb.rec;
+            const ConstPos pos = (ConstPos)jb.pos;
+            if (it.s2 > it.s1)
+                voigt_staged<1, kMaskHi>(t, grec, rec, pos, ca, s1, w0, lane, xs);
+            else
+                lines_batched<PROF, kMaskHi>(t, gpos, grec, grecx, ca, s1, w0, w1, lane, incl);
+            voigt_staged<1, kMaskNone>(t, grec, rec, pos, s1, s2, w0, lane, xs);
+            lines_batched<PROF, kMaskNone>(t, gpos, grec, grecx, s2, s3, w0, w1, lane, incl);
+            voigt_staged<1, kMaskNone>(t, grec, rec, pos, s3, s4, w0, lane, xs);
+            if (it.s3 < it.s4)
+                voigt_staged<1, kMaskLo>(t, grec, rec, pos, s4, cb, w0, lane, xs);
+            else
+                lines_batched<PROF, kMaskLo>(t, gpos, grec, grecx, s4, cb, w0, w1, lane, incl);
         } else {
-            lines_batched<PROF, kMaskBoth, false>(t, gpos, grec, grecx, ca, cb, w0, w1, lane, incl);
+            lines_batched<PROF, kMaskBoth>(t, gpos, grec, grecx, ca, cb, w0, w1, lane, incl);
         }
     } else if (PROF == GX_PROF_VOIGT) {
         const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
