@@ -516,18 +516,36 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
         const int nb = min(GX_WAVE, b - base);
         const ConstRec rb = rec + base;
         if (MASK != kMaskNone) {
+            // four lines at a time, the next four loading meanwhile
             const ConstPos pb = pos + base;
-            LinePos pn = pb[0];
-            VRec rn = rb[0];
-            for (int k = 0; k < nb; ++k) {
-                const LinePos lp = pn;
-                const VRec r = rn;
-                if (k + 1 < nb) {
-                    pn = pb[k + 1];
-                    rn = rb[k + 1];
+            int k = 0;
+            if (nb >= 8) {
+                VRec c0 = rb[0], c1 = rb[1], c2 = rb[2], c3 = rb[3];
+                LinePos p0 = pb[0], p1 = pb[1], p2 = pb[2], p3 = pb[3];
+                for (; k + 8 <= nb; k += 4) {
+                    const VRec n0 = rb[k + 4], n1 = rb[k + 5], n2 = rb[k + 6], n3 = rb[k + 7];
+                    const LinePos q0 = pb[k + 4], q1 = pb[k + 5], q2 = pb[k + 6], q3 = pb[k + 7];
+                    __builtin_amdgcn_sched_barrier(0);
+                    staged_masked<PPL, MASK>(t, xs + (k + 0) * PPL, c0, p0, nl, w0);
+                    staged_masked<PPL, MASK>(t, xs + (k + 1) * PPL, c1, p1, nl, w0);
+                    staged_masked<PPL, MASK>(t, xs + (k + 2) * PPL, c2, p2, nl, w0);
+                    staged_masked<PPL, MASK>(t, xs + (k + 3) * PPL, c3, p3, nl, w0);
+                    c0 = n0;
+                    c1 = n1;
+                    c2 = n2;
+                    c3 = n3;
+                    p0 = q0;
+                    p1 = q1;
+                    p2 = q2;
+                    p3 = q3;
                 }
-                staged_masked<PPL, MASK>(t, xs + k * PPL, r, lp, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 0) * PPL, c0, p0, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 1) * PPL, c1, p1, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 2) * PPL, c2, p2, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 3) * PPL, c3, p3, nl, w0);
+                k += 4;
             }
+            for (; k < nb; ++k) staged_masked<PPL, MASK>(t, xs + k * PPL, rb[k], pb[k], nl, w0);
             continue;
         }
         int k = 0;
@@ -617,26 +635,35 @@ __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp,
     t.tick(1);
 }
 
-// lines [a, b) through checked_line, records by scalar loads; the next
-// line's records load while this one is evaluated
+// lines [a, b) through checked_line, records by scalar loads, two lines at
+// a time, the next two loading while these are evaluated
 template <int PROF, int PPL, int MASK>
 __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, ConstRec rec,
                                               ConstRecX recx, int a, int b, int w0, int w1) {
-    if (a >= b) return;
-    LinePos lpn = pos[a];
-    VRec rn = rec[a];
-    VRecX rxn = recx[a];
-    for (int i = a; i < b; ++i) {
-        const LinePos lp = lpn;
-        const VRec r = rn;
-        const VRecX rx = rxn;
-        if (i + 1 < b) {
-            lpn = pos[i + 1];
-            rn = rec[i + 1];
-            rxn = recx[i + 1];
+    int i = a;
+    if (b - a >= 4) {
+        LinePos l0 = pos[i], l1 = pos[i + 1];
+        VRec r0 = rec[i], r1 = rec[i + 1];
+        VRecX x0 = recx[i], x1 = recx[i + 1];
+        for (; i + 4 <= b; i += 2) {
+            const LinePos m0 = pos[i + 2], m1 = pos[i + 3];
+            const VRec s0 = rec[i + 2], s1 = rec[i + 3];
+            const VRecX y0 = recx[i + 2], y1 = recx[i + 3];
+            __builtin_amdgcn_sched_barrier(0);
+            checked_line<PROF, PPL, MASK>(t, l0, r0, x0, w0, w1);
+            checked_line<PROF, PPL, MASK>(t, l1, r1, x1, w0, w1);
+            l0 = m0;
+            l1 = m1;
+            r0 = s0;
+            r1 = s1;
+            x0 = y0;
+            x1 = y1;
         }
-        checked_line<PROF, PPL, MASK>(t, lp, r, rx, w0, w1);
+        checked_line<PROF, PPL, MASK>(t, l0, r0, x0, w0, w1);
+        checked_line<PROF, PPL, MASK>(t, l1, r1, x1, w0, w1);
+        i += 2;
     }
+    for (; i < b; ++i) checked_line<PROF, PPL, MASK>(t, pos[i], rec[i], recx[i], w0, w1);
 }
 
 __device__ __forceinline__ float bcast(float v, int k) {
