@@ -453,131 +453,96 @@ __device__ __forceinline__ int slice_in_window(int j0, const LinePos& lp) {
 }
 
 // lines [a, b) whose every point in the tile is in Humlicek region I.  Per
-// batch of 64 lines, lane k loads line a + k's (cf, A, B) with one vector
-// load and stages X0q = (cf - (w0 + 64 q)) A + B, the line's x at lane 0 of
-// slice q, for every slice in the wave's LDS block xs (64 x PPL floats); the
-// batch's lines then read their X0q with one broadcast LDS read each and take
-// A, Kl, E2, H by scalar loads.  Per point x = X0q - lane A: one FMA, no
-// per-line vector work.  |X0q - x| <= 63 A, so x keeps FP32 accuracy except
+// batch of 64 lines, lane k loads line a + k's record with one vector load
+// and writes row k of the wave's LDS block xs: A, Kl, E2, H, then
+// X0q = (cf - (w0 + 64 q)) A + B, the line's x at lane 0 of slice q, for
+// every slice (rows of stage_row<PPL>() floats, 16-byte aligned).  The
+// batch's lines then read their row with broadcast ds_read_b128 and, per
+// point, x = X0q - lane A: one FMA, no per-line vector work and no scalar
+// loads in the loop (LDS reads return in order, so the loop waits only on
+// the rows it uses).  |X0q - x| <= 63 A, so x keeps FP32 accuracy except
 // where |x| << 63 A; in region I there |x| + y >= 15 makes y >= 15 - |x|,
 // and K's relative slope in x, below 2|x| / (x^2 + y^2), keeps the error of
 // K under 1e-6.
-// MASK kMaskNone: the windows cover the tile; four lines at a time, the next
-// four loading while these are evaluated; one Kahan fold per batch.
+// MASK kMaskNone: the windows cover the tile; one Kahan fold per batch.
 // kMaskHi / kMaskLo: the windows end / start inside the tile; slices outside
-// the window are skipped, the one the edge cuts is masked per lane.
+// the window are skipped, the one the edge cuts is masked per lane (window
+// bounds by scalar loads, four lines at a time).
 template <int PPL>
-__device__ __forceinline__ void staged_group(TileAcc<PPL>& t, const float* xs, const VRec& r,
-                                             float nl) {
-#pragma unroll
-    for (int q = 0; q < PPL; ++q)
-        t.part[q] += region1(fmaf(nl, r.A, xs[q]), r.Kl, r.E2, r.H);
-}
+constexpr int stage_row() { return 4 + (PPL + 3) / 4 * 4; }
 
 template <int PPL, int MASK>
-__device__ __forceinline__ void staged_masked(TileAcc<PPL>& t, const float* xs, const VRec& r,
+__device__ __forceinline__ void staged_masked(TileAcc<PPL>& t, const float* row,
                                               const LinePos& lp, float nl, int w0) {
+    const float A = row[0], Kl = row[1], E2 = row[2], H = row[3];
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
         const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
         if (st == 0) continue;
-        const float v = region1(fmaf(nl, r.A, xs[q]), r.Kl, r.E2, r.H);
+        const float v = region1(fmaf(nl, A, row[4 + q]), Kl, E2, H);
         t.part[q] += (st == 2 || in_window<MASK>(t.jj[q], lp)) ? v : 0.0f;
     }
     t.tick(1);
 }
 
 template <int PPL, int MASK>
-__device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, ConstRec rec,
-                                             ConstPos pos, int a, int b, int w0, int lane,
-                                             float* xs) {
+__device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, ConstPos pos, int a,
+                                             int b, int w0, int lane, float* xs) {
+    constexpr int R = stage_row<PPL>();
     if (a >= b) return;
     if (MASK == kMaskNone && t.n > 0) t.fold();
     const float nl = -(float)lane;
-    float cfn, An, Bn;
-    {
-        const int li = min(a + lane, b - 1);
-        cfn = grec[li].cf;
-        An = grec[li].A;
-        Bn = grec[li].B;
-    }
+    VRec vn = grec[min(a + lane, b - 1)];
     for (int base = a; base < b; base += GX_WAVE) {
-        const float cf = cfn, A = An, B = Bn;
-        {
-            // unconditional (the last batch reloads line b - 1): every
-            // batch has the same loads in flight
-            const int li = min(base + GX_WAVE + lane, b - 1);
-            cfn = grec[li].cf;
-            An = grec[li].A;
-            Bn = grec[li].B;
-        }
+        const VRec vc = vn;
+        // unconditional (the last batch reloads line b - 1): every batch has
+        // the same load in flight
+        vn = grec[min(base + GX_WAVE + lane, b - 1)];
+        float* row = xs + lane * R;
+        row[0] = vc.A;
+        row[1] = vc.Kl;
+        row[2] = vc.E2;
+        row[3] = vc.H;
 #pragma unroll
-        for (int q = 0; q < PPL; ++q) xs[lane * PPL + q] = fmaf(cf - (float)(w0 + GX_WAVE * q), A, B);
+        for (int q = 0; q < PPL; ++q) row[4 + q] = fmaf(vc.cf - (float)(w0 + GX_WAVE * q), vc.A, vc.B);
         const int nb = min(GX_WAVE, b - base);
-        const ConstRec rb = rec + base;
         if (MASK != kMaskNone) {
-            // four lines at a time, the next four loading meanwhile
             const ConstPos pb = pos + base;
             int k = 0;
             if (nb >= 8) {
-                VRec c0 = rb[0], c1 = rb[1], c2 = rb[2], c3 = rb[3];
                 LinePos p0 = pb[0], p1 = pb[1], p2 = pb[2], p3 = pb[3];
                 for (; k + 8 <= nb; k += 4) {
-                    const VRec n0 = rb[k + 4], n1 = rb[k + 5], n2 = rb[k + 6], n3 = rb[k + 7];
                     const LinePos q0 = pb[k + 4], q1 = pb[k + 5], q2 = pb[k + 6], q3 = pb[k + 7];
                     __builtin_amdgcn_sched_barrier(0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 0) * PPL, c0, p0, nl, w0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 1) * PPL, c1, p1, nl, w0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 2) * PPL, c2, p2, nl, w0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 3) * PPL, c3, p3, nl, w0);
-                    c0 = n0;
-                    c1 = n1;
-                    c2 = n2;
-                    c3 = n3;
+                    staged_masked<PPL, MASK>(t, xs + (k + 0) * R, p0, nl, w0);
+                    staged_masked<PPL, MASK>(t, xs + (k + 1) * R, p1, nl, w0);
+                    staged_masked<PPL, MASK>(t, xs + (k + 2) * R, p2, nl, w0);
+                    staged_masked<PPL, MASK>(t, xs + (k + 3) * R, p3, nl, w0);
                     p0 = q0;
                     p1 = q1;
                     p2 = q2;
                     p3 = q3;
                 }
-                staged_masked<PPL, MASK>(t, xs + (k + 0) * PPL, c0, p0, nl, w0);
-                staged_masked<PPL, MASK>(t, xs + (k + 1) * PPL, c1, p1, nl, w0);
-                staged_masked<PPL, MASK>(t, xs + (k + 2) * PPL, c2, p2, nl, w0);
-                staged_masked<PPL, MASK>(t, xs + (k + 3) * PPL, c3, p3, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 0) * R, p0, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 1) * R, p1, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 2) * R, p2, nl, w0);
+                staged_masked<PPL, MASK>(t, xs + (k + 3) * R, p3, nl, w0);
                 k += 4;
             }
-            for (; k < nb; ++k) staged_masked<PPL, MASK>(t, xs + k * PPL, rb[k], pb[k], nl, w0);
+            for (; k < nb; ++k) staged_masked<PPL, MASK>(t, xs + k * R, pb[k], nl, w0);
             continue;
         }
-        int k = 0;
-        if (nb >= 8) {
-            VRec c0 = rb[0], c1 = rb[1], c2 = rb[2], c3 = rb[3];
-            for (; k + 8 <= nb; k += 4) {
-                const VRec n0 = rb[k + 4], n1 = rb[k + 5], n2 = rb[k + 6], n3 = rb[k + 7];
-                __builtin_amdgcn_sched_barrier(0);
-                staged_group(t, xs + (k + 0) * PPL, c0, nl);
-                staged_group(t, xs + (k + 1) * PPL, c1, nl);
-                staged_group(t, xs + (k + 2) * PPL, c2, nl);
-                staged_group(t, xs + (k + 3) * PPL, c3, nl);
-                c0 = n0;
-                c1 = n1;
-                c2 = n2;
-                c3 = n3;
-            }
-            staged_group(t, xs + (k + 0) * PPL, c0, nl);
-            staged_group(t, xs + (k + 1) * PPL, c1, nl);
-            staged_group(t, xs + (k + 2) * PPL, c2, nl);
-            staged_group(t, xs + (k + 3) * PPL, c3, nl);
-            k += 4;
+#pragma unroll 4
+        for (int k = 0; k < nb; ++k) {
+            const float* rk = xs + k * R;
+            const float A = rk[0], Kl = rk[1], E2 = rk[2], H = rk[3];
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) t.part[q] += region1(fmaf(nl, A, rk[4 + q]), Kl, E2, H);
         }
-        for (; k < nb; ++k) staged_group(t, xs + k * PPL, rb[k], nl);
         t.fold();
     }
 }
 
-// lines with per-line decisions: window masks per lane (only the edges MASK
-// names; slices wholly outside the window are skipped, slices wholly inside
-// are unmasked), region I inline, regions II-IV per lane; the next line's
-// records load while this one is evaluated
 // one line with per-line decisions: window masks per lane (only the edges
 // MASK names; slices wholly outside the window are skipped, slices wholly
 // inside are unmasked), region I inline, regions II-IV per lane
@@ -824,17 +789,16 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
             // the region-I segments through the staged loop (6 VALU per
             // line against 14 with broadcasts), the rest batched
-            const ConstRec rec = (ConstRec)jb.rec;
             const ConstPos pos = (ConstPos)jb.pos;
             if (it.s2 > it.s1)
-                voigt_staged<1, kMaskHi>(t, grec, rec, pos, ca, s1, w0, lane, xs);
+                voigt_staged<1, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskHi>(t, gpos, grec, grecx, ca, s1, w0, w1, lane, incl);
-            voigt_staged<1, kMaskNone>(t, grec, rec, pos, s1, s2, w0, lane, xs);
+            voigt_staged<1, kMaskNone>(t, grec, pos, s1, s2, w0, lane, xs);
             lines_batched<PROF, kMaskNone>(t, gpos, grec, grecx, s2, s3, w0, w1, lane, incl);
-            voigt_staged<1, kMaskNone>(t, grec, rec, pos, s3, s4, w0, lane, xs);
+            voigt_staged<1, kMaskNone>(t, grec, pos, s3, s4, w0, lane, xs);
             if (it.s3 < it.s4)
-                voigt_staged<1, kMaskLo>(t, grec, rec, pos, s4, cb, w0, lane, xs);
+                voigt_staged<1, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskLo>(t, gpos, grec, grecx, s4, cb, w0, w1, lane, incl);
         } else {
@@ -851,14 +815,14 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             // the last one is far above (s3 < s4) so are all of [s4, hi)
             const GlobRec grec = (GlobRec)jb.rec;
             if (it.s2 > it.s1)
-                voigt_staged<PPL, kMaskHi>(t, grec, rec, pos, ca, s1, w0, lane, xs);
+                voigt_staged<PPL, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
-            voigt_staged<PPL, kMaskNone>(t, grec, rec, pos, s1, s2, w0, lane, xs);
+            voigt_staged<PPL, kMaskNone>(t, grec, pos, s1, s2, w0, lane, xs);
             lines_checked<PROF, PPL, kMaskNone>(t, pos, rec, recx, s2, s3, w0, w1);
-            voigt_staged<PPL, kMaskNone>(t, grec, rec, pos, s3, s4, w0, lane, xs);
+            voigt_staged<PPL, kMaskNone>(t, grec, pos, s3, s4, w0, lane, xs);
             if (it.s3 < it.s4)
-                voigt_staged<PPL, kMaskLo>(t, grec, rec, pos, s4, cb, w0, lane, xs);
+                voigt_staged<PPL, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
         } else {
@@ -949,13 +913,14 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
     __shared__ double red[SPLIT > 1 ? SPLIT * GX_WAVE * PPL : 1];
-    // per wave, 64 x PPL words: the staged X0 of one batch of 64 region-I
-    // lines, or at PPL 1 the W4 candidate scan of one batch
-    __shared__ float xs_all[block_threads<SPLIT>() * PPL];
+    // per wave, 64 rows of stage_row<PPL>() words: the staged rows of one
+    // batch of 64 region-I lines, or at PPL 1 the W4 candidate scan of one
+    // batch
+    __shared__ float xs_all[block_threads<SPLIT>() * stage_row<PPL>()];
     __shared__ int grab;
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float* xs = xs_all + wv * GX_WAVE * PPL;
+    float* xs = xs_all + wv * GX_WAVE * stage_row<PPL>();
     const int lb = mode == 3 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (mode == 0 || mode == 3) {
         if (SPLIT == 1) {
