@@ -631,6 +631,68 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
     for (; i < b; ++i) checked_line<PROF, PPL, MASK>(t, pos[i], rec[i], recx[i], w0, w1);
 }
 
+// Voigt lines centred near the tile (kMaskNone) staged like the region-I
+// loop: per batch of 64 lines lane k loads line k's LinePos, VRec and VRecX
+// with vector loads and writes a 12-word row (cf, A, B, Kl | E2, H, Kg, y |
+// c, k15); each line then reads its row with three broadcast ds_read_b128,
+// takes c and k15 into SGPRs (readfirstlane) for its scalar decisions and
+// runs checked_line on VGPR operands.  No scalar loads in the loop.
+constexpr int kNearRow = 12;
+
+// words per line row of the wave's LDS block (the wider of the two uses)
+template <int PPL>
+constexpr int wave_row() { return stage_row<PPL>() > kNearRow ? stage_row<PPL>() : kNearRow; }
+
+template <int PPL>
+__device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobRec grec,
+                                            GlobRecX grecx, int a, int b, int w0, int w1,
+                                            int lane, float* xs) {
+    if (a >= b) return;
+    int li = min(a + lane, b - 1);
+    LinePos pn = gpos[li];
+    VRec rn = grec[li];
+    VRecX xn = grecx[li];
+    for (int base = a; base < b; base += GX_WAVE) {
+        const LinePos pc = pn;
+        const VRec rc = rn;
+        const VRecX xc = xn;
+        li = min(base + GX_WAVE + lane, b - 1);
+        pn = gpos[li];
+        rn = grec[li];
+        xn = grecx[li];
+        float* row = xs + lane * kNearRow;
+        row[0] = rc.cf;
+        row[1] = rc.A;
+        row[2] = rc.B;
+        row[3] = rc.Kl;
+        row[4] = rc.E2;
+        row[5] = rc.H;
+        row[6] = rc.Kg;
+        row[7] = xc.y;
+        row[8] = __int_as_float(pc.c);
+        row[9] = __int_as_float(xc.k15);
+        const int nb = min(GX_WAVE, b - base);
+        for (int k = 0; k < nb; ++k) {
+            const float* rk = xs + k * kNearRow;
+            VRec r;
+            r.cf = rk[0];
+            r.A = rk[1];
+            r.B = rk[2];
+            r.Kl = rk[3];
+            r.E2 = rk[4];
+            r.H = rk[5];
+            r.Kg = rk[6];
+            VRecX rx;
+            rx.y = rk[7];
+            rx.k15 = __builtin_amdgcn_readfirstlane(__float_as_int(rk[9]));
+            LinePos lp;
+            lp.c = __builtin_amdgcn_readfirstlane(__float_as_int(rk[8]));
+            lp.ib = lp.ie = 0;  // the windows cover the tile (kMaskNone)
+            checked_line<GX_PROF_VOIGT, PPL, kMaskNone>(t, lp, r, rx, w0, w1);
+        }
+    }
+}
+
 __device__ __forceinline__ float bcast(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
@@ -819,7 +881,7 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             else
                 lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
             voigt_staged<PPL, kMaskNone>(t, grec, pos, s1, s2, w0, lane, xs);
-            lines_checked<PROF, PPL, kMaskNone>(t, pos, rec, recx, s2, s3, w0, w1);
+            near_staged<PPL>(t, (GlobPos)jb.pos, grec, (GlobRecX)jb.recx, s2, s3, w0, w1, lane, xs);
             voigt_staged<PPL, kMaskNone>(t, grec, pos, s3, s4, w0, lane, xs);
             if (it.s3 < it.s4)
                 voigt_staged<PPL, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
@@ -916,11 +978,11 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
     // per wave, 64 rows of stage_row<PPL>() words: the staged rows of one
     // batch of 64 region-I lines, or at PPL 1 the W4 candidate scan of one
     // batch
-    __shared__ float xs_all[block_threads<SPLIT>() * stage_row<PPL>()];
+    __shared__ float xs_all[block_threads<SPLIT>() * wave_row<PPL>()];
     __shared__ int grab;
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float* xs = xs_all + wv * GX_WAVE * stage_row<PPL>();
+    float* xs = xs_all + wv * GX_WAVE * wave_row<PPL>();
     const int lb = mode == 3 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (mode == 0 || mode == 3) {
         if (SPLIT == 1) {
