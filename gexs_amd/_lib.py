@@ -82,7 +82,9 @@ class Request(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("npoints", C.c_int32), ("sigma", C.POINTER(C.c_double)),
-                ("latency_ms", C.c_double), ("device_ms", C.c_double)]
+                ("latency_ms", C.c_double), ("device_ms", C.c_double),
+                ("tile_points", C.c_int32), ("waves_per_tile", C.c_int32),
+                ("line_groups", C.c_int32), ("reserved", C.c_int32)]
 
 
 _lib = None

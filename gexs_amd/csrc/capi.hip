@@ -1336,6 +1336,10 @@ static int fill_result(gexs_ctx* c, int rc, gxs::Result* r, gexs_result* out) {
     out->sigma = r->status == 0 ? r->sigma.data() : nullptr;
     out->latency_ms = r->latency_ms;
     out->device_ms = r->device_ms;
+    out->tile_points = r->tile_points;
+    out->waves_per_tile = r->waves_per_tile;
+    out->line_groups = r->line_groups;
+    out->reserved = 0;
     return GEXS_OK;
 }
 

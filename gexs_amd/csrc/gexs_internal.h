@@ -68,12 +68,16 @@ struct __attribute__((aligned(8))) VRecX {
 //   tile (no checks), [s2, s3) line centres near the tile (checked),
 //   [s3, s4) region I, [s4, hi) window starts inside the tile.
 // s1 = s4 = hi when no line covers the whole tile (every line checked).
+// With ngroups > 1 the tile's lines are cut into ngroups contiguous groups,
+// one work item each (group = 0 .. ngroups - 1); such an item writes its FP64
+// partial sums to row `slot` (tile * ngroups + group) of its job's Job::part
+// and k_group_sum adds the rows of a tile in group order.
 struct __attribute__((aligned(16))) WorkItem {
     int32_t lo, hi;
     int32_t w0, w1;
     int32_t job;
     int32_t s1, s2, s3, s4;
-    int32_t pad[3];
+    int32_t group, ngroups, slot;
 };
 
 // one cross-section
@@ -82,6 +86,7 @@ struct __attribute__((aligned(16))) Job {
     const VRec* rec;
     const VRecX* recx;
     double* out;         // grid (index = grid point)
+    double* part;        // rows of 64 * PPL FP64 partial sums, one per work item (line groups)
     int32_t n;
     int32_t profile;
     int32_t scale_slot;  // index into the scale array (uint64 bit pattern of max amplitude)
@@ -98,6 +103,7 @@ struct JobDesc {
     int32_t items_off;  // first work item
     int32_t ntiles;     // number of work items (wave-tiles)
     int32_t q_slot;     // >= 0: Q(T) is taken from this slot of a device partition pass
+    int32_t groups;     // line groups per tile (<= 1: one); items = ntiles * groups
 };
 
 // per-(T,P) physics of one job, FP64

@@ -28,6 +28,9 @@ hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
                  uint32_t* item_idx, int job0 = 0);
 hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, TPParams* tp);
+// items of tiles cut into line groups (WorkItem::ngroups > 1): every point of
+// such a tile = the sum of its groups' rows of Job::part (span = 64 * PPL)
+hipError_t group_sum(hipStream_t s, const WorkItem* items, int nitems, const Job* jobs, int span);
 hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx, int n,
                          WorkItem* out);
 // split 1: one wave per item; split 4: the four waves of a workgroup share an

@@ -306,13 +306,18 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.s1 = lf < rf ? lf : hi;
     it.s4 = lf < rf ? rf : hi;
     it.s2 = it.s3 = it.s4;  // [s1, s4) all region I until k_segments runs
-    it.pad[0] = it.pad[1] = it.pad[2] = 0;
-    const int slot = d.items_off + t;
-    items[slot] = it;
-    if (cost_key) {
-        const uint32_t c = (uint32_t)max(hi - lo, 0);
-        cost_key[slot] = 0xFFFFFFFFu - c;  // ascending key = heaviest first
-        item_idx[slot] = (uint32_t)slot;
+    const int G = max(d.groups, 1);
+    it.ngroups = G;
+    for (int g = 0; g < G; ++g) {
+        const int slot = d.items_off + t * G + g;
+        it.group = g;
+        it.slot = t * G + g;  // row of the job's partial sums
+        items[slot] = it;
+        if (cost_key) {
+            const uint32_t c = (uint32_t)max(hi - lo, 0) / (uint32_t)G;
+            cost_key[slot] = 0xFFFFFFFFu - c;  // ascending key = heaviest first
+            item_idx[slot] = (uint32_t)slot;
+        }
     }
 }
 
@@ -903,17 +908,35 @@ __device__ __forceinline__ void store_point(const Job& jb, int j, double v) {
         jb.out[j] = v;
 }
 
+// the item's lines: its group's contiguous share of the tile's [lo, hi)
+__device__ __forceinline__ void group_lines(const WorkItem& it, int* a, int* b) {
+    const int n = max(it.hi - it.lo, 0), G = max(it.ngroups, 1);
+    *a = it.lo + (int)(((long long)n * it.group) / G);
+    *b = it.lo + (int)(((long long)n * (it.group + 1)) / G);
+}
+
+// value k of the tile (point w0 + k): to the grid, or with line groups to
+// the item's row of partial sums
+template <int PPL>
+__device__ __forceinline__ void store_tile(const WorkItem& it, const Job& jb, int k, double v) {
+    if (it.ngroups > 1)
+        jb.part[(size_t)it.slot * (GX_WAVE * PPL) + k] = v;
+    else if (it.w0 + k < it.w1)
+        store_point(jb, it.w0 + k, v);
+}
+
 // one wave evaluates the whole item
 template <int PROF, int PPL>
 __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
                                          int lane, float* xs) {
     TileAcc<PPL> t;
-    sum_lines<PROF, PPL>(t, it, jb, lane, it.lo, it.hi, xs);
+    int a, b;
+    group_lines(it, &a, &b);
+    sum_lines<PROF, PPL>(t, it, jb, lane, a, b, xs);
 #pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-        const double v = ((double)t.sum[q] - (double)t.cmp[q]) * inv_scale;
-        if (t.jj[q] < it.w1) store_point(jb, t.jj[q], v);
-    }
+    for (int q = 0; q < PPL; ++q)
+        store_tile<PPL>(it, jb, GX_WAVE * q + lane,
+                        ((double)t.sum[q] - (double)t.cmp[q]) * inv_scale);
 }
 
 constexpr int kWavesPerBlock = GX_WG / GX_WAVE;  // SPLIT 1: items per workgroup
@@ -929,9 +952,11 @@ __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb
                                                double inv_scale, int lane, int wv,
                                                double* red, float* xs) {
     TileAcc<PPL> t;
-    const int n = max(it.hi - it.lo, 0);
-    const int ca = it.lo + (int)(((long long)n * wv) / SPLIT);
-    const int cb = it.lo + (int)(((long long)n * (wv + 1)) / SPLIT);
+    int a, b;
+    group_lines(it, &a, &b);
+    const int n = b - a;
+    const int ca = a + (int)(((long long)n * wv) / SPLIT);
+    const int cb = a + (int)(((long long)n * (wv + 1)) / SPLIT);
     sum_lines<PROF, PPL>(t, it, jb, lane, ca, cb, xs);
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
@@ -941,8 +966,7 @@ __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb
         double v = red[k];
 #pragma unroll
         for (int w = 1; w < SPLIT; ++w) v += red[(size_t)w * GX_WAVE * PPL + k];
-        const int j = it.w0 + k;
-        if (j < it.w1) store_point(jb, j, v * inv_scale);
+        store_tile<PPL>(it, jb, k, v * inv_scale);
     }
     __syncthreads();  // red is reused by the next item
 }
@@ -1028,6 +1052,24 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
                                                  lane, wv, red, xs);
         }
     }
+}
+
+// Line groups: point k of a tile is the sum of its groups' partial rows in
+// group order (fixed, so a result does not depend on the launch).  One
+// workgroup of `span` threads per item; only items of group 0 write.
+__global__ __launch_bounds__(512) void k_group_sum(const WorkItem* __restrict__ items, int nitems,
+                                                   const Job* __restrict__ jobs, int span) {
+    const int i = blockIdx.x;
+    if (i >= nitems) return;
+    const WorkItem it = items[i];
+    if (it.ngroups <= 1 || it.group != 0) return;
+    const int k = threadIdx.x;
+    if (k >= span || it.w0 + k >= it.w1) return;
+    const Job jb = jobs[it.job];
+    const double* row = jb.part + (size_t)it.slot * span + k;
+    double v = 0.0;
+    for (int g = 0; g < it.ngroups; ++g) v += row[(size_t)g * span];
+    store_point(jb, it.w0 + k, v);
 }
 
 // ------------------------------------------------------ plain FP64 baseline
@@ -1215,6 +1257,13 @@ hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs
     if (njobs <= 0 || max_tiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tiles, dim3(blocks_for(max_tiles, 256), njobs), dim3(256), 0, s, pos, jd,
                        tp, span, job0, items, cost_key, item_idx);
+    return hipGetLastError();
+}
+
+hipError_t group_sum(hipStream_t s, const WorkItem* items, int nitems, const Job* jobs, int span) {
+    if (nitems <= 0) return hipSuccess;
+    if (span <= 0 || span > 512) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_group_sum, dim3(nitems), dim3(span), 0, s, items, nitems, jobs, span);
     return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ struct Result {
     int npoints = 0;
     std::vector<double> sigma;
     double latency_ms = 0.0, device_ms = 0.0;
+    int tile_points = 0, waves_per_tile = 0, line_groups = 0;  // launch shape
 };
 
 // first-fit allocator over one device allocation (offsets in bytes)
@@ -99,8 +100,9 @@ class Server {
         uint64_t id;
         Request r;
         int npoints = 0, ntiles = 0;
-        int ppl = 1;    // tile width 64*ppl points and
-        int split = 4;  // waves per tile, both from the request's own size
+        int ppl = 1;    // tile width 64*ppl points,
+        int split = 4;  // waves per tile and
+        int groups = 1; // line groups per tile (workgroups per tile), all from the request's own size
         int64_t l0 = 0, n = 0;
         size_t foot = 0, off = 0;
         std::chrono::steady_clock::time_point arrival;
@@ -116,6 +118,8 @@ class Server {
         size_t tab_cap = 0;
         void* items = nullptr;  // work items
         size_t items_cap = 0;
+        void* part = nullptr;  // partial rows of line-grouped tiles
+        size_t part_cap = 0;
         void* small = nullptr;  // scale | neval | kmax | counter | Q | partition partials
         double* host_out = nullptr;  // pinned copy-out
         size_t host_cap = 0;
@@ -125,6 +129,7 @@ class Server {
     };
     void loop();
     void request_shape(Req& q) const;
+    void request_groups(Req& q) const;
     int launch(Slot& s, std::vector<Req*>& batch, std::string* err);
     void finish(Slot& s);
     size_t footprint(const Req& q) const;

@@ -124,7 +124,7 @@ Server::~Server() {
     (void)hipSetDevice(device_);
     for (auto& s : slots_) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
-        void* bufs[] = {s->tab, s->items, s->small};
+        void* bufs[] = {s->tab, s->items, s->small, s->part};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (s->host_out) (void)hipHostFree(s->host_out);
@@ -160,6 +160,10 @@ int Server::submit(const Request& r, uint64_t* id, std::string* err) {
     auto b = std::upper_bound(a, v_.nu_host.end(), hi);
     q->l0 = a - v_.nu_host.begin();
     q->n = b - a;
+    request_groups(*q);
+    q->res.tile_points = GX_WAVE * q->ppl;
+    q->res.waves_per_tile = q->split;
+    q->res.line_groups = q->groups;
     q->foot = footprint(*q);
     if ((int64_t)q->foot > budget_) {
         *err = "request footprint exceeds the server memory budget";
@@ -242,6 +246,29 @@ void Server::request_shape(Req& q) const {
     q.split = (int64_t)q.ntiles * split >= want_waves ? split : 16;
 }
 
+// Line groups: a request whose tiles give the device fewer than kFillWaves
+// waves also cuts every tile's lines into groups, one workgroup each (their
+// partial sums added in group order by k_group_sum), as long as a wave keeps
+// kMinLinesPerWave lines.  Lines per tile are estimated from the request's
+// selected lines, its width and the cutoff window.
+void Server::request_groups(Req& q) const {
+    constexpr int64_t kFillWaves = 4096;
+    constexpr int kMaxGroups = 64;
+    constexpr double kMinLinesPerWave = 256.0;
+    q.groups = 1;
+    const int64_t waves = (int64_t)q.ntiles * q.split;
+    if (waves >= kFillWaves || q.n <= 0) return;
+    const double dnu = (q.r.end - q.r.start) / (double)q.npoints;
+    const double span_cm = GX_WAVE * q.ppl * dnu;
+    const double cut = cutoff_of(ph_, q.r.P);
+    const double lines_per_tile = (double)q.n * std::min(1.0, (span_cm + 2.0 * cut) /
+                                                                  (q.r.end - q.r.start + 2.0 * ph_.half_width));
+    int g = (int)((kFillWaves + waves - 1) / waves);
+    g = std::min(g, kMaxGroups);
+    g = std::min(g, (int)(lines_per_tile / (kMinLinesPerWave * q.split)));
+    q.groups = std::max(g, 1);
+}
+
 int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     const int nreq = (int)batch.size();
     hipStream_t st = s.stream;
@@ -257,6 +284,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     std::vector<Job> jobs((size_t)nreq);
     std::vector<double> Ts((size_t)nreq);
     int items_total = 0, max_tiles = 0;
+    size_t part_total = 0;  // doubles
     int64_t max_n = 0;
     size_t host_total = 0;
     for (int k = 0; k < nreq; ++k) {
@@ -291,6 +319,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         d.items_off = items_total;
         d.ntiles = q->ntiles;
         d.q_slot = k;
+        d.groups = q->groups;
         Job& jb = jobs[(size_t)k];
         jb.pos = (const LinePos*)(base + pos_b);
         jb.rec = (const VRec*)(base + rec_b);
@@ -300,8 +329,10 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         jb.profile = ph_.profile;
         jb.scale_slot = k;
         jb.accumulate = 0;  // each point is written exactly once
+        jb.part = nullptr;  // set below, once the partial buffer is sized
         Ts[(size_t)k] = q->r.T;
-        items_total += q->ntiles;
+        items_total += q->ntiles * q->groups;
+        if (q->groups > 1) part_total += (size_t)q->ntiles * q->groups * GX_WAVE * q->ppl;
         max_tiles = std::max(max_tiles, q->ntiles);
         max_n = std::max<int64_t>(max_n, q->n);
         host_total += (size_t)q->npoints;
@@ -323,6 +354,16 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     if ((e = grow(&s.items, &s.items_cap, (size_t)std::max(items_total, 1) * sizeof(WorkItem))) !=
         hipSuccess)
         goto fail;
+    if (part_total > 0 && (e = grow(&s.part, &s.part_cap, part_total * 8)) != hipSuccess) goto fail;
+    {
+        size_t po = 0;
+        for (int k = 0; k < nreq; ++k) {
+            const Req* q = batch[(size_t)k];
+            if (q->groups <= 1) continue;
+            jobs[(size_t)k].part = (double*)s.part + po;
+            po += (size_t)q->ntiles * q->groups * GX_WAVE * q->ppl;
+        }
+    }
     if (s.stage_cap < tab_b) {
         if (s.stage) (void)hipHostFree(s.stage);
         s.stage = nullptr;
@@ -391,12 +432,20 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             goto fail;
         for (int k0 = 0; k0 < nreq;) {  // one profile launch per group, slots in dispatch order
             int k1 = k0, i1 = jds[(size_t)k0].items_off;
-            while (k1 < nreq && shape(batch[(size_t)k1]) == shape(batch[(size_t)k0]))
-                i1 += batch[(size_t)k1++]->ntiles;
+            bool grouped = false;
+            while (k1 < nreq && shape(batch[(size_t)k1]) == shape(batch[(size_t)k0])) {
+                grouped |= batch[(size_t)k1]->groups > 1;
+                i1 += batch[(size_t)k1]->ntiles * batch[(size_t)k1]->groups;
+                ++k1;
+            }
             const int i0 = jds[(size_t)k0].items_off;
             if ((e = gxk::profile_tiles(st, ph_.profile, batch[(size_t)k0]->ppl, batch[(size_t)k0]->split,
                                         d_items + i0,
                                         i1 - i0, d_jobs, d_scale, d_counter, nullptr, 3, 0)) != hipSuccess)
+                goto fail;
+            if (grouped &&
+                (e = gxk::group_sum(st, d_items + i0, i1 - i0, d_jobs, GX_WAVE * batch[(size_t)k0]->ppl)) !=
+                    hipSuccess)
                 goto fail;
             k0 = k1;
         }

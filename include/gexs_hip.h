@@ -245,6 +245,9 @@ typedef struct gexs_result {
     const double* sigma; /* valid until gexs_release(id) */
     double latency_ms;   /* submit -> completion */
     double device_ms;
+    /* the launch shape the server chose for the request (from its own size):
+     * tile width in points, waves per tile, workgroups (line groups) per tile */
+    int32_t tile_points, waves_per_tile, line_groups, reserved;
 } gexs_result;
 
 int gexs_serve_start(gexs_ctx* ctx, int max_inflight, int64_t mem_budget);

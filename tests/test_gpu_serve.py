@@ -75,3 +75,52 @@ def test_admission_refuses_oversized(server):
         m.submit(296.0, 1.0, 0.0, 40000.0, 1e-4)  # 4e8 points > 1 GiB budget
     with pytest.raises(GexsError):
         m.submit(-1.0, 1.0, 0.0, 10.0, 0.1)
+
+
+@pytest.fixture(scope="module")
+def dense_server():
+    # 2e6 lines over 500-15000 cm^-1: a 1 cm^-1 request of a few 64-point tiles
+    # has ~15 000 lines per tile, so the server cuts its tiles into line groups
+    L = synth.workload_list("voigt_tp_table_1e7", scale=0.2)
+    m = HipManager("voigt")
+    m.InitializeConstants(100.0, 296.0, 1.0, 0.01, 28.0, 1.0, 296.0, P_SCALE, cutoff=25.0)
+    m.load_list(L)
+    m.serve_start(max_inflight=3, mem_budget=2 << 30)
+    yield L, m
+    m.serve_stop()
+    m.close()
+
+
+DENSE = [(900.0, 50.0, 6000.0, 6200.0, 1.0), (2000.0, 1e-3, 9000.0, 9150.0, 1.0),
+         (300.0, 1.0, 4000.0, 4100.0, 0.1)]
+
+
+def test_line_groups_vs_oracle_and_mix(dense_server, O):
+    """Requests too small to fill the device run their tiles as several
+    workgroups each (line groups, partial sums added in group order): the
+    result matches the oracle and is the same alone and in a mix."""
+    L, m = dense_server
+    nu, E, A, g, _, _ = L.lines()
+    gam, nex = L.gamma.astype(np.float64), L.nexp.astype(np.float64)
+    A = L.A.astype(np.float64)
+    alone = []
+    for T, P, a, b, d in DENSE:
+        res = m.wait(m.submit(T, P, a, b, d))
+        tile_points, waves, groups = res["shape"]
+        print(f"T={T} P={P} [{a},{b}) d={d}: {tile_points}-point tiles, {waves} waves, "
+              f"{groups} line groups", flush=True)
+        assert groups > 1
+        alone.append(res["sigma"])
+        N = int(round((b - a) / d))
+        keep = (nu >= a - 100.0) & (nu <= b + 100.0) & (nu != 0)
+        Q = O.partition(L.E, L.g, T)
+        p = O.params("voigt", T=T, P=P, Q=Q, mass=28.0, cutoff=25.0, p_scale=P_SCALE)
+        ref, _ = O.cross_section(p, a, (b - a) / N, N, nu[keep], E[keep], A[keep], g[keep],
+                                 gam[keep], nex[keep], nthreads=8)
+        err = np.max(np.abs(res["sigma"] - ref)) / ref.max()
+        band = abs(res["sigma"].sum() - ref.sum()) / ref.sum()
+        print(f"  err {err:.2e} band {band:.2e}", flush=True)
+        assert err <= 1e-4 and band <= 1e-6
+    ids = [m.submit(*r) for r in DENSE * 3]
+    for k, i in enumerate(ids):
+        assert np.array_equal(m.wait(i)["sigma"], alone[k % len(DENSE)]), k
