@@ -698,6 +698,96 @@ __device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobR
     }
 }
 
+// Bin-averaged Gaussian (Doppler and the fixed-HWHM Gaussian) staged like
+// the region-I loop: per batch of 64 lines lane k writes row k (cf, A, B, Kg
+// | h, ib, ie); each line reads its row with two broadcast ds_read_b128 and
+// takes its window into SGPRs (readfirstlane).  The bins of neighbouring
+// points share an edge: point j spans x_j + h .. x_j - h (x falls by A = 2h
+// per point), so each lane evaluates erfc once, at its point's far edge
+// x_j - h, and takes the near edge from the lane before (DPP wave shift by
+// one; lane 0 from lane 63 of the slice before, or directly at a line's
+// first evaluated slice).  With F = erfc(|e|):
+//   both edges >= 0:  erf(a) - erf(b) = F(b) - F(a)
+//   both edges <  0:  erf(a) - erf(b) = F(a) - F(b)
+//   a >= 0 > b:       erf(a) - erf(b) = 2 - F(a) - F(b)
+// (no cancellation beyond that of the bin itself, as in bin_erf_f32).
+constexpr int kGaussRow = 8;
+
+__device__ __forceinline__ float wave_shr1(float v, float lane0) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(v),
+                                                      0x138 /* wave_shr:1 */, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float bin_from_edges(float ea, float Fa, float eb, float Fb) {
+    if (eb >= 0.0f) return Fb - Fa;
+    if (ea < 0.0f) return Fa - Fb;
+    return (2.0f - Fa) - Fb;
+}
+
+template <int PPL>
+__device__ __forceinline__ void gauss_staged(TileAcc<PPL>& t, GlobPos gpos, GlobRec grec,
+                                             GlobRecX grecx, int a, int b, int w0, int lane,
+                                             float* xs) {
+    if (a >= b) return;
+    int li = min(a + lane, b - 1);
+    LinePos pn = gpos[li];
+    VRec rn = grec[li];
+    VRecX xn = grecx[li];
+    for (int base = a; base < b; base += GX_WAVE) {
+        const LinePos pc = pn;
+        const VRec rc = rn;
+        const VRecX xc = xn;
+        li = min(base + GX_WAVE + lane, b - 1);
+        pn = gpos[li];
+        rn = grec[li];
+        xn = grecx[li];
+        float* row = xs + lane * kGaussRow;
+        row[0] = rc.cf;
+        row[1] = rc.A;
+        row[2] = rc.B;
+        row[3] = rc.Kg;
+        row[4] = xc.y;
+        row[5] = __int_as_float(pc.ib);
+        row[6] = __int_as_float(pc.ie);
+        const int nb = min(GX_WAVE, b - base);
+        for (int k = 0; k < nb; ++k) {
+            const float* rk = xs + k * kGaussRow;
+            const float cf = rk[0], A = rk[1], B = rk[2], Kg = rk[3], h = rk[4];
+            LinePos lp;
+            lp.ib = __builtin_amdgcn_readfirstlane(__float_as_int(rk[5]));
+            lp.ie = __builtin_amdgcn_readfirstlane(__float_as_int(rk[6]));
+            bool have = false;  // the slice before was evaluated
+            float prevE = 0.0f, prevF = 0.0f;
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                const int st = slice_in_window<kMaskBoth>(w0 + GX_WAVE * q, lp);
+                if (st == 0) {
+                    have = false;
+                    continue;
+                }
+                const float x = fmaf(cf - t.jf[q], A, B);
+                const float eb = x - h;
+                const float Fb = erfcf(fabsf(eb));
+                float ea, Fa;
+                if (have) {
+                    ea = wave_shr1(eb, prevE);
+                    Fa = wave_shr1(Fb, prevF);
+                } else {
+                    ea = x + h;
+                    Fa = erfcf(fabsf(ea));
+                }
+                const float v = bin_from_edges(ea, Fa, eb, Fb);
+                t.part[q] = fmaf(Kg, (st == 2 || in_window<kMaskBoth>(t.jj[q], lp)) ? v : 0.0f,
+                                 t.part[q]);
+                prevE = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(eb), GX_WAVE - 1));
+                prevF = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fb), GX_WAVE - 1));
+                have = true;
+            }
+            t.tick(1);
+        }
+    }
+}
+
 __device__ __forceinline__ float bcast(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
@@ -895,6 +985,9 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         } else {
             lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
         }
+    } else if (PROF == GX_PROF_DOPPLER) {
+        gauss_staged<PPL>(t, (GlobPos)jb.pos, (GlobRec)jb.rec, (GlobRecX)jb.recx, ca, cb, w0, lane,
+                          xs);
     } else {
         lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
     }
