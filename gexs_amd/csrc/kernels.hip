@@ -3,14 +3,17 @@
 // Per (T,P) the path is
 //   k_line_max      FP64 line amplitudes -> max (power-of-two scale of the FP32 sums)
 //   k_line_rec      FP64 S, gammaL, 1/w, window -> 32-B FP32 record per line, N_eval
-//   k_profile_tiles one wavefront per wave-tile of 64*PPL grid points; walks the
-//                   nu-sorted lines whose window meets the tile with scalar loads,
-//                   evaluates the profile in FP32 and keeps per-point Kahan sums in
-//                   registers; writes each point once (no atomics on the grid)
+//   k_segments      per tile, the lines centred near it (Humlicek regions II-IV)
+//   k_profile_tiles a tile of 64*PPL grid points per wave or per workgroup (its
+//                   waves share the tile's nu-sorted lines); lines are staged per
+//                   64-line batch into LDS rows, the profile is evaluated in FP32
+//                   with per-point Kahan sums in registers; each point is written
+//                   once (no atomics on the grid)
+//   k_group_sum     tiles cut into line groups: group partial rows added in order
 // Grid-dependent, T,P-independent steps (once per chunk and grid):
 //   k_line_pos      window [ib, ie) and nearest point c per line (FP64, bit-exact
 //                   to OpenMPManager.cpp:399-400)
-//   k_tiles         binary-searched line range per wave-tile -> work items
+//   k_tiles         binary-searched line range per tile -> work items
 // Baseline:
 //   k_fp64_points   one work-item per grid point, binary-searched line range,
 //                   FP64 W4 and FP64 sum (the "plain FP64" GPU path)
@@ -381,12 +384,13 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 }
 
 // Line records are read-only while a profile kernel runs; addressing them in
-// the constant address space lets the wave-uniform loads below issue as
-// scalar loads (s_load) into SGPRs, so per-line decisions are scalar branches.
-// (The host pass of this single-source file only parses device code.)
-// The batched loop below reads them with vector loads through global-space
-// pointers (plain pointers would be flat accesses, whose waits cannot be
-// counted per load).
+// the constant address space lets wave-uniform loads issue as scalar loads
+// (s_load) into SGPRs, so per-line decisions are scalar branches (the window
+// bounds of edge lines, the records of the generic checked loop).  The
+// staged and batched loops read records with vector loads through
+// global-space pointers (plain pointers would be flat accesses, whose waits
+// cannot be counted per load).  (The host pass of this single-source file
+// only parses device code.)
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) LinePos* ConstPos;
 typedef const __attribute__((address_space(4))) VRec* ConstRec;
