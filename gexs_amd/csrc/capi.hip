@@ -160,22 +160,18 @@ struct gexs_ctx {
     DevArr item_tab; // window and job tables of the tiles kernel
     const void* last_neval = nullptr;  // N_eval of the last pass (device)
 
-    // execute path: an FP32C pass runs whole (table upload, counter fill,
-    // k_line_max, k_line_rec, k_segments, profile kernel) on one of two lane
-    // streams with its own buffer set, the lanes alternating, so consecutive
-    // passes overlap on the device.  `stream` carries everything else; a pass
-    // first waits for the work on `stream` before it (event ctl), and
-    // join_lanes makes `stream` wait for both lanes.
-    hipStream_t lane[2] = {nullptr, nullptr};
+    // execute path: the per-line pre-pass of a pass runs on `pre` into one of
+    // two buffer sets while the profile kernel of the previous pass runs on
+    // `stream` from the other set
+    hipStream_t pre = nullptr;
     struct PassBuf {
         DevArr tab, small, rec, recx, items;
-        hipEvent_t prof_done = nullptr, joined = nullptr;
-        const double* out = nullptr;  // result grid of the last pass on this lane
+        hipEvent_t pre_done = nullptr, prof_done = nullptr;
         bool used = false;
     };
     PassBuf pb[2];
     int pbuf = 0;
-    hipEvent_t ctl = nullptr;
+    hipEvent_t geom = nullptr;  // on `stream` after positions and tiles were (re)built
     int cus = 256;  // compute units of the device
     PosKey pos_key;
     PosKey items_key;
@@ -312,25 +308,6 @@ int sort_and_commit(gexs_ctx* c, int64_t n, bool has_gamma, bool drop_zero) {
     return GEXS_OK;
 }
 
-// `stream` waits for everything enqueued on the two lanes so far: called by
-// every entry point that reads results, rewrites what a pass reads (lines,
-// positions, tiles, result grids) or synchronises
-int join_lanes(gexs_ctx* c) {
-    for (auto& b : c->pb) {
-        if (!b.used) continue;
-        const int k = (int)(&b - c->pb);
-        HIPCHK(c, hipEventRecord(b.joined, c->lane[k]));
-        HIPCHK(c, hipStreamWaitEvent(c->stream, b.joined, 0));
-    }
-    return GEXS_OK;
-}
-
-int enter(gexs_ctx* c) {
-    int rc = set_device(c);
-    if (rc != GEXS_OK) return rc;
-    return join_lanes(c);
-}
-
 // window + nearest point of every current line for one grid window:
 // lines [l0, l0 + n) of the current lines take part in the pass
 int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
@@ -344,8 +321,6 @@ int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
     key.n = n;
     key.gen = c->gen;
     if (key == c->pos_key) return GEXS_OK;
-    int rc;
-    if ((rc = join_lanes(c)) != GEXS_OK) return rc;  // passes in flight read c->pos
     HIPCHK(c, c->pos.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(LinePos)));
     JobDesc d{};
     d.l0 = l0;
@@ -359,6 +334,7 @@ int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
                             c->pos_tp.as<TPParams>(), c->pos.as<LinePos>()));
     c->pos_key = key;
     c->items_key = PosKey();  // tiles depend on positions
+    HIPCHK(c, hipEventRecord(c->geom, c->stream));
     return GEXS_OK;
 }
 
@@ -427,17 +403,13 @@ int launch_blocks(const gexs_ctx* c) {
 // job tables (host copies, uploaded when the items are rebuilt); sorted
 // heaviest first when order == 1; with order == 2 in grid order and split
 // into GX_XCDS contiguous ranges of equal line count for the per-XCD queues.
-// Rebuilds run on c->stream, after the passes in flight (they read c->items).
+// Rebuilds run on c->stream and record c->geom.
 int ensure_items(gexs_ctx* c, int njobs, const TPParams* htp, const JobDesc* hjd) {
     const int span = GX_WAVE * c->ppl;
     const PosKey key = c->pos_key;
     if (key == c->items_key && c->items_span == span && c->items_order == c->order &&
         c->items_njobs == njobs)
         return GEXS_OK;
-    {
-        int rc;
-        if ((rc = join_lanes(c)) != GEXS_OK) return rc;
-    }
     const int per = tiles_per_job(c);
     const int total = per * njobs;
     HIPCHK(c, c->items.ensure((size_t)std::max(total, 1) * sizeof(WorkItem)));
@@ -486,6 +458,7 @@ int ensure_items(gexs_ctx* c, int njobs, const TPParams* htp, const JobDesc* hjd
     c->items_order = c->order;
     c->items_njobs = njobs;
     c->n_items = total;
+    HIPCHK(c, hipEventRecord(c->geom, c->stream));
     return GEXS_OK;
 }
 
@@ -505,14 +478,11 @@ int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
 }
 
 // The FP32C pass of one execute over the lines and window of c->pos_key into
-// out.  The whole pass (table upload, counter fill, k_line_max, k_line_rec,
-// k_segments, profile kernel) runs on one lane with that lane's buffer set,
-// after the work on c->stream before it (ctl) and, when the pass before it
-// on the other lane wrote the same result grid, after that pass's profile
-// kernel (the accumulation order of chunks is kept).  Consecutive passes
-// thus overlap: the next pass's pre-pass and the start of its profile kernel
-// run while this profile kernel drains.  Every pass computes exactly what it
-// would alone.
+// out.  The per-line pre-pass (table upload, counter fill, k_line_max,
+// k_line_rec, k_segments) runs on c->pre into one of two buffer sets, after
+// the profile kernel that last used that set; the profile kernel runs on
+// c->stream after it.  The pre-pass of the next execute thus overlaps this
+// pass's profile kernel; every pass computes exactly what it would alone.
 int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     const int n = (int)c->pos_key.n;
     const int64_t l0 = c->pos_key.l0;
@@ -528,9 +498,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     d.ntiles = per;
     d.q_slot = -1;
     if ((rc = ensure_items(c, 1, &tp, &d)) != GEXS_OK) return rc;
-    const int me = c->pbuf;
-    gexs_ctx::PassBuf& B = c->pb[me];
-    const gexs_ctx::PassBuf& O = c->pb[me ^ 1];
+    gexs_ctx::PassBuf& B = c->pb[c->pbuf];
     c->pbuf ^= 1;
     PassTables t;
     if ((rc = pass_tables_in(c, B.tab, B.small, 1, &t)) != GEXS_OK) return rc;
@@ -550,9 +518,9 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     jb.profile = c->profile;
     jb.scale_slot = 0;
     jb.accumulate = 1;
-    hipStream_t P = c->lane[me];
-    HIPCHK(c, hipEventRecord(c->ctl, c->stream));
-    HIPCHK(c, hipStreamWaitEvent(P, c->ctl, 0));  // positions, tiles, zeroed grids
+    hipStream_t P = c->pre;
+    HIPCHK(c, hipStreamWaitEvent(P, c->geom, 0));  // positions and tiles
+    if (B.used) HIPCHK(c, hipStreamWaitEvent(P, B.prof_done, 0));  // buffer set free
     HIPCHK(c, hipEventRecord(c->ev_start, P));
     HIPCHK(c, c->staging.upload(B.tab.p, h.data(), t.bytes, P));
     HIPCHK(c, gxk::fill_u64(P, (unsigned long long*)B.small.p, t.zero_u64, 0ull));
@@ -569,8 +537,9 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     if (voigt)
         HIPCHK(c, gxk::segments(P, c->items.as<WorkItem>(), B.items.as<WorkItem>(), c->n_items, t.jd,
                                 c->pos.as<LinePos>(), t.kmax, c->profile));
-    if (O.used && O.out == out) HIPCHK(c, hipStreamWaitEvent(P, O.prof_done, 0));
-    hipStream_t s = P;
+    HIPCHK(c, hipEventRecord(B.pre_done, P));
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipStreamWaitEvent(s, B.pre_done, 0));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split,
                                  voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>(), c->n_items, t.jobs,
@@ -578,7 +547,6 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
                                  launch_blocks(c)));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     HIPCHK(c, hipEventRecord(B.prof_done, s));
-    B.out = out;
     B.used = true;
     c->last.n_tiles = c->n_items;
     c->last_neval = t.neval;
@@ -634,13 +602,12 @@ int gexs_create(int device, int profile, gexs_ctx** out) {
     c->profile = profile;
     bool ok = set_device(c) == GEXS_OK &&
               hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipStreamCreateWithFlags(&c->lane[0], hipStreamNonBlocking) == hipSuccess &&
-              hipStreamCreateWithFlags(&c->lane[1], hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&c->ctl, hipEventDisableTiming) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->pre, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->geom, hipEventDisableTiming) == hipSuccess &&
               next_events(c) == GEXS_OK;
     for (auto& b : c->pb)
-        ok = ok && hipEventCreateWithFlags(&b.prof_done, hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&b.joined, hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&b.pre_done, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&b.prof_done, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         gexs_destroy(c);
         return GEXS_EHIP;
@@ -673,8 +640,6 @@ int gexs_destroy(gexs_ctx* c) {
     if (!c) return GEXS_EINVAL;
     (void)hipSetDevice(c->device);
     c->server.reset();
-    for (hipStream_t l : c->lane)
-        if (l) (void)hipStreamSynchronize(l);
     (void)hipStreamSynchronize(c->stream);
     for (double* p : c->results) (void)hipFree(p);
     c->results.clear();
@@ -683,13 +648,13 @@ int gexs_destroy(gexs_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->h_tab) (void)hipHostFree(c->h_tab);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+    if (c->pre) (void)hipStreamSynchronize(c->pre);
     for (auto& b : c->pb) {
+        if (b.pre_done) (void)hipEventDestroy(b.pre_done);
         if (b.prof_done) (void)hipEventDestroy(b.prof_done);
-        if (b.joined) (void)hipEventDestroy(b.joined);
     }
-    if (c->ctl) (void)hipEventDestroy(c->ctl);
-    for (hipStream_t l : c->lane)
-        if (l) (void)hipStreamDestroy(l);
+    if (c->geom) (void)hipEventDestroy(c->geom);
+    if (c->pre) (void)hipStreamDestroy(c->pre);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GEXS_OK;
@@ -699,7 +664,7 @@ int gexs_init_vectors(gexs_ctx* c, int npoints, int num_cs, int64_t mem_budget,
                       int64_t* line_capacity) {
     if (!c || npoints <= 0 || num_cs <= 0) return GEXS_EINVAL;
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (double* p : c->results) (void)hipFree(p);
     c->results.assign((size_t)num_cs, nullptr);
@@ -741,7 +706,7 @@ int gexs_transfer_freq(gexs_ctx* c, const double* freq, int n) {
     if (!c || !freq || n <= 0) return GEXS_EINVAL;
     if (n != c->npoints) return c->fail(GEXS_ESTATE, "transfer_freq: n differs from init_vectors npoints");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     c->start = freq[0];
     c->have_grid = true;
     for (double* p : c->results) HIPCHK(c, gxk::fill_f64(c->stream, p, n, 0.0));
@@ -760,7 +725,7 @@ static int transfer_common(gexs_ctx* c, int64_t n, const double* E, const double
     if (needs_gamma && !has_gamma)
         return c->fail(GEXS_EINVAL, "Voigt profiles need gamma and n per line");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     hipStream_t s = c->stream;
     HIPCHK(c, hipStreamSynchronize(s));  // TransferVectors waits for earlier work
     size_t nb = (size_t)std::max<int64_t>(n, 1);
@@ -821,11 +786,10 @@ int gexs_execute(gexs_ctx* c, int npoints, int64_t nlines, int start_idx, double
     if ((rc = ensure_pos(c, tp, 0, c->n)) != GEXS_OK) return rc;
     if ((rc = next_events(c)) != GEXS_OK) return rc;
     if (c->k.precision == GEXS_FP64) {
-        if ((rc = join_lanes(c)) != GEXS_OK) return rc;
         HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
         rc = run_fp64(c, tp, out, true);
     } else {
-        rc = run_fp32c(c, tp, out, true);  // on a lane stream
+        rc = run_fp32c(c, tp, out, true);  // records ev_start on the pre-pass stream
     }
     c->last_neval_pending = true;
     return rc;
@@ -836,7 +800,7 @@ int gexs_transfer_results(gexs_ctx* c, double* out, int n, int cs_index, int acc
     if (cs_index < 0 || cs_index >= (int)c->results.size()) return c->fail(GEXS_EINVAL, "bad cs_index");
     if (n > c->npoints) return c->fail(GEXS_EINVAL, "n exceeds the grid");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!accumulate) {
         HIPCHK(c, hipMemcpy(out, c->results[(size_t)cs_index], (size_t)n * 8, hipMemcpyDeviceToHost));
@@ -850,15 +814,11 @@ int gexs_transfer_results(gexs_ctx* c, double* out, int n, int cs_index, int acc
 
 int gexs_ready(gexs_ctx* c) {
     if (!c) return GEXS_EINVAL;
-    for (hipStream_t l : c->lane)
-        if (hipStreamQuery(l) != hipSuccess) return 0;
     return hipStreamQuery(c->stream) == hipSuccess ? 1 : 0;
 }
 
 int gexs_sync(gexs_ctx* c) {
     if (!c) return GEXS_EINVAL;
-    int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GEXS_OK;
 }
@@ -868,8 +828,6 @@ const char* gexs_last_error(const gexs_ctx* c) { return c ? c->err.c_str() : "nu
 int gexs_last_stats(gexs_ctx* c, gexs_stats* out) {
     if (!c || !out) return GEXS_EINVAL;
     if (c->last_neval_pending) {
-        int rc;
-        if ((rc = enter(c)) != GEXS_OK) return rc;
         HIPCHK(c, hipStreamSynchronize(c->stream));
         unsigned long long ne = 0;
         if (c->last_neval) HIPCHK(c, hipMemcpy(&ne, c->last_neval, 8, hipMemcpyDeviceToHost));
@@ -886,7 +844,7 @@ int gexs_last_stats(gexs_ctx* c, gexs_stats* out) {
 int gexs_timing(gexs_ctx* c, int reset, double* kernel_ms, double* total_ms, int* count) {
     if (!c) return GEXS_EINVAL;
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double ks = 0.0, ts = 0.0;
     const int n = c->tcount;
@@ -913,7 +871,7 @@ int gexs_line_windows(gexs_ctx* c, int64_t n, double* nu_sorted, int32_t* ib, in
         return c->fail(GEXS_ESTATE, "no pass over the current lines yet (execute first)");
     if (n != c->pos_key.n) return c->fail(GEXS_EINVAL, "n differs from the number of lines of the last pass");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (n == 0) return GEXS_OK;
     if (nu_sorted)
@@ -966,7 +924,7 @@ int gexs_load_list(gexs_ctx* c, const gexs_states* st, const gexs_transitions* t
         if ((int64_t)tr->upper[i] >= st->n || (int64_t)tr->lower[i] >= st->n)
             return c->fail(GEXS_EINVAL, "transition index outside the state table");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     hipStream_t s = c->stream;
     HIPCHK(c, hipStreamSynchronize(s));
     const int64_t n = tr->n;
@@ -1019,7 +977,7 @@ int gexs_partition(gexs_ctx* c, const double* T, int nT, double* Q_out) {
     if (!c || !T || !Q_out || nT <= 0) return GEXS_EINVAL;
     if (c->n_states <= 0) return c->fail(GEXS_ESTATE, "no resident state table (gexs_load_list)");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     const int nblk = 512;
     HIPCHK(c, c->q_T.ensure((size_t)nT * 8));
     HIPCHK(c, c->q_part.ensure((size_t)nT * nblk * 8));
@@ -1082,7 +1040,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     if (!c->have_k) return c->fail(GEXS_ESTATE, "run_batch before init_constants");
     if (c->server) return c->fail(GEXS_ESTATE, "run_batch while serving");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     const double dfreq = (grid->end - grid->start) / (double)grid->npoints;  // BaseProfile.cpp:10
     int nq = 0;
     for (int i = 0; i < ntp; ++i) {
@@ -1264,7 +1222,7 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
     if (!c || !tp || ntp <= 0 || !grid || grid->npoints <= 0) return GEXS_EINVAL;
     if (c->k.precision != GEXS_FP32C) return c->fail(GEXS_ESTATE, "autotune applies to the FP32C path");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
     const int cus = std::max(1, prop.multiProcessorCount);
@@ -1324,7 +1282,7 @@ int gexs_serve_start(gexs_ctx* c, int max_inflight, int64_t mem_budget) {
     if ((c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD) && !c->has_gamma)
         return c->fail(GEXS_ESTATE, "Voigt serving needs gamma and n");
     int rc;
-    if ((rc = enter(c)) != GEXS_OK) return rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     gxs::ListView v;
     v.nu = c->nu.as<double>();
