@@ -301,6 +301,7 @@ def serve_bench(a):
         lat = np.zeros(n)
         srv = np.zeros(n)  # submit -> completion as the server saw it
         dev = np.zeros(n)  # device time of the request's batch
+        nev = np.zeros(n)  # line-point evaluations of the request
         pending = {}
         i = 0
         t0 = time.perf_counter()
@@ -316,6 +317,7 @@ def serve_bench(a):
                     lat[k] = time.perf_counter() - t0 - arr[k]
                     srv[k] = r["latency_ms"]
                     dev[k] = r["device_ms"]
+                    nev[k] = r["n_eval"]
                     m.release(rid)
             if i < n and not pending:
                 time.sleep(max(0.0, min(arr[i] - (time.perf_counter() - t0), 0.002)))
@@ -328,12 +330,20 @@ def serve_bench(a):
                        "server_p50_ms": float(np.percentile(srv, 50)),
                        "server_p99_ms": float(np.percentile(srv, 99)),
                        "device_p50_ms": float(np.percentile(dev, 50)),
-                       "mean_points": float(np.mean([round((r[3] - r[2]) / r[4]) for r in reqs]))})
+                       "device_p99_ms": float(np.percentile(dev, 99)),
+                       "mean_points": float(np.mean([round((r[3] - r[2]) / r[4]) for r in reqs])),
+                       "evals_per_s": float(nev.sum() / span),
+                       # the slowest requests: points, N_eval, latency, server and device ms
+                       "slowest": [[int(round((reqs[k][3] - reqs[k][2]) / reqs[k][4])),
+                                    float(f"{nev[k]:.3g}"), round(float(lat[k] * 1e3), 2),
+                                    round(float(srv[k]), 2), round(float(dev[k]), 2)]
+                                   for k in np.argsort(lat)[::-1][:3]]})
         all_reqs.append(reqs)
         log(f"[bench] serve {rate:6.1f} req/s offered: {n / span:7.1f} req/s, "
             f"p50 {sweeps[-1]['p50_ms']:.2f} ms, p99 {sweeps[-1]['p99_ms']:.2f} ms (server "
-            f"{sweeps[-1]['server_p50_ms']:.2f} / {sweeps[-1]['server_p99_ms']:.2f} ms, device p50 "
-            f"{sweeps[-1]['device_p50_ms']:.2f} ms)")
+            f"{sweeps[-1]['server_p50_ms']:.2f} / {sweeps[-1]['server_p99_ms']:.2f} ms, device "
+            f"{sweeps[-1]['device_p50_ms']:.2f} / {sweeps[-1]['device_p99_ms']:.2f} ms), {sweeps[-1]['evals_per_s']:.3g} evals/s; slowest "
+            f"[points, N_eval, latency, server, device ms]: {sweeps[-1]['slowest']}")
     m.serve_stop()
     m.close()
     # one request at a time on the plain FP64 path: service times, then the

@@ -84,7 +84,7 @@ class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("npoints", C.c_int32), ("sigma", C.POINTER(C.c_double)),
                 ("latency_ms", C.c_double), ("device_ms", C.c_double),
                 ("tile_points", C.c_int32), ("waves_per_tile", C.c_int32),
-                ("line_groups", C.c_int32), ("reserved", C.c_int32)]
+                ("line_groups", C.c_int32), ("reserved", C.c_int32), ("n_eval", C.c_int64)]
 
 
 _lib = None

@@ -1340,6 +1340,7 @@ static int fill_result(gexs_ctx* c, int rc, gxs::Result* r, gexs_result* out) {
     out->waves_per_tile = r->waves_per_tile;
     out->line_groups = r->line_groups;
     out->reserved = 0;
+    out->n_eval = r->n_eval;
     return GEXS_OK;
 }
 

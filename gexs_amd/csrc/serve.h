@@ -64,6 +64,7 @@ struct Result {
     std::vector<double> sigma;
     double latency_ms = 0.0, device_ms = 0.0;
     int tile_points = 0, waves_per_tile = 0, line_groups = 0;  // launch shape
+    int64_t n_eval = 0;  // line-point evaluations of the request
 };
 
 // first-fit allocator over one device allocation (offsets in bytes)
@@ -122,6 +123,7 @@ class Server {
         size_t part_cap = 0;
         void* small = nullptr;  // scale | neval | kmax | counter | Q | partition partials
         double* host_out = nullptr;  // pinned copy-out
+        unsigned long long* host_neval = nullptr;  // pinned N_eval per request (kMaxBatch)
         size_t host_cap = 0;
         void* stage = nullptr;  // pinned staging of the tables
         size_t stage_cap = 0;

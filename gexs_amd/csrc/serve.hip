@@ -21,15 +21,43 @@ double cutoff_of(const Physics& ph, double P) {
     return ph.cutoff > 0.0 ? ph.cutoff : std::min(25.0 * P, 100.0);  // OpenMPManager.cpp:386
 }
 
+// Slot buffers grow by at least half their size (to a whole MiB), so that a
+// run of slightly larger batches reallocates a few times only: hipFree waits
+// for the whole device and pinning host pages costs milliseconds, both on
+// the path of the batch being launched.
+size_t grown(size_t cap, size_t want) {
+    if (cap >= want) return cap;
+    const size_t mib = size_t(1) << 20;
+    return (std::max(want, cap + cap / 2) + mib - 1) / mib * mib;
+}
+
 hipError_t grow(void** p, size_t* cap, size_t want) {
     if (*p && *cap >= want) return hipSuccess;
+    want = grown(*cap, std::max<size_t>(want, kAlign));
     if (*p) (void)hipFree(*p);
     *p = nullptr;
-    want = std::max<size_t>(want, kAlign);
     hipError_t e = hipMalloc(p, want);
     *cap = e == hipSuccess ? want : 0;
     return e;
 }
+
+hipError_t grow_pinned(void** p, size_t* cap, size_t want) {
+    if (*p && *cap >= want) return hipSuccess;
+    want = grown(*cap, std::max<size_t>(want, kAlign));
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    hipError_t e = hipHostMalloc(p, want, hipHostMallocDefault);
+    *cap = e == hipSuccess ? want : 0;
+    return e;
+}
+
+// Capacity each slot reserves at start: tables of kMaxBatch requests,
+// 2^16 work items, 8 MiB of line-group partials and 2^21 output points
+// (16 MiB pinned), which hold the batches of typical requests without
+// growing.
+constexpr size_t kReserveItems = size_t(1) << 16;
+constexpr size_t kReservePart = size_t(8) << 20;
+constexpr size_t kReservePoints = size_t(1) << 21;
 }  // namespace
 
 // ------------------------------------------------------------------ Arena
@@ -101,14 +129,29 @@ int Server::start(std::string* err) {
         return -2;
     }
     for (int i = 0; i < max_inflight_; ++i) {
-        std::unique_ptr<Slot> s(new Slot());
+        // owned by slots_ before anything is created in it, so that the
+        // destructor releases what a failed start leaves
+        slots_.push_back(std::unique_ptr<Slot>(new Slot()));
+        Slot* s = slots_.back().get();
         s->owner = this;
         if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
             *err = "stream/event creation failed";
             return -3;
         }
-        slots_.push_back(std::move(s));
+        const size_t tab_b = kMaxBatch * (sizeof(TPParams) + sizeof(JobDesc) + sizeof(Job) + 8) + 4 * kAlign;
+        size_t host_b = 0;
+        if (grow(&s->tab, &s->tab_cap, tab_b) != hipSuccess ||
+            grow(&s->items, &s->items_cap, kReserveItems * sizeof(WorkItem)) != hipSuccess ||
+            grow(&s->part, &s->part_cap, kReservePart) != hipSuccess ||
+            grow_pinned(&s->stage, &s->stage_cap, tab_b) != hipSuccess ||
+            grow_pinned((void**)&s->host_out, &host_b, kReservePoints * 8) != hipSuccess ||
+            hipHostMalloc((void**)&s->host_neval, kMaxBatch * 8, hipHostMallocDefault) != hipSuccess) {
+            s->host_cap = host_b / 8;
+            *err = "slot buffer allocation failed";
+            return -2;
+        }
+        s->host_cap = host_b / 8;
     }
     thr_ = std::thread(&Server::loop, this);
     return 0;
@@ -128,6 +171,7 @@ Server::~Server() {
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (s->host_out) (void)hipHostFree(s->host_out);
+        if (s->host_neval) (void)hipHostFree(s->host_neval);
         if (s->stage) (void)hipHostFree(s->stage);
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -215,7 +259,9 @@ void Server::finish(Slot& s) {
     (void)hipEventElapsedTime(&ms, s.ev0, s.ev1);
     const auto now = std::chrono::steady_clock::now();
     size_t hoff = 0;
+    int k = 0;
     for (Req* q : s.reqs) {
+        q->res.n_eval = (int64_t)s.host_neval[k++];
         q->res.sigma.assign(s.host_out + hoff, s.host_out + hoff + q->npoints);
         hoff += (size_t)q->npoints;
         q->res.npoints = q->npoints;
@@ -364,21 +410,12 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             po += (size_t)q->ntiles * q->groups * GX_WAVE * q->ppl;
         }
     }
-    if (s.stage_cap < tab_b) {
-        if (s.stage) (void)hipHostFree(s.stage);
-        s.stage = nullptr;
-        s.stage_cap = 0;
-        if ((e = hipHostMalloc(&s.stage, tab_b, hipHostMallocDefault)) != hipSuccess) goto fail;
-        s.stage_cap = tab_b;
-    }
+    if ((e = grow_pinned(&s.stage, &s.stage_cap, tab_b)) != hipSuccess) goto fail;
     if (s.host_cap < host_total) {
-        if (s.host_out) (void)hipHostFree(s.host_out);
-        s.host_out = nullptr;
-        s.host_cap = 0;
-        if ((e = hipHostMalloc((void**)&s.host_out, std::max<size_t>(host_total, 1) * 8,
-                               hipHostMallocDefault)) != hipSuccess)
-            goto fail;
-        s.host_cap = host_total;
+        size_t host_b = s.host_cap * 8;
+        e = grow_pinned((void**)&s.host_out, &host_b, host_total * 8);
+        s.host_cap = host_b / 8;
+        if (e != hipSuccess) goto fail;
     }
     {
         // the slot's previous batch has completed (the slot was free), so its
@@ -457,6 +494,9 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
                 goto fail;
             hoff += np;
         }
+        if ((e = hipMemcpyAsync(s.host_neval, d_neval, (size_t)nreq * 8, hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+            goto fail;
         if ((e = hipEventRecord(s.ev1, st)) != hipSuccess) goto fail;
     }
     s.reqs = batch;

@@ -243,7 +243,7 @@ class HipManager:
             return dict(status=r.status)
         sigma = np.ctypeslib.as_array(r.sigma, shape=(r.npoints,)).copy()
         return dict(status=0, sigma=sigma, latency_ms=r.latency_ms, device_ms=r.device_ms,
-                    shape=(r.tile_points, r.waves_per_tile, r.line_groups))
+                    shape=(r.tile_points, r.waves_per_tile, r.line_groups), n_eval=r.n_eval)
 
     def poll(self, rid):
         r = _lib.Result()

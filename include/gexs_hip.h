@@ -248,6 +248,7 @@ typedef struct gexs_result {
     /* the launch shape the server chose for the request (from its own size):
      * tile width in points, waves per tile, workgroups (line groups) per tile */
     int32_t tile_points, waves_per_tile, line_groups, reserved;
+    int64_t n_eval;      /* line-point evaluations of the request (N_eval) */
 } gexs_result;
 
 int gexs_serve_start(gexs_ctx* ctx, int max_inflight, int64_t mem_budget);

@@ -2,7 +2,8 @@
 
 A request's result must be the same alone and in a mix (bit for bit: every
 request owns its records, tiles and output, and no reduction crosses
-requests), must match the oracle (peak-relative 1e-4, band strength 1e-6),
+requests), must match the oracle (peak-relative 1e-4, band strength 1e-6,
+N_eval exact),
 and requests larger than the memory budget are refused at submit.
 """
 import numpy as np
@@ -57,8 +58,9 @@ def test_served_vs_oracle(server, O):
         keep = (nu >= a - 100.0) & (nu <= b + 100.0) & (nu != 0)
         Q = O.partition(L.E, L.g, T)
         p = O.params("voigt", T=T, P=P, Q=Q, mass=28.0, cutoff=25.0, p_scale=P_SCALE)
-        ref, _ = O.cross_section(p, a, (b - a) / N, N, nu[keep], E[keep], A[keep], g[keep],
-                                 gam[keep], nex[keep], nthreads=8)
+        ref, ne = O.cross_section(p, a, (b - a) / N, N, nu[keep], E[keep], A[keep], g[keep],
+                                  gam[keep], nex[keep], nthreads=8)
+        assert res["n_eval"] == ne
         if ref.max() == 0:
             assert np.all(res["sigma"] == 0)
             continue
