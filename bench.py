@@ -429,8 +429,8 @@ def valu_probe():
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "the cross-section path runs on one GPU here; --gpus must be "
+                                   "1 (sharding by wavenumber range over GPUs is not built)"}))
         sys.exit(2)
     if a.calib_copy:
         calib_copy(a.calib_copy)
