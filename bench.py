@@ -180,7 +180,7 @@ def table_bench(a):
     n_eval = st["n_eval"]
     value = n_eval * a.table_steps / el
     log(f"[bench] table: {len(tps)} x-sections in {el * 1e3 / a.table_steps:.1f} ms per batch, "
-        f"{n_eval:.4e} evals, {value:.4e} evals/s, kernel {k_ms / cnt:.1f} ms")
+        f"{n_eval:.4e} evals, {value:.4e} evals/s, device {k_ms / cnt:.1f} ms per batch")
     m.close()
     m64 = HipManager("voigt", precision="fp64")
     m64.InitializeConstants(100.0, 1000.0, 1.0, dnu, spec["mass"], 1.0, synth.T_REF,
@@ -199,7 +199,7 @@ def table_bench(a):
                         f"{N} points, cutoff {spec['cutoff']} cm-1",
             "value": value, "unit": "evals/s", "xsec_per_s": len(tps) * a.table_steps / el,
             "ms_per_batch": el * 1e3 / a.table_steps, "n_eval_per_batch": n_eval,
-            "kernel_ms_per_batch": k_ms / cnt, "graph": True, "tuning": best,
+            "device_ms_per_batch": k_ms / cnt, "graph": True, "tuning": best,
             "fp64_gpu_path": {"value": v64, "unit": "evals/s", "sample": "6 of the 120 (T,P)",
                               "speedup_fp32c_over_fp64": value / v64}}
 
@@ -246,12 +246,12 @@ def stream_bench(a):
     v64 = m64.stats()["n_eval"] / e64
     m64.close()
     log(f"[bench] stream: {st['n_eval']:.4e} evals in {el * 1e3 / steps:.1f} ms per pass, "
-        f"{value:.4e} evals/s (kernel {k_ms / cnt:.1f} ms); FP64 path {v64:.4e} evals/s")
+        f"{value:.4e} evals/s (device {k_ms / cnt:.1f} ms per pass); FP64 path {v64:.4e} evals/s")
     return {"workload": f"{name}: {lst.n_trans} transitions / {lst.n_states} states, T={T} K, "
                         f"P={P} bar, grid {start}-{end} cm-1 x {N} points, cutoff "
                         f"{spec['cutoff']} cm-1",
             "value": value, "unit": "evals/s", "ms_per_pass": el * 1e3 / steps,
-            "kernel_ms": k_ms / cnt, "n_eval": st["n_eval"], "n_lines": st["n_lines"],
+            "device_ms": k_ms / cnt, "n_eval": st["n_eval"], "n_lines": st["n_lines"],
             "tuning": best,
             "fp64_gpu_path": {"value": v64, "unit": "evals/s", "speedup_fp32c_over_fp64": value / v64}}
 

@@ -128,7 +128,9 @@ class HipManager:
                     kernel_ms=s.kernel_ms, total_ms=s.total_ms)
 
     def timing(self, reset=False):
-        """(sum of profile-kernel ms, sum of whole-pass ms, count) since the last reset."""
+        """(sum of kernel ms, sum of total ms, count) since the last reset: per
+        execute its profile kernel and whole pass, per run_batch the batch's
+        device work and the same from before its table upload."""
         k, t, n = C.c_double(), C.c_double(), C.c_int()
         self._chk(self._L.gexs_timing(self._h, 1 if reset else 0, C.byref(k), C.byref(t),
                                       C.byref(n)))

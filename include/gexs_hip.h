@@ -115,9 +115,11 @@ int gexs_ready(gexs_ctx* ctx);
 int gexs_sync(gexs_ctx* ctx);
 const char* gexs_last_error(const gexs_ctx* ctx);
 
-/* Counters of the last gexs_execute: N_eval (sum over lines with S != 0 of
- * the window length), lines, and the device time of its profile kernel and
- * of the whole execute in ms (HIP events on the ctx stream). */
+/* Counters of the last gexs_execute or gexs_run_batch: N_eval (sum over
+ * lines with S != 0 of the window length), lines, tiles and device times in
+ * ms (HIP events on the ctx stream).  kernel_ms: an execute's profile kernel;
+ * a batch's device work from the table upload through the profile kernel
+ * (one graph launch when graphs are on).  total_ms adds what ran before it. */
 typedef struct gexs_stats {
     int64_t n_eval;
     int64_t n_lines;
@@ -127,8 +129,8 @@ typedef struct gexs_stats {
 } gexs_stats;
 int gexs_last_stats(gexs_ctx* ctx, gexs_stats* out);
 
-/* Sums of the HIP-event durations (ms) of the profile kernel and of the whole
- * pass over every execute / run_batch since the last reset, and their count
+/* Sums of the kernel_ms and total_ms of gexs_last_stats over every execute /
+ * run_batch since the last reset, and their count
  * (at most 2048 are kept).  Waits for the ctx stream.  reset != 0 clears. */
 int gexs_timing(gexs_ctx* ctx, int reset, double* kernel_ms, double* total_ms, int* count);
 
@@ -211,8 +213,9 @@ int gexs_set_launch(gexs_ctx* ctx, int persistent_blocks, int use_graph);
 /* Autotuner of the FP32C profile kernel: runs the batch once per candidate
  * (points per lane 1, 2, 4, 8 = tile widths of 64..512 points; 1, 4 or 16
  * waves per tile; grid, heaviest-first or per-XCD tile order; one launch slot
- * per tile or 4 / 8 persistent workgroups per CU), times the profile kernel with
- * HIP events, and keeps the fastest on the ctx.  report (may be NULL)
+ * per tile or 4 / 8 persistent workgroups per CU), times the batch's device work
+ * with HIP events (the pre-pass in it is the same for every candidate), and
+ * keeps the fastest on the ctx.  report (may be NULL)
  * receives up to report_cap candidates; *n_report their count. */
 typedef struct gexs_tuning {
     int32_t points_per_lane;
