@@ -293,14 +293,16 @@ void Server::request_shape(Req& q) const {
 }
 
 // Line groups: a request whose tiles give the device fewer than kFillWaves
-// waves also cuts every tile's lines into groups, one workgroup each (their
-// partial sums added in group order by k_group_sum), as long as a wave keeps
-// kMinLinesPerWave lines.  Lines per tile are estimated from the request's
-// selected lines, its width and the cutoff window.
+// waves (16 per SIMD) also cuts every tile's lines into groups, one
+// workgroup each (their partial sums added in group order by k_group_sum),
+// as long as a wave keeps kMinLinesPerWave lines.  A few hundred 16-wave
+// workgroups land one or two per CU, unevenly; groups even that out.  Lines
+// per tile are estimated from the request's selected lines, its width and
+// the cutoff window.
 void Server::request_groups(Req& q) const {
-    constexpr int64_t kFillWaves = 4096;
+    constexpr int64_t kFillWaves = 16384;
     constexpr int kMaxGroups = 64;
-    constexpr double kMinLinesPerWave = 256.0;
+    constexpr double kMinLinesPerWave = 128.0;
     q.groups = 1;
     const int64_t waves = (int64_t)q.ntiles * q.split;
     if (waves >= kFillWaves || q.n <= 0) return;
