@@ -1013,11 +1013,13 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, const PassTables& t) {
         const int nblk = 256;
         HIPCHK(c, hipMemcpyAsync(c->q_T.p, (char*)c->h_tab + t.bytes, (size_t)nq * 8,
                                  hipMemcpyHostToDevice, s));
+        // Q(T) into the jobs' TPParams and the pass counters zeroed, one launch
         HIPCHK(c, gxk::partition(s, c->st_E.as<double>(), c->st_g.as<int32_t>(), c->n_states, c->q_T.as<double>(),
-                                 nq, c->q_part.as<double>(), nblk, c->q_Q.as<double>()));
-        HIPCHK(c, gxk::take_q(s, t.jd, njobs, c->q_Q.as<double>(), t.tp));
+                                 nq, c->q_part.as<double>(), nblk, c->q_Q.as<double>(), t.jd, njobs, t.tp,
+                                 (unsigned long long*)c->small.p, (int)t.zero_u64));
+    } else {
+        HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)c->small.p, t.zero_u64, 0ull));
     }
-    HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)c->small.p, t.zero_u64, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
     HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),

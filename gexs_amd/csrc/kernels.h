@@ -14,6 +14,10 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
 hipError_t line_max(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const LinePos* pos, const JobDesc* jd, int njobs, int64_t max_n,
                     const TPParams* tp, unsigned long long* scale_bits);
+// line_pos and line_max in one launch (positions made for this pass alone)
+hipError_t line_pos_max(hipStream_t s, const double* nu, const double* E, const double* A,
+                        const int* g, const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
+                        LinePos* pos, unsigned long long* scale_bits);
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
@@ -27,7 +31,6 @@ hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
                  uint32_t* item_idx, int job0 = 0);
-hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, TPParams* tp);
 // items of tiles cut into line groups (WorkItem::ngroups > 1): every point of
 // such a tile = the sum of its groups' rows of Job::part (span = 64 * PPL)
 hipError_t group_sum(hipStream_t s, const WorkItem* items, int nitems, const Job* jobs, int span);
@@ -48,8 +51,12 @@ hipError_t line_f64(hipStream_t s, const double* nu, const double* E, const doub
 hipError_t fp64_points(hipStream_t s, int profile, const double* nu, const LinePos* pos,
                        const LineF64* lf, int n, double start, double dfreq, int win_lo,
                        int win_hi, double* out);
+// Q(T) of nT temperatures (dQ); with jd, also tp[j].Q for the jobs whose
+// q_slot names a temperature, and clear[0 .. nclear) zeroed, in the same launch
 hipError_t partition(hipStream_t s, const double* E, const int* g, int64_t n, const double* dT,
-                     int nT, double* partial, int nblk, double* dQ);
+                     int nT, double* partial, int nblk, double* dQ, const JobDesc* jd = nullptr,
+                     int njobs = 0, TPParams* tp = nullptr, unsigned long long* clear = nullptr,
+                     int nclear = 0);
 hipError_t fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v);
 hipError_t fill_f64(hipStream_t s, double* p, int64_t n, double v);
 hipError_t add_f64(hipStream_t s, double* dst, const double* src, int64_t n);

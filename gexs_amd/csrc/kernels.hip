@@ -110,35 +110,40 @@ __device__ __forceinline__ int block_max_i(int v, int* sh) {
 }  // namespace
 
 // ------------------------------------------------------------- k_line_pos
+namespace {
+// window [ib, ie) and nearest grid index c of a line at x
+__device__ __forceinline__ LinePos window_of(const TPParams& p, double x) {
+#pragma clang fp contract(off)
+    const double v = x == 0.0 ? 1e-6 : x;
+    // OpenMPManager.cpp:399-400: round(((nu - cut) - start)/dnu), clamped
+    double b = round((v - p.cutoff - p.start) / p.dfreq);
+    double e = round((v + p.cutoff - p.start) / p.dfreq);
+    const double lo = (double)p.win_lo, hi = (double)p.win_hi;
+    double bb = (b < lo) ? lo : b;
+    double ee = (hi < e) ? hi : e;
+    // keep the integers in range; far-away lines have empty windows anyway
+    bb = fmin(bb, 2.0e9);
+    ee = fmax(ee, -2.0e9);
+    double c = round((v - p.start) / p.dfreq);
+    c = fmin(fmax(c, -1.0e9), 1.0e9);
+    LinePos lp;
+    lp.ib = (int)bb;
+    lp.ie = (int)ee;
+    lp.c = (int)c;
+    lp.cf = (float)c;
+    return lp;
+}
+}  // namespace
+
 // blockIdx.y = job; each job has its own lines, grid and window
 __global__ __launch_bounds__(256) void k_line_pos(const double* __restrict__ nu,
                                                   const JobDesc* __restrict__ jd,
                                                   const TPParams* __restrict__ tp,
                                                   LinePos* __restrict__ pos) {
-#pragma clang fp contract(off)
     const JobDesc d = jd[blockIdx.y];
     const TPParams p = tp[blockIdx.y];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += gridDim.x * blockDim.x) {
-        const double x = nu[d.l0 + i];
-        const double v = x == 0.0 ? 1e-6 : x;
-        // OpenMPManager.cpp:399-400: round(((nu - cut) - start)/dnu), clamped
-        double b = round((v - p.cutoff - p.start) / p.dfreq);
-        double e = round((v + p.cutoff - p.start) / p.dfreq);
-        const double lo = (double)p.win_lo, hi = (double)p.win_hi;
-        double bb = (b < lo) ? lo : b;
-        double ee = (hi < e) ? hi : e;
-        // keep the integers in range; far-away lines have empty windows anyway
-        bb = fmin(bb, 2.0e9);
-        ee = fmax(ee, -2.0e9);
-        double c = round((v - p.start) / p.dfreq);
-        c = fmin(fmax(c, -1.0e9), 1.0e9);
-        LinePos lp;
-        lp.ib = (int)bb;
-        lp.ie = (int)ee;
-        lp.c = (int)c;
-        lp.cf = (float)c;
-        pos[d.pos_off + i] = lp;
-    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += gridDim.x * blockDim.x)
+        pos[d.pos_off + i] = window_of(p, nu[d.l0 + i]);
 }
 
 // ------------------------------------------------------------- k_line_max
@@ -158,6 +163,34 @@ __global__ __launch_bounds__(256) void k_line_max(const double* __restrict__ nu,
         const LinePos lp = pos[d.pos_off + i];
         if (lp.ie <= lp.ib) continue;  // only lines that reach the grid set the scale
         const int64_t l = d.l0 + i;
+        double S = line_strength(p, nu[l], E[l], A[l], g[l]);
+        double amp = line_amp(p, S, inv_width(p, nu[l]));
+        if (isfinite(amp)) m = fmax(m, amp);
+    }
+    __shared__ double sh[4];
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) atomic_max_pos(&scale_bits[k], m);
+}
+
+// k_line_pos and k_line_max in one pass (jobs whose positions are computed
+// for this pass alone, as a served batch's)
+__global__ __launch_bounds__(256) void k_line_pos_max(const double* __restrict__ nu,
+                                                      const double* __restrict__ E,
+                                                      const double* __restrict__ A,
+                                                      const int* __restrict__ g,
+                                                      const JobDesc* __restrict__ jd,
+                                                      const TPParams* __restrict__ tp,
+                                                      LinePos* __restrict__ pos,
+                                                      unsigned long long* __restrict__ scale_bits) {
+    const int k = blockIdx.y;
+    const TPParams p = tp[k];
+    const JobDesc d = jd[k];
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += gridDim.x * blockDim.x) {
+        const int64_t l = d.l0 + i;
+        const LinePos lp = window_of(p, nu[l]);
+        pos[d.pos_off + i] = lp;
+        if (lp.ie <= lp.ib) continue;
         double S = line_strength(p, nu[l], E[l], A[l], g[l]);
         double amp = line_amp(p, S, inv_width(p, nu[l]));
         if (isfinite(amp)) m = fmax(m, amp);
@@ -357,13 +390,6 @@ __global__ __launch_bounds__(256) void k_segments(const WorkItem* in, WorkItem* 
     it.s2 = s2;
     it.s3 = s3;
     out[t] = it;
-}
-
-// tp[k].Q = Q[q_slot] for jobs whose Q comes from a device partition pass
-__global__ void k_take_q(const JobDesc* __restrict__ jd, int njobs, const double* __restrict__ Q,
-                         TPParams* __restrict__ tp) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < njobs && jd[k].q_slot >= 0) tp[k].Q = Q[jd[k].q_slot];
 }
 
 __global__ __launch_bounds__(256) void k_permute_items(const WorkItem* __restrict__ in,
@@ -1274,8 +1300,15 @@ __global__ __launch_bounds__(256) void k_partition(const double* __restrict__ E,
     if (threadIdx.x == 0) partial[(int64_t)k * nblk + blockIdx.x] = red[0];
 }
 
+// Q[k] = sum of the partials of temperature k; with jd, also tp[j].Q = Q[k]
+// for every job j whose q_slot is k, and block 0 zeroes clear[0 .. nclear)
+// (the pass counters, so that one launch does both)
 __global__ __launch_bounds__(256) void k_partition_final(const double* __restrict__ partial,
-                                                         int nblk, double* __restrict__ Q) {
+                                                         int nblk, double* __restrict__ Q,
+                                                         const JobDesc* __restrict__ jd, int njobs,
+                                                         TPParams* __restrict__ tp,
+                                                         unsigned long long* __restrict__ clear,
+                                                         int nclear) {
     __shared__ double red[256];
     const int k = blockIdx.x;
     double s = 0.0;
@@ -1287,6 +1320,11 @@ __global__ __launch_bounds__(256) void k_partition_final(const double* __restric
         __syncthreads();
     }
     if (threadIdx.x == 0) Q[k] = red[0];
+    if (jd != nullptr)
+        for (int j = threadIdx.x; j < njobs; j += 256)
+            if (jd[j].q_slot == k) tp[j].Q = red[0];
+    if (k == 0)
+        for (int i = threadIdx.x; i < nclear; i += 256) clear[i] = 0ull;
 }
 
 // ------------------------------------------------------------- small kernels
@@ -1328,6 +1366,16 @@ hipError_t line_max(hipStream_t s, const double* nu, const double* E, const doub
     return hipGetLastError();
 }
 
+hipError_t line_pos_max(hipStream_t s, const double* nu, const double* E, const double* A,
+                        const int* g, const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
+                        LinePos* pos, unsigned long long* scale_bits) {
+    if (njobs <= 0 || max_n <= 0) return hipSuccess;
+    int gx = std::min(blocks_for(max_n, 256), 1024);
+    hipLaunchKernelGGL(k_line_pos_max, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, jd, tp, pos,
+                       scale_bits);
+    return hipGetLastError();
+}
+
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
@@ -1361,12 +1409,6 @@ hipError_t group_sum(hipStream_t s, const WorkItem* items, int nitems, const Job
     if (nitems <= 0) return hipSuccess;
     if (span <= 0 || span > 512) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_group_sum, dim3(nitems), dim3(span), 0, s, items, nitems, jobs, span);
-    return hipGetLastError();
-}
-
-hipError_t take_q(hipStream_t s, const JobDesc* jd, int njobs, const double* Q, TPParams* tp) {
-    if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_take_q, dim3(blocks_for(njobs, 256)), dim3(256), 0, s, jd, njobs, Q, tp);
     return hipGetLastError();
 }
 
@@ -1467,13 +1509,15 @@ hipError_t fp64_points(hipStream_t s, int profile, const double* nu, const LineP
 }
 
 hipError_t partition(hipStream_t s, const double* E, const int* g, int64_t n, const double* dT,
-                     int nT, double* partial, int nblk, double* dQ) {
+                     int nT, double* partial, int nblk, double* dQ, const JobDesc* jd, int njobs,
+                     TPParams* tp, unsigned long long* clear, int nclear) {
     if (nT <= 0) return hipSuccess;
     int64_t chunk = (n + nblk - 1) / nblk;
     if (chunk < 1) chunk = 1;
     hipLaunchKernelGGL(k_partition, dim3(nblk, nT), dim3(256), 0, s, E, g, n, chunk, dT, partial,
                        nblk);
-    hipLaunchKernelGGL(k_partition_final, dim3(nT), dim3(256), 0, s, partial, nblk, dQ);
+    hipLaunchKernelGGL(k_partition_final, dim3(nT), dim3(256), 0, s, partial, nblk, dQ, jd, njobs, tp,
+                       clear, nclear);
     return hipGetLastError();
 }
 

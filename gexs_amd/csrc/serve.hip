@@ -442,14 +442,12 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         WorkItem* d_items = (WorkItem*)s.items;
         if ((e = hipEventRecord(s.ev0, st)) != hipSuccess) goto fail;
         if ((e = hipMemcpyAsync(tab, s.stage, tab_b, hipMemcpyHostToDevice, st)) != hipSuccess) goto fail;
-        if ((e = gxk::partition(st, v_.sE, v_.sg, v_.n_states, d_T, nreq, d_part, kQBlocks, d_Q)) !=
-            hipSuccess)
+        // Q(T) into every request's TPParams, and scale | neval | kmax zeroed
+        if ((e = gxk::partition(st, v_.sE, v_.sg, v_.n_states, d_T, nreq, d_part, kQBlocks, d_Q, d_jd, nreq,
+                                d_tp, d_scale, 3 * kMaxBatch)) != hipSuccess)
             goto fail;
-        if ((e = gxk::take_q(st, d_jd, nreq, d_Q, d_tp)) != hipSuccess) goto fail;
-        if ((e = gxk::fill_u64(st, d_scale, 3 * kMaxBatch, 0ull)) != hipSuccess) goto fail;  // + neval, kmax
-        if ((e = gxk::line_pos(st, v_.nu, d_jd, nreq, max_n, d_tp, (LinePos*)base)) != hipSuccess) goto fail;
-        if ((e = gxk::line_max(st, v_.nu, v_.E, v_.A, v_.g, (const LinePos*)base, d_jd, nreq, max_n, d_tp,
-                               d_scale)) != hipSuccess)
+        if ((e = gxk::line_pos_max(st, v_.nu, v_.E, v_.A, v_.g, d_jd, nreq, max_n, d_tp, (LinePos*)base,
+                                   d_scale)) != hipSuccess)
             goto fail;
         if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, (const LinePos*)base, d_jd,
                                nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval)) !=
