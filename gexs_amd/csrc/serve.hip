@@ -275,10 +275,12 @@ void Server::finish(Slot& s) {
 }
 
 // Launch shape of a request: the widest tile up to the configured one that
-// still gives every SIMD of the device a wave (8 K waves), and 16 waves per
-// tile when even 64-point tiles give fewer; small requests thus spread their
-// lines over more waves.  The shape depends on the request alone, so a
-// result is the same in any batch.
+// still gives every SIMD of the device a wave (8 K waves), with the
+// configured waves per tile; smaller requests spread their lines over more
+// workgroups by line groups (request_groups), which measured faster than 16
+// waves per tile (one workgroup holding a tile's near-centre lines waits on
+// its slowest wave).  The shape depends on the request alone, so a result
+// is the same in any batch.
 void Server::request_shape(Req& q) const {
     const int64_t want_waves = 8 * 1024;
     const int split = std::max(ph_.split, 1);
@@ -289,7 +291,7 @@ void Server::request_shape(Req& q) const {
             break;
         }
     q.ntiles = (q.npoints + GX_WAVE * q.ppl - 1) / (GX_WAVE * q.ppl);
-    q.split = (int64_t)q.ntiles * split >= want_waves ? split : 16;
+    q.split = split;
 }
 
 // Line groups: a request whose tiles give the device fewer than kFillWaves
@@ -301,7 +303,7 @@ void Server::request_shape(Req& q) const {
 // the cutoff window.
 void Server::request_groups(Req& q) const {
     constexpr int64_t kFillWaves = 16384;
-    constexpr int kMaxGroups = 64;
+    constexpr int kMaxGroups = 256;
     constexpr double kMinLinesPerWave = 128.0;
     q.groups = 1;
     const int64_t waves = (int64_t)q.ntiles * q.split;
