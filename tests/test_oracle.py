@@ -125,6 +125,36 @@ def test_region1_s_form_equals_humlicek_region1(O):
         assert np.max(np.abs(s_form(x, y, e2, hh) - h) / h) > 1e-6
 
 
+def test_region2_gauss_hermite_pairs_equal_humlicek_region2(O):
+    """The profile kernel evaluates Humlicek region II (5.5 <= |x| + y < 15) as
+    the 4-point Gauss-Hermite sum in +-a pairs,
+    K = sum_k (2 w_k / pi) y B_k / (B_k^2 - 4 a_k^2 x^2), B_k = x^2 + y^2 + a_k^2.
+    With nodes and weights from numpy's hermgauss it equals the oracle's W4 to
+    the 7 published digits of Humlicek's coefficients, also in FP32 (the
+    kernel's precision) within 1e-6; swapped weights do not."""
+    t, w = np.polynomial.hermite.hermgauss(4)
+    a2 = np.sort(t[t > 0] ** 2)
+    wk = w[t > 0][np.argsort(t[t > 0])]
+    rng = np.random.default_rng(16)
+    y = rng.uniform(0, 15, 6000)
+    x = rng.uniform(np.maximum(5.5 - y, 0.0), 15.0 - y)
+    keep = (x + y >= 5.5) & (x + y < 15.0) & (y > 1e-3)
+    x, y = x[keep], y[keep]
+    h = np.array([O.humlic(a, b) for a, b in zip(x, y)])
+
+    def pairs(x, y, a2, c, dt):
+        x, y, a2, c = dt(x), dt(y), a2.astype(dt), c.astype(dt)
+        x2 = x * x
+        b = (x2 + y * y)[:, None] + a2
+        d = b * b - dt(4) * a2 * x2[:, None]
+        return y * np.sum(c * b / d, axis=1)
+
+    c = 2 * wk / np.pi
+    assert np.max(np.abs(pairs(x, y, a2, c, np.float64) - h) / h) < 1e-7
+    assert np.max(np.abs(pairs(x, y, a2, c, np.float32) - h) / h) < 1e-6
+    assert np.max(np.abs(pairs(x, y, a2, c[::-1], np.float64) - h) / h) > 1e-3
+
+
 def test_humlic_matches_reference_bit_for_bit(O, R):
     rng = np.random.default_rng(6)
     for x, y in zip(rng.uniform(0, 20, 5000), 10 ** rng.uniform(-5, 2, 5000)):
