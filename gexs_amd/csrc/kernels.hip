@@ -947,21 +947,24 @@ __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRe
 
 // Kahan sums of the lines [ca, cb) of the item (a sub-range of [lo, hi)) at
 // the 64*PPL points of the tile; lines stay in nu order across the segments
+// (init false: add to the sums already in t)
 template <int PROF, int PPL>
 __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, const Job& jb,
-                                          int lane, int ca, int cb, float* xs) {
+                                          int lane, int ca, int cb, float* xs, bool init = true) {
     const ConstPos pos = (ConstPos)jb.pos;
     const ConstRec rec = (ConstRec)jb.rec;
     const ConstRecX recx = (ConstRecX)jb.recx;
+    if (init) {
 #pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-        t.sum[q] = 0.0f;
-        t.cmp[q] = 0.0f;
-        t.part[q] = 0.0f;
-        t.jj[q] = it.w0 + lane + GX_WAVE * q;
-        t.jf[q] = (float)t.jj[q];
+        for (int q = 0; q < PPL; ++q) {
+            t.sum[q] = 0.0f;
+            t.cmp[q] = 0.0f;
+            t.part[q] = 0.0f;
+            t.jj[q] = it.w0 + lane + GX_WAVE * q;
+            t.jf[q] = (float)t.jj[q];
+        }
+        t.n = 0;
     }
-    t.n = 0;
     const int w0 = it.w0, w1 = it.w1;
     if constexpr (PPL == 1) {
         // region-I segments staged, the others by vector loads broadcast
@@ -1031,11 +1034,33 @@ __device__ __forceinline__ void store_point(const Job& jb, int j, double v) {
         jb.out[j] = v;
 }
 
-// the item's lines: its group's contiguous share of the tile's [lo, hi)
-__device__ __forceinline__ void group_lines(const WorkItem& it, int* a, int* b) {
-    const int n = max(it.hi - it.lo, 0), G = max(it.ngroups, 1);
-    *a = it.lo + (int)(((long long)n * it.group) / G);
-    *b = it.lo + (int)(((long long)n * (it.group + 1)) / G);
+// line k * n / K of a range of n lines starting at lo
+__device__ __forceinline__ int share_cut(int lo, int n, int k, int K) {
+    return lo + (int)(((long long)n * k) / K);
+}
+
+// Kahan sums of share k of K of the tile's lines [lo, hi) (K = line groups x
+// waves per group).  The near-centre lines [s2, s3) evaluate W4 over part of
+// the tile and cost up to ~20x a region-I line, and they are contiguous in nu
+// order, so one equal cut of [lo, hi) gives them all to one share.  When a
+// Voigt tile has at least K of them, each of [lo, s2), [s2, s3) and [s3, hi)
+// is cut into K equal contiguous shares instead and share k sums its three
+// pieces in nu order: every share carries 1/K of each segment's work.
+template <int PROF, int PPL>
+__device__ __forceinline__ void sum_share(TileAcc<PPL>& t, const WorkItem& it, const Job& jb,
+                                          int lane, int k, int K, float* xs) {
+    const int lo = it.lo, hi = max(it.hi, it.lo);
+    const int m2 = min(max(it.s2, lo), hi), m3 = min(max(it.s3, lo), hi);
+    const bool three = PROF == GX_PROF_VOIGT && K > 1 && it.s1 < it.s4 && m3 - m2 >= K;
+    const int np = three ? 3 : 1;
+    // one loop, so the kernel holds one copy of sum_lines
+#pragma nounroll
+    for (int p = 0; p < np; ++p) {
+        const int a = !three ? lo : p == 0 ? lo : p == 1 ? m2 : m3;
+        const int b = !three ? hi : p == 0 ? m2 : p == 1 ? m3 : hi;
+        sum_lines<PROF, PPL>(t, it, jb, lane, share_cut(a, b - a, k, K),
+                             share_cut(a, b - a, k + 1, K), xs, p == 0);
+    }
 }
 
 // value k of the tile (point w0 + k): to the grid, or with line groups to
@@ -1053,9 +1078,7 @@ template <int PROF, int PPL>
 __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
                                          int lane, float* xs) {
     TileAcc<PPL> t;
-    int a, b;
-    group_lines(it, &a, &b);
-    sum_lines<PROF, PPL>(t, it, jb, lane, a, b, xs);
+    sum_share<PROF, PPL>(t, it, jb, lane, it.group, max(it.ngroups, 1), xs);
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
         store_tile<PPL>(it, jb, GX_WAVE * q + lane,
@@ -1068,19 +1091,14 @@ constexpr int kWavesPerBlock = GX_WG / GX_WAVE;  // SPLIT 1: items per workgroup
 template <int SPLIT>
 constexpr int block_threads() { return SPLIT > 4 ? SPLIT * GX_WAVE : GX_WG; }
 
-// the SPLIT waves of the workgroup take contiguous shares of the item's
-// lines; their FP64 partial sums are added in wave order through LDS
+// the SPLIT waves of the workgroup take the item's shares group * SPLIT + wv
+// of its lines; their FP64 partial sums are added in wave order through LDS
 template <int PROF, int PPL, int SPLIT>
 __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb,
                                                double inv_scale, int lane, int wv,
                                                double* red, float* xs) {
     TileAcc<PPL> t;
-    int a, b;
-    group_lines(it, &a, &b);
-    const int n = b - a;
-    const int ca = a + (int)(((long long)n * wv) / SPLIT);
-    const int cb = a + (int)(((long long)n * (wv + 1)) / SPLIT);
-    sum_lines<PROF, PPL>(t, it, jb, lane, ca, cb, xs);
+    sum_share<PROF, PPL>(t, it, jb, lane, it.group * SPLIT + wv, max(it.ngroups, 1) * SPLIT, xs);
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
         red[(size_t)wv * GX_WAVE * PPL + GX_WAVE * q + lane] = (double)t.sum[q] - (double)t.cmp[q];
