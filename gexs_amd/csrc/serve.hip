@@ -298,10 +298,10 @@ void Server::request_shape(Req& q) const {
 // waves (16 per SIMD) also cuts every tile's lines into groups, one
 // workgroup each (their partial sums added in group order by k_group_sum),
 // as long as a wave keeps kMinLinesPerWave lines.  A few hundred
-// workgroups land one or two per CU, unevenly, and the one holding a tile's
-// near-centre lines (W4) runs longest; more, smaller workgroups even that
-// out.  Lines per tile are estimated from the request's selected lines, its
-// width and the cutoff window.
+// workgroups land one or two per CU, unevenly; more, smaller workgroups
+// even that out (each group takes an equal share of a tile's near-centre
+// lines too).  Lines per tile are estimated from the request's selected
+// lines, its width and the cutoff window.
 void Server::request_groups(Req& q) const {
     constexpr int64_t kFillWaves = 16384;
     constexpr int kMaxGroups = 256;
