@@ -79,16 +79,24 @@ def test_launch_modes_agree_bitwise(batch_mgr, split):
         assert np.array_equal(o, outs[0]), k
 
 
-def test_split_close_to_single_wave(batch_mgr):
+@pytest.mark.parametrize("ppl", [1, 4])
+@pytest.mark.parametrize("split", [4, 16])
+def test_split_close_to_single_wave(batch_mgr, ppl, split):
+    """Waves of a split tile take equal shares of the lines before, inside
+    and after the tile's near-centre lines (three pieces each when the tile
+    has at least `split` near lines, as most tiles of this table do): every
+    line is summed once, so only the grouping of the FP32 partial sums
+    differs from one wave per tile."""
     L, m = batch_mgr
     tps = table(6)
-    m.set_tuning(4, 1, 1)
+    m.set_tuning(ppl, 1, 1)
     a = m.run_batch(tps, *GRID)
-    m.set_tuning(4, 1, 4)
+    m.set_tuning(ppl, 1, split)
     b = m.run_batch(tps, *GRID)
     m.set_tuning(4, 1, 1)
-    # only the grouping of the FP32 partial sums differs
     assert np.max(np.abs(a - b)) / np.max(a) < 1e-6
+    # and the integrated band strengths within the band tolerance
+    assert np.max(np.abs(a.sum(axis=1) - b.sum(axis=1)) / a.sum(axis=1)) < 1e-6
 
 
 def test_graph_follows_tuning_and_buffer_changes(batch_mgr):
