@@ -577,8 +577,9 @@ def main():
         "config": {
             "workload": f"{a.workload}: {prof}, {lst.n_trans} transitions / {lst.n_states} "
                         f"states, T={T} K, P={P} bar, grid {start}-{end} cm-1 x {N} points, "
-                        f"cutoff {spec['cutoff']} cm-1, FP64 pre-pass + FP32 Kahan-compensated "
-                        f"profile sum",
+                        f"cutoff {spec['cutoff']} cm-1, list values (E, nu, A, gamma, n) held "
+                        f"in FP64 ({lst.A.dtype.name} in the input list), FP64 pre-pass + FP32 "
+                        f"Kahan-compensated profile sum",
             "n_eval_per_step": n_eval,
             "xsec_per_s": steps / elapsed,
             "tiles": st["n_tiles"],
@@ -587,14 +588,17 @@ def main():
                                                      "persistent_blocks": a.persistent},
         },
         "roofline": {
-            # compute-bound: FP32 vector ALU work against the FP32 peak (the
-            # kernel issues no matrix instructions)
-            "bound": "mfma",
+            # compute-bound on the FP32 vector ALU (no matrix instructions):
+            # achieved against the FP32 vector peak; frac_unpacked against
+            # the rate without packed FP32 instructions (half the peak),
+            # which this build does not emit
+            "bound": "valu",
             "compute": "fp32 VALU",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
+            "frac_unpacked": achieved / (PEAK_FP32 / 2),
             "traffic": traffic_from_profile(a.workload),
             "kernel": f"k_profile_tiles<{prof.upper()}>",
             "valu_probe": probe,

@@ -23,7 +23,7 @@ EXPORTS = [
     "gexs_create", "gexs_destroy", "gexs_init_vectors", "gexs_init_constants",
     "gexs_transfer_freq", "gexs_transfer_lines", "gexs_transfer_lines_device", "gexs_execute",
     "gexs_transfer_results", "gexs_ready", "gexs_sync", "gexs_last_error", "gexs_last_stats",
-    "gexs_timing", "gexs_line_windows", "gexs_pass_lines", "gexs_set_tuning", "gexs_load_list", "gexs_line_count",
+    "gexs_timing", "gexs_line_windows", "gexs_pass_lines", "gexs_pass_items", "gexs_set_tuning", "gexs_load_list", "gexs_line_count",
     "gexs_partition", "gexs_run_batch", "gexs_batch_device_out",
     "gexs_set_launch", "gexs_autotune",
     "gexs_serve_start", "gexs_submit", "gexs_poll", "gexs_wait", "gexs_release",
@@ -57,7 +57,8 @@ class States(C.Structure):
 
 class Transitions(C.Structure):
     _fields_ = [("n", C.c_int64), ("upper", C.c_void_p), ("lower", C.c_void_p),
-                ("A", C.c_void_p), ("gamma", C.c_void_p), ("nexp", C.c_void_p)]
+                ("A", C.c_void_p), ("gamma", C.c_void_p), ("nexp", C.c_void_p),
+                ("A64", C.c_void_p), ("gamma64", C.c_void_p), ("nexp64", C.c_void_p)]
 
 
 class Grid(C.Structure):
@@ -118,6 +119,7 @@ def load():
         "gexs_timing": (i32, [P, i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(i32)]),
         "gexs_line_windows": (i32, [P, i64, P, P, P]),
         "gexs_pass_lines": (i32, [P, C.POINTER(i64), C.POINTER(i64)]),
+        "gexs_pass_items": (i32, [P, i64, P, C.POINTER(i64)]),
         "gexs_set_tuning": (i32, [P, i32, i32, i32]),
         "gexs_load_list": (i32, [P, C.POINTER(States), C.POINTER(Transitions)]),
         "gexs_line_count": (i32, [P, C.POINTER(i64)]),

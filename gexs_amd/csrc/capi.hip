@@ -159,6 +159,7 @@ struct gexs_ctx {
     DevArr counter;  // bounds of the per-XCD tile queues
     DevArr item_tab; // window and job tables of the tiles kernel
     const void* last_neval = nullptr;  // N_eval of the last pass (device)
+    const WorkItem* last_items = nullptr;  // work items the last FP32C profile kernel read
 
     // execute path: the per-line pre-pass of a pass runs on `pre` into one of
     // two buffer sets while the profile kernel of the previous pass runs on
@@ -550,6 +551,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     B.used = true;
     c->last.n_tiles = c->n_items;
     c->last_neval = t.neval;
+    c->last_items = voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>();
     return GEXS_OK;
 }
 
@@ -896,6 +898,25 @@ int gexs_pass_lines(gexs_ctx* c, int64_t* first, int64_t* count) {
     return GEXS_OK;
 }
 
+int gexs_pass_items(gexs_ctx* c, int64_t cap, int32_t* out, int64_t* n) {
+    if (!c || !n || cap < 0 || (cap > 0 && !out)) return GEXS_EINVAL;
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t m = c->last_items ? c->last.n_tiles : 0;
+    *n = m;
+    if (m == 0 || cap == 0) return GEXS_OK;
+    if (cap < m) return c->fail(GEXS_EINVAL, "pass_items: cap is smaller than the item count");
+    std::vector<WorkItem> h((size_t)m);
+    HIPCHK(c, hipMemcpy(h.data(), c->last_items, (size_t)m * sizeof(WorkItem), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m; ++i) {
+        const WorkItem& w = h[(size_t)i];
+        const int32_t v[10] = {w.lo, w.hi, w.w0, w.w1, w.s1, w.s2, w.s3, w.s4, w.job, w.group};
+        std::memcpy(out + 10 * i, v, sizeof v);
+    }
+    return GEXS_OK;
+}
+
 int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order, int waves_per_tile) {
     if (!c) return GEXS_EINVAL;
     if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4 && points_per_lane != 8)
@@ -912,11 +933,12 @@ int gexs_set_tuning(gexs_ctx* c, int points_per_lane, int order, int waves_per_t
 // ------------------------------------------------------------ resident list
 int gexs_load_list(gexs_ctx* c, const gexs_states* st, const gexs_transitions* tr) {
     if (!c || !st || !tr || st->n <= 0 || tr->n < 0 || !st->E || !st->g) return GEXS_EINVAL;
-    if (tr->n > 0 && (!tr->upper || !tr->lower || !tr->A)) return GEXS_EINVAL;
+    if (tr->n > 0 && (!tr->upper || !tr->lower || (!tr->A && !tr->A64))) return GEXS_EINVAL;
     if (c->server) return c->fail(GEXS_ESTATE, "the list cannot change while serving");
     if (tr->n > 0x7FFFFFFF) return c->fail(GEXS_EINVAL, "more than 2^31-1 transitions");
     if (st->n > 0xFFFFFFFFll) return c->fail(GEXS_EINVAL, "more than 2^32 states");
-    const bool has_gamma = tr->gamma != nullptr && tr->nexp != nullptr;
+    const bool has_gamma = (tr->gamma != nullptr || tr->gamma64 != nullptr) &&
+                           (tr->nexp != nullptr || tr->nexp64 != nullptr);
     if ((c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD) && !has_gamma)
         return c->fail(GEXS_EINVAL, "Voigt profiles need gamma and n per transition");
     // indices are checked on the host before any device gather uses them
@@ -949,17 +971,25 @@ int gexs_load_list(gexs_ctx* c, const gexs_states* st, const gexs_transitions* t
         HIPCHK(c, gxk::assemble_lines(s, c->st_E.as<double>(), c->st_g.as<int32_t>(), up.as<uint32_t>(),
                                       lo.as<uint32_t>(), n, c->r_nu.as<double>(), c->r_E.as<double>(),
                                       c->r_g.as<int32_t>()));
-        HIPCHK(c, hipMemcpyAsync(f32.p, tr->A, (size_t)n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, c->r_A.as<double>()));
+        // an FP64 array is copied as it is; an FP32 one is widened on the
+        // device through the f32 staging buffer (one at a time: the copy
+        // into f32 must wait for the widening that read it before)
+        auto put = [&](const double* h64, const float* h32, DevArr& dst) -> int {
+            if (h64) {
+                HIPCHK(c, hipMemcpyAsync(dst.p, h64, (size_t)n * 8, hipMemcpyHostToDevice, s));
+                return GEXS_OK;
+            }
+            HIPCHK(c, hipStreamSynchronize(s));
+            HIPCHK(c, hipMemcpyAsync(f32.p, h32, (size_t)n * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, dst.as<double>()));
+            return GEXS_OK;
+        };
+        if ((rc = put(tr->A64, tr->A, c->r_A)) != GEXS_OK) return rc;
         if (has_gamma) {
             HIPCHK(c, c->r_gamma.ensure(nb * 8));
             HIPCHK(c, c->r_nexp.ensure(nb * 8));
-            HIPCHK(c, hipStreamSynchronize(s));
-            HIPCHK(c, hipMemcpyAsync(f32.p, tr->gamma, (size_t)n * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, c->r_gamma.as<double>()));
-            HIPCHK(c, hipStreamSynchronize(s));
-            HIPCHK(c, hipMemcpyAsync(f32.p, tr->nexp, (size_t)n * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(c, gxk::widen_f32(s, f32.as<float>(), n, c->r_nexp.as<double>()));
+            if ((rc = put(tr->gamma64, tr->gamma, c->r_gamma)) != GEXS_OK) return rc;
+            if ((rc = put(tr->nexp64, tr->nexp, c->r_nexp)) != GEXS_OK) return rc;
         }
     }
     if ((rc = sort_and_commit(c, n, has_gamma, true)) != GEXS_OK) return rc;
@@ -1191,6 +1221,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     }
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
     c->last.n_tiles = c->n_items;
+    c->last_items = c->items.as<WorkItem>();
     if (out) HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<unsigned long long> ne((size_t)ntp, 0ull);
@@ -1330,26 +1361,26 @@ int gexs_submit(gexs_ctx* c, const gexs_request* req, uint64_t* id) {
     return GEXS_OK;
 }
 
-static int fill_result(gexs_ctx* c, int rc, gxs::Result* r, gexs_result* out) {
+static int fill_result(gexs_ctx* c, int rc, const gxs::ResultView& r, gexs_result* out) {
     if (rc == -6) return c->fail(GEXS_ENOTFOUND, "unknown request id");
     if (rc != 0) return c->fail(rc, "request lookup failed");
-    out->status = r->status;
-    out->npoints = r->status == 0 ? r->npoints : 0;
-    out->sigma = r->status == 0 ? r->sigma.data() : nullptr;
-    out->latency_ms = r->latency_ms;
-    out->device_ms = r->device_ms;
-    out->tile_points = r->tile_points;
-    out->waves_per_tile = r->waves_per_tile;
-    out->line_groups = r->line_groups;
+    out->status = r.status;
+    out->npoints = r.npoints;
+    out->sigma = r.sigma;
+    out->latency_ms = r.latency_ms;
+    out->device_ms = r.device_ms;
+    out->tile_points = r.tile_points;
+    out->waves_per_tile = r.waves_per_tile;
+    out->line_groups = r.line_groups;
     out->reserved = 0;
-    out->n_eval = r->n_eval;
+    out->n_eval = r.n_eval;
     return GEXS_OK;
 }
 
 int gexs_poll(gexs_ctx* c, uint64_t id, gexs_result* out) {
     if (!c || !out) return GEXS_EINVAL;
     if (!c->server) return c->fail(GEXS_ESTATE, "serving not started");
-    gxs::Result* r = nullptr;
+    gxs::ResultView r;
     int rc = c->server->poll(id, &r);
     return fill_result(c, rc, r, out);
 }
@@ -1357,7 +1388,7 @@ int gexs_poll(gexs_ctx* c, uint64_t id, gexs_result* out) {
 int gexs_wait(gexs_ctx* c, uint64_t id, gexs_result* out) {
     if (!c || !out) return GEXS_EINVAL;
     if (!c->server) return c->fail(GEXS_ESTATE, "serving not started");
-    gxs::Result* r = nullptr;
+    gxs::ResultView r;
     int rc = c->server->wait(id, &r);
     return fill_result(c, rc, r, out);
 }

@@ -67,6 +67,18 @@ struct Result {
     int64_t n_eval = 0;  // line-point evaluations of the request
 };
 
+// A request's state as one consistent record: poll and wait fill it while
+// holding the server lock.  sigma points into the request's result, which
+// is not written again once status is 0 and stays valid until release.
+struct ResultView {
+    int status = 1;
+    int npoints = 0;
+    const double* sigma = nullptr;
+    double latency_ms = 0.0, device_ms = 0.0;
+    int tile_points = 0, waves_per_tile = 0, line_groups = 0;
+    int64_t n_eval = 0;
+};
+
 // first-fit allocator over one device allocation (offsets in bytes)
 class Arena {
    public:
@@ -90,8 +102,8 @@ class Server {
     ~Server();
     int start(std::string* err);
     int submit(const Request& r, uint64_t* id, std::string* err);
-    int poll(uint64_t id, Result** out);
-    int wait(uint64_t id, Result** out);
+    int poll(uint64_t id, ResultView* out);
+    int wait(uint64_t id, ResultView* out);
     int release(uint64_t id);
     // counters since start
     uint64_t batches() const { return batches_.load(); }

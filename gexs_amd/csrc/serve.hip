@@ -227,21 +227,37 @@ int Server::submit(const Request& r, uint64_t* id, std::string* err) {
     return 0;
 }
 
-int Server::poll(uint64_t id, Result** out) {
+namespace {
+// the caller holds the server lock
+void view_of(const Result& r, ResultView* v) {
+    const int st = r.status;
+    v->status = st;
+    v->npoints = st == 0 ? r.npoints : 0;
+    v->sigma = st == 0 ? r.sigma.data() : nullptr;
+    v->latency_ms = r.latency_ms;
+    v->device_ms = r.device_ms;
+    v->tile_points = r.tile_points;
+    v->waves_per_tile = r.waves_per_tile;
+    v->line_groups = r.line_groups;
+    v->n_eval = r.n_eval;
+}
+}  // namespace
+
+int Server::poll(uint64_t id, ResultView* out) {
     std::lock_guard<std::mutex> g(mu_);
     auto it = reqs_.find(id);
     if (it == reqs_.end()) return -6;
-    *out = &it->second->res;
+    view_of(it->second->res, out);
     return 0;
 }
 
-int Server::wait(uint64_t id, Result** out) {
+int Server::wait(uint64_t id, ResultView* out) {
     std::unique_lock<std::mutex> g(mu_);
     auto it = reqs_.find(id);
     if (it == reqs_.end()) return -6;
     Req* q = it->second.get();
     cv_done_.wait(g, [&] { return q->res.status != 1; });
-    *out = &q->res;
+    view_of(q->res, out);
     return 0;
 }
 
@@ -542,6 +558,10 @@ void Server::loop() {
             // the arena's regions of this batch belong to this thread
             g.unlock();
             const int lrc = launch(*s, batch, &err);
+            // copies and kernels enqueued before a failure may still be
+            // writing the batch's arena regions and slot buffers: they are
+            // given back only once the slot's stream has drained
+            if (lrc != 0) (void)hipStreamSynchronize(s->stream);
             g.lock();
             if (lrc != 0) {
                 for (Req* q : batch) {

@@ -152,23 +152,39 @@ class HipManager:
         self._chk(self._L.gexs_line_windows(self._h, int(n), _ptr(nu), _ptr(ib), _ptr(ie)))
         return nu, ib, ie
 
+    def pass_items(self):
+        """Work items of the last FP32C pass as an int32 array (n, 10): lo, hi,
+        w0, w1, s1, s2, s3, s4, job, group (line range, grid points, segments)."""
+        n = C.c_int64()
+        self._chk(self._L.gexs_pass_items(self._h, 0, None, C.byref(n)))
+        out = np.empty((n.value, 10), np.int32)
+        if n.value:
+            self._chk(self._L.gexs_pass_items(self._h, n.value, _ptr(out), C.byref(n)))
+        return out
+
     def set_tuning(self, points_per_lane=4, order=1, waves_per_tile=1):
         self._chk(self._L.gexs_set_tuning(self._h, int(points_per_lane), int(order),
                                           int(waves_per_tile)))
 
     # ------------------------------------------------- resident list / batch
     def load_list(self, lst):
-        """Upload a synth.LineList (states + transitions) and sort it on the device."""
+        """Upload a synth.LineList (states + transitions) and sort it on the device.
+        A, gamma and n go up in the list's own precision (FP32 or FP64); the
+        device holds them in FP64 either way."""
         E = np.ascontiguousarray(lst.E, dtype=np.float64)
         g = np.ascontiguousarray(lst.g, dtype=np.int32)
         up = np.ascontiguousarray(lst.upper, dtype=np.uint32)
         lo = np.ascontiguousarray(lst.lower, dtype=np.uint32)
-        A = np.ascontiguousarray(lst.A, dtype=np.float32)
-        ga = np.ascontiguousarray(lst.gamma, dtype=np.float32)
-        nx = np.ascontiguousarray(lst.nexp, dtype=np.float32)
         st = _lib.States(E.shape[0], E.ctypes.data, g.ctypes.data)
-        tr = _lib.Transitions(up.shape[0], up.ctypes.data, lo.ctypes.data, A.ctypes.data,
-                              ga.ctypes.data, nx.ctypes.data)
+        f64 = lst.A.dtype == np.float64
+        dt = np.float64 if f64 else np.float32
+        A = np.ascontiguousarray(lst.A, dtype=dt)
+        ga = np.ascontiguousarray(lst.gamma, dtype=dt)
+        nx = np.ascontiguousarray(lst.nexp, dtype=dt)
+        arrs = [A.ctypes.data, ga.ctypes.data, nx.ctypes.data]
+        none = [None, None, None]
+        tr = _lib.Transitions(up.shape[0], up.ctypes.data, lo.ctypes.data,
+                              *(none + arrs if f64 else arrs + none))
         self._chk(self._L.gexs_load_list(self._h, C.byref(st), C.byref(tr)))
         n = C.c_int64()
         self._chk(self._L.gexs_line_count(self._h, C.byref(n)))

@@ -143,6 +143,16 @@ int gexs_line_windows(gexs_ctx* ctx, int64_t n, double* nu_sorted, int32_t* ib, 
  * lines with start - half_width <= nu <= end + half_width,
  * Profiles/BaseProfile.cpp:152-157). */
 int gexs_pass_lines(gexs_ctx* ctx, int64_t* first, int64_t* count);
+/* The work items (tiles) the last FP32C profile kernel read, 10 int32 each:
+ * {lo, hi, w0, w1, s1, s2, s3, s4, job, group} -- the tile's line range
+ * [lo, hi) among the pass's lines, its grid points [w0, w1) and the segment
+ * bounds: [lo, s1) windows ending inside the tile, [s1, s4) windows covering
+ * it, of which [s2, s3) are centred near it (Humlicek regions II-IV
+ * possible), [s4, hi) windows starting inside it.  *n receives the item
+ * count (0 after an FP64 pass); out needs cap >= *n rows (cap 0: count only).
+ * Waits for the ctx stream.  No counterpart in the reference: it lets a
+ * caller see which line shares a launch shape gave each wave. */
+int gexs_pass_items(gexs_ctx* ctx, int64_t cap, int32_t* out, int64_t* n);
 
 /* Tuning of the FP32C profile kernel: points per lane (1, 2, 4 or 8; a
  * wave covers 64*ppl grid points), the dispatch order of the tiles (0 = grid
@@ -166,6 +176,12 @@ typedef struct gexs_transitions {
     const float* A;         /* Einstein A, s^-1 */
     const float* gamma;     /* Lorentz gamma_ref (NULL: Doppler-only list) */
     const float* nexp;      /* temperature exponent */
+    /* FP64 lists: each non-NULL array here is used in place of its FP32
+     * twin above (which may then be NULL), so an FP64 list keeps its values
+     * (the device holds A, gamma and n in FP64 either way). */
+    const double* A64;
+    const double* gamma64;
+    const double* nexp64;
 } gexs_transitions;
 
 /* Upload, assemble lines (nu = E_u - E_l, E'' = E_l, g' = g_u), drop lines

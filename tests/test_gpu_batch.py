@@ -79,12 +79,12 @@ def test_launch_modes_agree_bitwise(batch_mgr, split):
         assert np.array_equal(o, outs[0]), k
 
 
-@pytest.mark.parametrize("ppl", [1, 4])
+@pytest.mark.parametrize("ppl", [1, 2, 4])
 @pytest.mark.parametrize("split", [4, 16])
 def test_split_close_to_single_wave(batch_mgr, ppl, split):
     """Waves of a split tile take equal shares of the lines before, inside
     and after the tile's near-centre lines (three pieces each when the tile
-    has at least `split` near lines, as most tiles of this table do): every
+    has at least `split` near lines, as many tiles of this table do): every
     line is summed once, so only the grouping of the FP32 partial sums
     differs from one wave per tile."""
     L, m = batch_mgr
@@ -93,6 +93,11 @@ def test_split_close_to_single_wave(batch_mgr, ppl, split):
     a = m.run_batch(tps, *GRID)
     m.set_tuning(ppl, 1, split)
     b = m.run_batch(tps, *GRID)
+    it = m.pass_items()
+    three = int(np.sum((it[:, 4] < it[:, 7]) & (it[:, 6] - it[:, 5] >= split)))
+    print(f"ppl={ppl} split={split}: {three} of {it.shape[0]} items take three-piece shares",
+          flush=True)
+    assert three > 0
     m.set_tuning(4, 1, 1)
     assert np.max(np.abs(a - b)) / np.max(a) < 1e-6
     # and the integrated band strengths within the band tolerance

@@ -78,18 +78,40 @@ def c1_small():
     return L, L.lines()
 
 
-@pytest.mark.parametrize("ppl,split", [(1, 1), (4, 1), (8, 1), (1, 4), (4, 4), (8, 4), (1, 16),
-                                       (4, 16)])
-def test_voigt_fp32c_vs_oracle(O, c1_small, ppl, split):
+@pytest.fixture(scope="module")
+def c1_small_ref(O, c1_small):
     L, lines = c1_small
     T, P = 296.0, 1.0
     Q = O.partition(L.E, L.g, T)
-    log(f"voigt fp32c ppl={ppl} split={split} lines={lines[0].shape[0]}")
-    out, st, _ = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0, ppl=ppl,
-                             split=split)
     ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0)
+    return Q, ref, ne
+
+
+# every (points per lane, waves per tile) the autotuner can pick, with the tile
+# orders of the launch shapes it chose on the 10^6-line workload (order 3:
+# dispatch order, the timed shape at 2 points per lane and 4 waves per tile)
+SHAPES = [(ppl, split, order) for ppl in (1, 2, 4, 8) for split in (1, 4, 16)
+          for order in ((1, 3) if (ppl, split) == (2, 4) else (1,))]
+
+
+@pytest.mark.parametrize("ppl,split,order", SHAPES)
+def test_voigt_fp32c_vs_oracle(O, c1_small, c1_small_ref, ppl, split, order):
+    L, lines = c1_small
+    T, P = 296.0, 1.0
+    Q, ref, ne = c1_small_ref
+    log(f"voigt fp32c ppl={ppl} split={split} order={order} lines={lines[0].shape[0]}")
+    out, st, m = run_manager("voigt", lines, T, P, Q, 0.0, 10000.0, 1_000_000, 25.0, ppl=ppl,
+                             split=split, order=order)
     assert st["n_eval"] == ne
-    check(out, ref, 1e-4, 1e-6, f"voigt fp32c ppl={ppl} split={split}")
+    check(out, ref, 1e-4, 1e-6, f"voigt fp32c ppl={ppl} split={split} order={order}")
+    if split > 1:
+        # the waves of a tile take three-piece shares (before, inside and after
+        # the near-centre lines [s2, s3)) where a tile has >= split near lines
+        it = m.pass_items()
+        assert it.shape[0] == st["n_tiles"] and np.all(it[:, 3] - it[:, 2] <= 64 * ppl)
+        three = int(np.sum((it[:, 4] < it[:, 7]) & (it[:, 6] - it[:, 5] >= split)))
+        log(f"  {three} of {it.shape[0]} tiles take three-piece shares")
+        assert three > 0
 
 
 def test_voigt_fp64_vs_oracle(O, c1_small):

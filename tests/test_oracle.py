@@ -287,3 +287,103 @@ def test_neval_counts_windows(O):
     p = O.params("voigt", T=296.0, P=1.0, Q=100.0, cutoff=25.0)
     ib, ie = O.windows(nu, 25.0, start, dnu, 0, N)
     assert O.neval(p, start, dnu, N, nu, E, A, g) == int(np.maximum(ie - ib, 0).sum())
+
+
+# ------------------------------------------- branches the GPU defaults use
+def test_pressure_mode_scales_gamma_by_p_times_pscale(O):
+    """pressure_mode 1 (gammaL times P * p_scale, as the reference GPU path,
+    cross_kernal_v2.cu:142) against pressure_mode 0 (the reference CPU form,
+    OpenMPManager.cpp:398, pinned bit for bit against the reference build):
+    with P * p_scale = 1 the two agree bit for bit, and with P * p_scale = k
+    (k a power of two, so every product is exact) mode 1 equals mode 0 with
+    gamma_ref scaled by k -- line widths and the whole Voigt cross-section.
+    p_scale applied twice, or P alone, fails: (P, p_scale) = (1, 2), (4, 0.5)
+    and (0.25, 8) all give k = 2."""
+    L, (nu, E, A, g, gam, nex) = lines_small(0.0005)
+    T, N, start, dnu = 700.0, 50000, 0.0, 0.2
+    Q = O.partition(L.E, L.g, T)
+    base = O.params("voigt", T=T, P=123.0, Q=Q, cutoff=25.0, pressure_mode=0)
+    ref0, _ = O.cross_section(base, start, dnu, N, nu, E, A, g, gam, nex)
+    _, gl0, _ = O.line_params(base, nu, E, A, g, gam, nex)
+    for P, ps in ((1.0, 1.0), (0.5, 2.0), (8.0, 0.125)):
+        p1 = O.params("voigt", T=T, P=P, Q=Q, cutoff=25.0, pressure_mode=1, p_scale=ps)
+        assert np.array_equal(O.line_params(p1, nu, E, A, g, gam, nex)[1], gl0)
+        out, _ = O.cross_section(p1, start, dnu, N, nu, E, A, g, gam, nex)
+        assert np.array_equal(out, ref0)
+    ref2, _ = O.cross_section(base, start, dnu, N, nu, E, A, g, 2.0 * gam, nex)
+    _, gl2, _ = O.line_params(base, nu, E, A, g, 2.0 * gam, nex)
+    for P, ps in ((1.0, 2.0), (4.0, 0.5), (0.25, 8.0)):
+        p1 = O.params("voigt", T=T, P=P, Q=Q, cutoff=25.0, pressure_mode=1, p_scale=ps)
+        assert np.array_equal(O.line_params(p1, nu, E, A, g, gam, nex)[1], gl2)
+        out, _ = O.cross_section(p1, start, dnu, N, nu, E, A, g, gam, nex)
+        assert np.array_equal(out, ref2)
+    # and against the closed form gamma (T_ref/T)^n P p_scale in high precision
+    import mpmath as mp
+    mp.mp.dps = 30
+    p1 = O.params("voigt", T=T, P=3.0, Q=Q, cutoff=25.0, pressure_mode=1, p_scale=1 / 1.01325)
+    gl = O.line_params(p1, nu, E, A, g, gam, nex)[1]
+    for k in range(0, nu.shape[0], 97):
+        want = mp.mpf(gam[k]) * (mp.mpf(296) / T) ** mp.mpf(nex[k]) * 3 * (1 / mp.mpf("1.01325"))
+        assert abs(gl[k] - float(want)) <= 4e-16 * float(want)
+
+
+def test_textbook_width_is_the_doppler_b(O):
+    """width_conv 1 (the reference GPU width, GPUManager.cpp:35 with
+    cross_kernal_v2.cu:441): 1/w = 1/b_D, b_D = (nu/c) sqrt(2 k T N_A / m),
+    the 1/e half width of the Doppler profile, written here from the physical
+    constants; width_conv 0 (the reference CPU form, pinned against its build)
+    is the HWHM, b_D sqrt(ln 2).  So width_conv 1 at mass m equals width_conv 0
+    at mass m ln 2, for the widths and the whole Voigt cross-section, to the
+    rounding of the 9-digit sqrt(ln 2) constant (defines.h).  A textbook
+    branch equal to the CPU one fails both."""
+    L, (nu, E, A, g, gam, nex) = lines_small(0.0005)
+    T, N, start, dnu, m = 1500.0, 50000, 0.0, 0.2, 28.0
+    Q = O.partition(L.E, L.g, T)
+    kB, NA, c = 1.380658e-16, 6.0221412927e23, 2.99792458e10
+    pt = O.params("voigt", T=T, P=0.3, Q=Q, cutoff=25.0, mass=m, width_conv=1)
+    iw_t = O.line_params(pt, nu, E, A, g, gam, nex)[2]
+    bD = nu / c * np.sqrt(2.0 * kB * T * NA / m)
+    assert np.max(np.abs(iw_t * bD - 1.0)) < 1e-9
+    pc = O.params("voigt", T=T, P=0.3, Q=Q, cutoff=25.0, mass=m * 0.69314718056, width_conv=0)
+    iw_c = O.line_params(pc, nu, E, A, g, gam, nex)[2]
+    assert np.max(np.abs(iw_c / iw_t - 1.0)) < 1e-9
+    a, _ = O.cross_section(pt, start, dnu, N, nu, E, A, g, gam, nex)
+    b, _ = O.cross_section(pc, start, dnu, N, nu, E, A, g, gam, nex)
+    assert np.max(np.abs(a - b)) / a.max() < 1e-8
+    # the CPU convention at the same mass is sqrt(ln 2) narrower
+    p0 = O.params("voigt", T=T, P=0.3, Q=Q, cutoff=25.0, mass=m, width_conv=0)
+    iw_0 = O.line_params(p0, nu, E, A, g, gam, nex)[2]
+    assert np.allclose(iw_0 / iw_t, 1 / math.sqrt(math.log(2.0)), rtol=1e-9)
+
+
+def test_fixed_hwhm_gaussian_closed_form_and_half_maximum(O):
+    """GAUSSIAN: a bin-averaged Gaussian of fixed half width at half maximum h
+    (HALF-WIDTH, Input.cpp:168-169; gammaG = sqrt(ln2)/h as
+    cross_kernal.cu:647-716): w = h / sqrt(ln 2), and per grid point
+    sigma_j = S/(2 dnu) [erf((d + dnu/2)/w) - erf((d - dnu/2)/w)], d = |nu - f_j|,
+    written here with scipy's erf.  On a grid much finer than h the profile
+    falls to half its peak at nu0 +- h and integrates to S.  A width of 1/h,
+    or of h itself, fails."""
+    nu, E, A, g = np.array([1500.0037]), np.array([300.0]), np.array([2.0]), np.array([3], np.int32)
+    T, Q, h = 800.0, 50.0, 0.04
+    for N, start, dnu in ((2000, 1498.0, 0.002), (200000, 1499.0, 1e-5)):
+        p = O.params("gaussian", T=T, Q=Q, cutoff=10 * h, hwhm=h)
+        out, _ = O.cross_section(p, start, dnu, N, nu, E, A, g)
+        S, _, iw = O.line_params(p, nu, E, A, g, np.zeros(1), np.zeros(1))
+        w = h / math.sqrt(0.69314718056)
+        assert iw[0] == pytest.approx(1.0 / w, rel=1e-9)
+        f = start + np.arange(N) * dnu
+        d = np.abs(nu[0] - f)
+        want = S[0] * 0.5 / dnu * (erf((d + dnu / 2) / w) - erf((d - dnu / 2) / w))
+        ib = round((nu[0] - 10 * h - start) / dnu)
+        ie = round((nu[0] + 10 * h - start) / dnu)
+        want[:ib] = 0
+        want[ie:] = 0
+        assert np.max(np.abs(out - want)) < 1e-9 * want.max()
+        assert out.sum() * dnu == pytest.approx(S[0], rel=1e-9)
+    # the fine grid: half maximum at nu0 +- h (linear interpolation between points)
+    peak = out.max()
+    for side in (-1.0, 1.0):
+        fx = nu[0] + side * h
+        val = np.interp(fx, f, out)
+        assert val == pytest.approx(0.5 * peak, rel=2e-4)
