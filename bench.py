@@ -61,8 +61,10 @@ def parse():
                     help="skip the batched (T,P) table (10^7 lines, 120 cross-sections)")
     ap.add_argument("--table-steps", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving sweep")
-    ap.add_argument("--stream", action="store_true",
-                    help="also run the 10^8-transition hot list (T = 2000 K, P = 0.1 bar)")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the 10^8-transition hot list (T = 2000 K, P = 0.1 bar)")
+    ap.add_argument("--no-profiles", action="store_true",
+                    help="skip the Doppler (2e5 lines, cutoff 1 cm-1) and Voigt-Quad workloads")
     ap.add_argument("--serve-rates", default="10,25,50,100,200,500")
     return ap.parse_args()
 
@@ -429,6 +431,126 @@ def valu_probe():
     return res
 
 
+def execute_bench(a, workload, profile=None, steps=None, warm=None, fp64_steps=None,
+                  tuned=None, log_candidates=False):
+    """ExecuteCrossSection over the resident list of `workload` (its profile,
+    or `profile`): autotuned launch shape (or `tuned`, or --ppl), `warm`
+    untimed then `steps` timed steps, HIP events around each profile kernel;
+    then the plain FP64 path on the same input.  Returns the measurements
+    and the manager inputs the headline also uses."""
+    from gexs_amd import synth
+    from gexs_amd.manager import HipManager
+    spec = synth.WORKLOADS[workload]
+    prof = profile or spec["profile"]
+    steps = a.steps if steps is None else steps
+    warm = a.warmup if warm is None else warm
+    fp64_steps = a.fp64_steps if fp64_steps is None else fp64_steps
+    T = spec["T"] if spec["T"] else 1000.0
+    P = spec["P"] if spec["P"] is not None else 0.0
+    start, end, N = spec["grid"]
+    dnu = (end - start) / N
+    lst = big_list(workload)
+    m = HipManager(prof)
+    if tuned is not None:
+        m.set_tuning(tuned["points_per_lane"], tuned["order"], tuned["waves_per_tile"])
+        m.set_launch(tuned["persistent_blocks"], False)
+    elif a.ppl:
+        m.set_tuning(a.ppl, a.order, a.split)
+        m.set_launch(a.persistent, False)
+    m.InitializeVectors(N, steps + warm)
+    m.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF, synth.P_SCALE_BAR,
+                          cutoff=spec["cutoff"])
+    m.TransferFreq(start + np.arange(N, dtype=np.float64) * dnu)
+    t0 = time.perf_counter()
+    n = m.load_list(lst)  # H2D, device assembly and nu sort; outside the timed region
+    Q = float(m.partition([T])[0])
+    log(f"[bench] {workload} ({prof}): list resident + sorted in {time.perf_counter() - t0:.2f} s, "
+        f"Q(T)={Q:.6g}")
+    tuning = None
+    if tuned is None and not a.no_autotune and not a.ppl:
+        best, cands = m.autotune([(T, P, Q)], start, end, N)
+        tuning = {"best": best, "candidates": cands}
+        log(f"[bench] {workload} ({prof}) autotune: {best}")
+        if log_candidates:
+            for c in sorted(cands, key=lambda c: c["kernel_ms"]):
+                log(f"[bench]   candidate {c}")
+    for _ in range(warm):
+        m.ExecuteCrossSection(N, n, 0, T, P, Q)
+    m.sync()
+    n_eval = m.stats()["n_eval"]
+    m.timing(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.ExecuteCrossSection(N, n, 0, T, P, Q)
+    m.sync()
+    elapsed = time.perf_counter() - t0
+    k_ms, _, cnt = m.timing()
+    assert cnt == steps, (cnt, steps)
+    st = m.stats()
+    m.close()
+    value = n_eval * steps / elapsed
+    k_avg = k_ms / cnt
+    flops = FLOP_PER_EVAL[prof]
+    achieved = n_eval * flops / (k_avg * 1e-3)
+    log(f"[bench] {workload} ({prof}): {steps} steps in {elapsed * 1e3:.1f} ms; kernel avg "
+        f"{k_avg:.4f} ms; N_eval {n_eval:.4e}; {value:.4e} evals/s")
+    fp64 = None
+    if fp64_steps > 0:
+        m64 = HipManager(prof, precision="fp64")
+        m64.InitializeVectors(N, fp64_steps + 1)
+        m64.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF,
+                                synth.P_SCALE_BAR, cutoff=spec["cutoff"])
+        m64.TransferFreq(start + np.arange(N, dtype=np.float64) * dnu)
+        m64.load_list(lst)
+        m64.ExecuteCrossSection(N, n, 0, T, P, Q)
+        m64.sync()
+        t1 = time.perf_counter()
+        for _ in range(fp64_steps):
+            m64.ExecuteCrossSection(N, n, 0, T, P, Q)
+        m64.sync()
+        e64 = time.perf_counter() - t1
+        v64 = n_eval * fp64_steps / e64
+        fp64 = {"value": v64, "unit": "evals/s", "ms_per_step": e64 * 1e3 / fp64_steps,
+                "speedup_fp32c_over_fp64": value / v64}
+        log(f"[bench] {workload} ({prof}): plain FP64 path {v64:.4e} evals/s (FP32C "
+            f"{value / v64:.2f}x faster)")
+        m64.close()
+    used = tuning["best"] if tuning else (tuned or {
+        "points_per_lane": a.ppl or 4, "waves_per_tile": a.split, "order": a.order,
+        "persistent_blocks": a.persistent})
+    return {"workload": workload, "profile": prof, "lst": lst, "T": T, "P": P, "Q": Q, "n": n,
+            "spec": spec, "steps": steps, "warm": warm, "elapsed": elapsed, "value": value,
+            "n_eval": n_eval, "k_avg": k_avg, "flops": flops, "achieved": achieved,
+            "tuning": tuning, "used": used, "tiles": st["n_tiles"], "fp64": fp64}
+
+
+def roofline_of(r, traffic=None, probe=None):
+    """The profile kernel's FP32 VALU rate against the MI355X FP32 vector peak
+    (frac_unpacked: against half of it, the rate without packed FP32
+    instructions, which this build does not emit)."""
+    return {"bound": "valu", "compute": "fp32 VALU", "achieved": r["achieved"] / 1e12,
+            "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": r["achieved"] / PEAK_FP32,
+            "frac_unpacked": r["achieved"] / (PEAK_FP32 / 2), "traffic": traffic,
+            "kernel": f"k_profile_tiles<{r['profile'].upper()}>", "valu_probe": probe,
+            "kernel_ms_avg": r["k_avg"], "flop_per_eval": r["flops"]}
+
+
+def side_workload(a, workload, profile=None):
+    """An extra execute-path workload for the JSON: throughput, roofline and
+    the plain FP64 path beside it."""
+    r = execute_bench(a, workload, profile=profile, steps=max(5, a.steps // 2),
+                      warm=max(1, a.warmup), fp64_steps=max(1, a.fp64_steps))
+    sp = r["spec"]
+    start, end, N = sp["grid"]
+    return {"workload": f"{workload}: {r['profile']}, {r['lst'].n_trans} transitions / "
+                        f"{r['lst'].n_states} states, T={r['T']} K, P={r['P']} bar, grid {start}-"
+                        f"{end} cm-1 x {N} points, cutoff {sp['cutoff']} cm-1",
+            "value": r["value"], "unit": "evals/s", "ms_per_step": r["elapsed"] * 1e3 / r["steps"],
+            "steps": r["steps"], "n_eval_per_step": r["n_eval"], "tuning": r["used"],
+            "roofline": roofline_of(r, traffic_from_profile(f"{workload}_{r['profile']}")),
+            "fp64_gpu_path": r["fp64"]}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -441,79 +563,16 @@ def main():
     from gexs_amd import synth
     from gexs_amd.manager import HipManager
 
-    spec = synth.WORKLOADS[a.workload]
-    prof = spec["profile"]
-    T = spec["T"] if spec["T"] else 1000.0
-    P = spec["P"] if spec["P"] is not None else 0.0
+    log(f"[bench] generating {a.workload}")
+    r = execute_bench(a, a.workload, log_candidates=True)
+    spec, prof, lst = r["spec"], r["profile"], r["lst"]
+    T, P, Q = r["T"], r["P"], r["Q"]
     start, end, N = spec["grid"]
     dnu = (end - start) / N
-    log(f"[bench] generating {a.workload}")
-    lst = synth.workload_list(a.workload)
+    steps, warm, elapsed, value = r["steps"], r["warm"], r["elapsed"], r["value"]
+    n_eval, k_avg, achieved, flops = r["n_eval"], r["k_avg"], r["achieved"], r["flops"]
+    tuning, fp64 = r["tuning"], r["fp64"]
     log(f"[bench] {lst.n_trans} transitions over {lst.n_states} states")
-
-    steps, warm = a.steps, a.warmup
-    m = HipManager(prof)
-    if a.ppl:
-        m.set_tuning(a.ppl, a.order, a.split)
-        m.set_launch(a.persistent, False)
-    m.InitializeVectors(N, steps + warm)
-    m.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF, synth.P_SCALE_BAR,
-                          cutoff=spec["cutoff"])
-    m.TransferFreq(start + np.arange(N, dtype=np.float64) * dnu)
-    t0 = time.perf_counter()
-    n = m.load_list(lst)  # H2D, device assembly and nu sort; outside the timed region
-    Q = float(m.partition([T])[0])
-    log(f"[bench] list resident + sorted in {time.perf_counter() - t0:.2f} s, Q(T)={Q:.6g}")
-    tuning = None
-    if not a.no_autotune and not a.ppl:
-        best, cands = m.autotune([(T, P, Q)], start, end, N)
-        tuning = {"best": best, "candidates": cands}
-        log(f"[bench] autotune: {best}")
-        for c in sorted(cands, key=lambda c: c["kernel_ms"]):
-            log(f"[bench]   candidate {c}")
-    for _ in range(warm):
-        m.ExecuteCrossSection(N, n, 0, T, P, Q)
-    m.sync()
-    n_eval = m.stats()["n_eval"]
-    m.timing(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        m.ExecuteCrossSection(N, n, 0, T, P, Q)
-    m.sync()
-    elapsed = time.perf_counter() - t0
-    k_ms, tot_ms, cnt = m.timing()
-    assert cnt == steps, (cnt, steps)
-    st = m.stats()
-    value = n_eval * steps / elapsed
-    k_avg = k_ms / cnt
-    flops = FLOP_PER_EVAL[prof]
-    achieved = n_eval * flops / (k_avg * 1e-3)
-    log(f"[bench] {steps} steps in {elapsed * 1e3:.1f} ms; kernel avg {k_avg:.3f} ms; "
-        f"N_eval {n_eval:.4e}; {value:.4e} evals/s")
-
-    # plain FP64 GPU path on the same input
-    fp64 = None
-    if a.fp64_steps > 0:
-        m64 = HipManager(prof, precision="fp64")
-        m64.InitializeVectors(N, a.fp64_steps + 1)
-        m64.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF,
-                                synth.P_SCALE_BAR, cutoff=spec["cutoff"])
-        m64.TransferFreq(start + np.arange(N, dtype=np.float64) * dnu)
-        m64.load_list(lst)
-        m64.ExecuteCrossSection(N, n, 0, T, P, Q)
-        m64.sync()
-        t1 = time.perf_counter()
-        for _ in range(a.fp64_steps):
-            m64.ExecuteCrossSection(N, n, 0, T, P, Q)
-        m64.sync()
-        e64 = time.perf_counter() - t1
-        v64 = n_eval * a.fp64_steps / e64
-        fp64 = {"value": v64, "unit": "evals/s", "ms_per_step": e64 * 1e3 / a.fp64_steps,
-                "speedup_fp32c_over_fp64": value / v64}
-        log(f"[bench] plain FP64 path {v64:.4e} evals/s ({value / v64:.2f}x slower than FP32C)")
-        m64.close()
-
-    m.close()
 
     # PCIe-inclusive: the BaseManager boundary hands over host buffers, so
     # time steps that also move the line arrays in (TransferVectors, which
@@ -548,9 +607,13 @@ def main():
     log(f"[bench] PCIe-inclusive step {pcie['ms_per_step']:.2f} ms, {pcie['value']:.4e} evals/s")
     mp.close()
 
+    doppler = voigt_quad = None
+    if not a.no_profiles:
+        doppler = side_workload(a, "doppler_2e5")
+        voigt_quad = side_workload(a, "voigt_1e6", profile="voigt_quad")
     table = None if a.no_table else table_bench(a)
     served = None if a.no_serve else serve_bench(a)
-    stream = stream_bench(a) if a.stream else None
+    stream = None if a.no_stream else stream_bench(a)
 
     probe = None if a.no_probe else valu_probe()
 
@@ -582,32 +645,15 @@ def main():
                         f"Kahan-compensated profile sum",
             "n_eval_per_step": n_eval,
             "xsec_per_s": steps / elapsed,
-            "tiles": st["n_tiles"],
-            "tuning": tuning["best"] if tuning else {"points_per_lane": a.ppl or 4,
-                                                     "waves_per_tile": a.split, "order": a.order,
-                                                     "persistent_blocks": a.persistent},
+            "tiles": r["tiles"],
+            "tuning": r["used"],
         },
-        "roofline": {
-            # compute-bound on the FP32 vector ALU (no matrix instructions):
-            # achieved against the FP32 vector peak; frac_unpacked against
-            # the rate without packed FP32 instructions (half the peak),
-            # which this build does not emit
-            "bound": "valu",
-            "compute": "fp32 VALU",
-            "achieved": achieved / 1e12,
-            "peak": PEAK_FP32 / 1e12,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32,
-            "frac_unpacked": achieved / (PEAK_FP32 / 2),
-            "traffic": traffic_from_profile(a.workload),
-            "kernel": f"k_profile_tiles<{prof.upper()}>",
-            "valu_probe": probe,
-            "kernel_ms_avg": k_avg,
-            "flop_per_eval": flops,
-        },
+        "roofline": roofline_of(r, traffic_from_profile(a.workload), probe),
         "cpu_baseline": cpu,
         "fp64_gpu_path": fp64,
         "pcie_inclusive": pcie,
+        "doppler": doppler,
+        "voigt_quad": voigt_quad,
         "batched": table,
         "served": served,
         "stream": stream,

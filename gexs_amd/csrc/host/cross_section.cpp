@@ -268,11 +268,15 @@ void CrossSection::OutputProfile(FILE* out) {
     for (size_t i = 0; i < tp_.size(); ++i) {
         std::vector<double>& intens = results_[i];
         manager_->TransferResults(freq_.data(), intens.data(), npoints_, (int)i);
-        if (!out) continue;
         if (i == 0) {
             double cross = 0.0;
             for (double v : intens) cross += v;
             cross *= dfreq_;
+            integrated_ = cross;
+        }
+        if (!out) continue;
+        if (i == 0) {
+            const double cross = integrated_;
             std::fprintf(out,
                          "Our summed intensity = %16.6E compared to the integrated intensity = "
                          "%16.6E with a difference of %12.6f %%\n",

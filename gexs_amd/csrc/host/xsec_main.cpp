@@ -35,6 +35,9 @@ int main(int argc, char** argv) {
         xs.OutputProfile(stdout);
         std::fprintf(stderr, "lines read %lld, selected %lld, chunks %d\n", (long long)xs.LinesRead(),
                      (long long)xs.LinesSelected(), xs.Chunks());
+        // the two band-strength sums of the report line at full precision
+        std::fprintf(stderr, "band strength: summed %.17g integrated %.17g\n", xs.TotalIntensity(),
+                     xs.IntegratedIntensity());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "gexs_xsec: %s\n", e.what());
         return 1;

@@ -623,8 +623,7 @@ __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp,
                 float k;
                 if (PROF == GX_PROF_VOIGT_QUAD) {
                     const float rr = x * x + rx.y * rx.y;
-                    k = rr < 100.0f ? (float)gx::voigt916_f64((double)x, (double)rx.y)
-                                    : gx::gh11_f32(x, rx.y);
+                    k = rr < 100.0f ? gx::voigt916_f32(x, rx.y) : gx::gh11_f32(x, rx.y);
                 } else {
                     k = gx::bin_erf_f32(x, rx.y);
                 }
@@ -674,9 +673,15 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
 // runs checked_line on VGPR operands.  No scalar loads in the loop.
 constexpr int kNearRow = 12;
 
-// words per line row of the wave's LDS block (the wider of the two uses)
+// words per line row of the wave's LDS block (the widest row the profile
+// stages: Voigt region-I and near-centre rows, or bin-Gauss rows)
 template <int PPL>
-constexpr int wave_row() { return stage_row<PPL>() > kNearRow ? stage_row<PPL>() : kNearRow; }
+constexpr int gauss_row();
+template <int PROF, int PPL>
+constexpr int wave_row() {
+    if (PROF == GX_PROF_DOPPLER) return gauss_row<PPL>();
+    return stage_row<PPL>() > kNearRow ? stage_row<PPL>() : kNearRow;
+}
 
 template <int PPL>
 __device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobRec grec,
@@ -729,19 +734,22 @@ __device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobR
 }
 
 // Bin-averaged Gaussian (Doppler and the fixed-HWHM Gaussian) staged like
-// the region-I loop: per batch of 64 lines lane k writes row k (cf, A, B, Kg
-// | h, ib, ie); each line reads its row with two broadcast ds_read_b128 and
-// takes its window into SGPRs (readfirstlane).  The bins of neighbouring
-// points share an edge: point j spans x_j + h .. x_j - h (x falls by A = 2h
-// per point), so each lane evaluates erfc once, at its point's far edge
-// x_j - h, and takes the near edge from the lane before (DPP wave shift by
-// one; lane 0 from lane 63 of the slice before, or directly at a line's
-// first evaluated slice).  With F = erfc(|e|):
+// the region-I loop.  The bins of neighbouring points share an edge: point j
+// spans x_j + h .. x_j - h (x falls by A = 2h per point), so each lane
+// evaluates erfc once, at its point's far edge x_j - h, and takes the near
+// edge from the lane before (DPP wave shift by one).  Lane 0's near edge in
+// slice q is the far edge of point w0 + 64q - 1, E_q: per batch of 64 lines,
+// lane k writes row k (cf, A, B, Kg | h, ib, ie, - | E_q, F_q for every
+// slice), computing E_q with the very expression lane 63 of slice q - 1
+// evaluates, so each shared edge is one FP32 value and a line's bins
+// telescope exactly.  The slices of a line are then independent (no value
+// passes from one slice to the next).  With F = erfc(|e|):
 //   both edges >= 0:  erf(a) - erf(b) = F(b) - F(a)
 //   both edges <  0:  erf(a) - erf(b) = F(a) - F(b)
 //   a >= 0 > b:       erf(a) - erf(b) = 2 - F(a) - F(b)
 // (no cancellation beyond that of the bin itself, as in bin_erf_f32).
-constexpr int kGaussRow = 8;
+template <int PPL>
+constexpr int gauss_row() { return 8 + (2 * PPL + 3) / 4 * 4; }
 
 __device__ __forceinline__ float wave_shr1(float v, float lane0) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(v),
@@ -758,6 +766,7 @@ template <int PPL>
 __device__ __forceinline__ void gauss_staged(TileAcc<PPL>& t, GlobPos gpos, GlobRec grec,
                                              GlobRecX grecx, int a, int b, int w0, int lane,
                                              float* xs) {
+    constexpr int R = gauss_row<PPL>();
     if (a >= b) return;
     int li = min(a + lane, b - 1);
     LinePos pn = gpos[li];
@@ -771,7 +780,7 @@ __device__ __forceinline__ void gauss_staged(TileAcc<PPL>& t, GlobPos gpos, Glob
         pn = gpos[li];
         rn = grec[li];
         xn = grecx[li];
-        float* row = xs + lane * kGaussRow;
+        float* row = xs + lane * R;
         row[0] = rc.cf;
         row[1] = rc.A;
         row[2] = rc.B;
@@ -779,39 +788,31 @@ __device__ __forceinline__ void gauss_staged(TileAcc<PPL>& t, GlobPos gpos, Glob
         row[4] = xc.y;
         row[5] = __int_as_float(pc.ib);
         row[6] = __int_as_float(pc.ie);
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) {
+            const float e = fmaf(rc.cf - (float)(w0 + GX_WAVE * q - 1), rc.A, rc.B) - xc.y;
+            row[8 + 2 * q] = e;
+            row[9 + 2 * q] = erfcf(fabsf(e));
+        }
         const int nb = min(GX_WAVE, b - base);
         for (int k = 0; k < nb; ++k) {
-            const float* rk = xs + k * kGaussRow;
+            const float* rk = xs + k * R;
             const float cf = rk[0], A = rk[1], B = rk[2], Kg = rk[3], h = rk[4];
             LinePos lp;
             lp.ib = __builtin_amdgcn_readfirstlane(__float_as_int(rk[5]));
             lp.ie = __builtin_amdgcn_readfirstlane(__float_as_int(rk[6]));
-            bool have = false;  // the slice before was evaluated
-            float prevE = 0.0f, prevF = 0.0f;
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
                 const int st = slice_in_window<kMaskBoth>(w0 + GX_WAVE * q, lp);
-                if (st == 0) {
-                    have = false;
-                    continue;
-                }
+                if (st == 0) continue;
                 const float x = fmaf(cf - t.jf[q], A, B);
                 const float eb = x - h;
                 const float Fb = erfcf(fabsf(eb));
-                float ea, Fa;
-                if (have) {
-                    ea = wave_shr1(eb, prevE);
-                    Fa = wave_shr1(Fb, prevF);
-                } else {
-                    ea = x + h;
-                    Fa = erfcf(fabsf(ea));
-                }
+                const float ea = wave_shr1(eb, rk[8 + 2 * q]);
+                const float Fa = wave_shr1(Fb, rk[9 + 2 * q]);
                 const float v = bin_from_edges(ea, Fa, eb, Fb);
                 t.part[q] = fmaf(Kg, (st == 2 || in_window<kMaskBoth>(t.jj[q], lp)) ? v : 0.0f,
                                  t.part[q]);
-                prevE = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(eb), GX_WAVE - 1));
-                prevF = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fb), GX_WAVE - 1));
-                have = true;
             }
             t.tick(1);
         }
@@ -1092,13 +1093,16 @@ template <int SPLIT>
 constexpr int block_threads() { return SPLIT > 4 ? SPLIT * GX_WAVE : GX_WG; }
 
 // the SPLIT waves of the workgroup take the item's shares group * SPLIT + wv
-// of its lines; their FP64 partial sums are added in wave order through LDS
+// of its lines; their FP64 partial sums are added in wave order through LDS.
+// red shares its LDS with the waves' staging rows (xs): it is written only
+// once every wave of the workgroup has left its line loops.
 template <int PROF, int PPL, int SPLIT>
 __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb,
                                                double inv_scale, int lane, int wv,
                                                double* red, float* xs) {
     TileAcc<PPL> t;
     sum_share<PROF, PPL>(t, it, jb, lane, it.group * SPLIT + wv, max(it.ngroups, 1) * SPLIT, xs);
+    __syncthreads();  // the staging rows are free
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
         red[(size_t)wv * GX_WAVE * PPL + GX_WAVE * q + lane] = (double)t.sum[q] - (double)t.cmp[q];
@@ -1109,7 +1113,7 @@ __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb
         for (int w = 1; w < SPLIT; ++w) v += red[(size_t)w * GX_WAVE * PPL + k];
         store_tile<PPL>(it, jb, k, v * inv_scale);
     }
-    __syncthreads();  // red is reused by the next item
+    __syncthreads();  // red (the staging rows) is reused by the next item
 }
 
 // block b runs on XCD b % GX_XCDS; map the blocks of one XCD to one
@@ -1129,25 +1133,34 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // grid order, split into GX_XCDS cost-balanced contiguous ranges), then from
 // the other queues.  xq = GX_XCDS counters (zero at launch), xqb = the
 // GX_XCDS + 1 queue bounds.
-// amdgpu_waves_per_eu(7): at 4 points per lane the kernel takes 86 VGPRs
-// unconstrained (5 waves per SIMD); held to 72 (7 waves) it spills two values
-// outside its loops and runs 4 % faster, the extra waves hiding the scalar
-// and LDS load latency of the line loops.  Wider tiles keep their registers.
+// Waves per SIMD the register allocation aims at.  7 (72 VGPRs): at 4 points
+// per lane the kernel takes 86 VGPRs unconstrained (5 waves per SIMD); held
+// to 72 it spills two values outside its loops and runs 4 % faster, the
+// extra waves hiding the scalar and LDS load latency of the line loops.
+// 8-point tiles need up to ~150 VGPRs and 4-point tiles with 16 waves per
+// workgroup ~100: held to 72 they spill inside the line loops, so they aim at
+// 3 and 4 waves.
+template <int PPL, int SPLIT>
+constexpr int waves_target() { return PPL >= 8 ? 3 : ((PPL == 4 && SPLIT == 16) ? 4 : 7); }
+
 template <int PROF, int PPL, int SPLIT>
-__global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves_per_eu(7))) void k_profile_tiles(const WorkItem* __restrict__ items,
+__global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves_per_eu(waves_target<PPL, SPLIT>()))) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
                                                          const unsigned long long* __restrict__ scale_bits,
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
-    __shared__ double red[SPLIT > 1 ? SPLIT * GX_WAVE * PPL : 1];
-    // per wave, 64 rows of stage_row<PPL>() words: the staged rows of one
-    // batch of 64 region-I lines, or at PPL 1 the W4 candidate scan of one
-    // batch
-    __shared__ float xs_all[block_threads<SPLIT>() * wave_row<PPL>()];
+    // per wave, 64 rows of wave_row() words: the staged rows of one batch of
+    // 64 lines, or at PPL 1 the W4 candidate scan of one batch; after the
+    // line loops of a split item, the waves' FP64 partial sums (red)
+    constexpr int kXsWords = block_threads<SPLIT>() * wave_row<PROF, PPL>();
+    constexpr int kRedWords = SPLIT > 1 ? 2 * SPLIT * GX_WAVE * PPL : 0;
+    __shared__ __attribute__((aligned(16))) float smem[kXsWords > kRedWords ? kXsWords : kRedWords];
     __shared__ int grab;
+    float* xs_all = smem;
+    double* red = reinterpret_cast<double*>(smem);
     const int lane = threadIdx.x & (GX_WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float* xs = xs_all + wv * GX_WAVE * wave_row<PPL>();
+    float* xs = xs_all + wv * GX_WAVE * wave_row<PROF, PPL>();
     const int lb = mode == 3 ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     if (mode == 0 || mode == 3) {
         if (SPLIT == 1) {

@@ -88,6 +88,8 @@ class CrossSection {
     const std::vector<TemperaturePressure>& TP() const { return tp_; }
     const std::vector<double>& Result(int i) const { return results_.at((size_t)i); }
     double TotalIntensity() const { return total_intens_; }
+    // sum(sigma) * dfreq of the first (T,P), set by OutputProfile
+    double IntegratedIntensity() const { return integrated_; }
     int64_t LinesSelected() const { return selected_; }
     int64_t LinesRead() const { return read_; }
     int Chunks() const { return chunks_; }
@@ -108,7 +110,7 @@ class CrossSection {
     std::vector<int> h_g_;
     int npoints_ = 0;
     double dfreq_ = 0.0, start_ = 0.0, end_ = 0.0, hw_ = 100.0;
-    double min_nu_ = 0.0, max_nu_ = 0.0, total_intens_ = 0.0;
+    double min_nu_ = 0.0, max_nu_ = 0.0, total_intens_ = 0.0, integrated_ = 0.0;
     int64_t capacity_ = 0, ntrans_ = 0, selected_ = 0, read_ = 0;
     int chunks_ = 0;
     bool pressure_ = true;

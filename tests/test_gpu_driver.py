@@ -18,9 +18,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "gexs_amd", "gexs_xsec")
 
 
-def run_cli(text, cwd):
+def run_cli(text, cwd, raw=False):
     r = subprocess.run([CLI], input=text, capture_output=True, text=True, cwd=cwd, timeout=300)
     assert r.returncode == 0, r.stderr
+    if raw:
+        return r.stdout, r.stderr
     blocks = {}
     band = None
     for line in r.stdout.splitlines():
@@ -31,6 +33,25 @@ def run_cli(text, cwd):
         elif line.startswith("Our summed intensity"):
             band = line
     return {k: np.array(v) for k, v in blocks.items()}, band, r.stderr
+
+
+def band_sums(stderr):
+    """(summed line intensity, sum(sigma) * dnu) as gexs_xsec reports them at
+    full precision on stderr (the stdout report line prints 7 digits)."""
+    for line in stderr.splitlines():
+        if line.startswith("band strength:"):
+            w = line.split()
+            return float(w[3]), float(w[5])
+    raise AssertionError("no band strength line on stderr")
+
+
+def report_numbers(line):
+    """The three numbers of the report line (BaseProfile.cpp:289)."""
+    import re
+    m = re.match(r"Our summed intensity = +(\S+) compared to the integrated intensity = +(\S+) "
+                 r"with a difference of +(\S+) %$", line)
+    assert m, line
+    return tuple(float(v) for v in m.groups())
 
 
 def broadeners(J):
@@ -92,11 +113,86 @@ def test_exomol_driver_vs_oracle(O, tmp_path):
         Q = O.partition(L.E, L.g, T)
         p = O.params("voigt", T=T, P=P, Q=Q, cutoff=25.0, p_scale=1 / 1.01325)
         ref, _ = O.cross_section(p, 0.0, 10000.0 / N, N, *args, nthreads=8)
-        err = np.max(np.abs(out[k] - ref)) / ref.max()
+        e = np.max(np.abs(out[k] - ref)) / ref.max()
         bnd = abs(out[k].sum() - ref.sum()) / ref.sum()
-        print(f"T={T} P={P}: peak-rel {err:.2e} band {bnd:.2e}", flush=True)
-        assert err <= 1e-4 and bnd <= 1e-6
-    assert band is not None
+        print(f"T={T} P={P}: peak-rel {e:.2e} band {bnd:.2e}", flush=True)
+        assert e <= 1e-4 and bnd <= 1e-6
+        if k == 0:
+            ref0 = ref
+    # the band-strength report (BaseProfile.cpp:250-267, 283-291): the summed
+    # line intensity of the first (T,P) equals the oracle's to FP64 rounding,
+    # and sum(sigma) * dnu the oracle's within the band tolerance
+    T0 = tps[0][0]
+    total = O.total_intensity(T0, O.partition(L.E, L.g, T0), *args[:4])
+    summed, integ = band_sums(err)
+    assert abs(summed - total) <= 1e-12 * total, (summed, total)
+    want = ref0.sum() * (10000.0 / N)
+    assert abs(integ - want) <= 1e-6 * want, (integ, want)
+    a, b, pct = report_numbers(band)
+    assert abs(a - summed) <= 5e-7 * summed and abs(b - integ) <= 5e-7 * integ
+    assert abs(pct - (summed - integ) * 100.0 / summed) <= 5e-6
+
+
+def parse_output(text):
+    """Lines of gexs_xsec / OutputProfile text: ("band", numbers),
+    ("head", text), ("blank", "") or ("pt", k, frequency text, sigma)."""
+    rows = []
+    for line in text.splitlines():
+        if line.startswith("Our summed intensity"):
+            rows.append(("band", report_numbers(line)))
+        elif line.startswith("["):
+            k, rest = line[1:].split("]", 1)
+            f, sg = rest.split()
+            rows.append(("pt", int(k), f, float(sg)))
+        elif line.strip() == "":
+            rows.append(("blank", ""))
+        else:
+            rows.append(("head", line))
+    return rows
+
+
+@pytest.mark.parametrize("precision", ["fp32c", "fp64"])
+def test_golden_text_output(O, tmp_path, precision):
+    """gexs_xsec on a tiny ExoMol list (tests/golden/xsec_tiny, written by
+    make_xsec_golden.py from the reference's own ComputeVoigtTP and
+    compute_partition, compiled from its sources) under the reference CPU
+    conventions.  The output has the reference's layout line for line
+    (OutputProfile, BaseProfile.cpp:269-301): the report line, one header per
+    (T,P), one "[i] nu sigma" line per point with the same frequency text;
+    sigma within 1e-4 of the peak and 1e-6 in band strength (FP32C), or
+    2e-8 of the peak (FP64: the 9 digits of the text); the summed line
+    intensity as printed."""
+    import shutil
+    gold = os.path.join(ROOT, "tests", "golden", "xsec_tiny")
+    for f in ("tiny.states", "tiny.trans"):
+        shutil.copy(os.path.join(gold, f), tmp_path / f)
+    cfg = open(os.path.join(gold, "input.txt")).read()
+    if precision == "fp64":
+        cfg = cfg.replace("PROFILE VOIGT", "PROFILE VOIGT\nPRECISION fp64")
+    out, err = run_cli(cfg, tmp_path, raw=True)
+    assert "selected 33," in err
+    got = parse_output(out)
+    want = parse_output(open(os.path.join(gold, "expected.txt")).read())
+    assert len(got) == len(want)
+    assert [r[0] for r in got] == [r[0] for r in want]
+    for g, w in zip(got, want):
+        if g[0] == "head":
+            assert g == w
+        elif g[0] == "pt":
+            assert g[1:3] == w[1:3]
+    for k in (0, 1):
+        gs = np.array([r[3] for r in got if r[0] == "pt" and r[1] == k])
+        ws = np.array([r[3] for r in want if r[0] == "pt" and r[1] == k])
+        e = np.max(np.abs(gs - ws)) / ws.max()
+        b = abs(gs.sum() - ws.sum()) / ws.sum()
+        print(f"{precision} block {k}: peak-rel {e:.2e} band {b:.2e}", flush=True)
+        if precision == "fp64":
+            assert e <= 2e-8
+        else:
+            assert e <= 1e-4 and b <= 1e-6
+    (a, b_, _), (wa, wb, _) = got[0][1], want[0][1]
+    assert abs(a - wa) <= 5e-7 * wa  # the same FP64 sum over the same lines, at print precision
+    assert abs(b_ - wb) <= (1e-6 if precision == "fp32c" else 1e-8) * wb + 5e-7 * wb
 
 
 def par_record(nu, S, A, gair, gself, E, nair, gup):
