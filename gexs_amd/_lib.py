@@ -25,7 +25,7 @@ EXPORTS = [
     "gexs_transfer_results", "gexs_ready", "gexs_sync", "gexs_last_error", "gexs_last_stats",
     "gexs_timing", "gexs_line_windows", "gexs_pass_lines", "gexs_pass_items", "gexs_set_tuning", "gexs_load_list", "gexs_line_count",
     "gexs_partition", "gexs_run_batch", "gexs_batch_device_out",
-    "gexs_set_launch", "gexs_autotune",
+    "gexs_set_launch", "gexs_autotune", "gexs_run_fixed",
     "gexs_serve_start", "gexs_submit", "gexs_poll", "gexs_wait", "gexs_release",
     "gexs_serve_stop",
 ]
@@ -74,6 +74,15 @@ class Tuning(C.Structure):
     _fields_ = [("points_per_lane", C.c_int32), ("order", C.c_int32),
                 ("persistent_blocks", C.c_int32), ("waves_per_tile", C.c_int32),
                 ("kernel_ms", C.c_double)]
+
+
+class LinesDev(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nu", C.c_void_p), ("E", C.c_void_p), ("A", C.c_void_p),
+                ("g", C.c_void_p), ("gamma", C.c_void_p), ("nexp", C.c_void_p)]
+
+
+FIXED_MAX_LINES = 1 << 20
+FIXED_MAX_TILES = 1 << 17
 
 
 class Request(C.Structure):
@@ -129,6 +138,8 @@ def load():
         "gexs_set_launch": (i32, [P, i32, i32]),
         "gexs_autotune": (i32, [P, C.POINTER(TP), i32, C.POINTER(Grid), C.POINTER(Tuning),
                                 C.POINTER(Tuning), i32, C.POINTER(i32)]),
+        "gexs_run_fixed": (i32, [P, C.POINTER(LinesDev), C.POINTER(TP), C.POINTER(Grid), P,
+                                 C.POINTER(i64)]),
         "gexs_serve_start": (i32, [P, i32, i64]),
         "gexs_submit": (i32, [P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
         "gexs_poll": (i32, [P, C.c_uint64, C.POINTER(Result)]),

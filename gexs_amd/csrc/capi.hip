@@ -115,6 +115,45 @@ struct PosKey {
     }
 };
 
+// ------------------------------------------------------ no-allocation pass
+// gexs_run_fixed keeps every intermediate of its pass in one file-scope
+// device array (each GPU the library runs on holds its own copy until the
+// process ends): the pass tables and counters, then per line its window
+// (LinePos), FP32 record (VRec, VRecX), then per tile its work item.
+constexpr int64_t kFixedMaxLines = GEXS_FIXED_MAX_LINES;
+constexpr int64_t kFixedMaxTiles = GEXS_FIXED_MAX_TILES;
+constexpr size_t kFixedTabOff = 0;     // TPParams | JobDesc | Job (256-B aligned each)
+constexpr size_t kFixedCntOff = 1024;  // xq (8 x u32) | scale | neval | kmax: 7 x u64
+constexpr int kFixedCntWords = 7;
+constexpr size_t kFixedPosOff = 2048;
+constexpr size_t kFixedRecOff = kFixedPosOff + (size_t)kFixedMaxLines * sizeof(LinePos);
+constexpr size_t kFixedRecxOff = kFixedRecOff + (size_t)kFixedMaxLines * sizeof(VRec);
+constexpr size_t kFixedItemsOff = kFixedRecxOff + (size_t)kFixedMaxLines * sizeof(VRecX);
+constexpr size_t kFixedBytes = kFixedItemsOff + (size_t)kFixedMaxTiles * sizeof(WorkItem);
+static_assert(3 * 256 <= kFixedCntOff, "pass tables overlap the counters");
+
+}  // namespace
+
+static __device__ __attribute__((aligned(256))) unsigned char scratch[kFixedBytes];
+
+namespace {
+// device address of `scratch` per device (hipGetSymbolAddress, taken once per
+// device with it current; the array's host-side name is not a device address)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// called by gexs_create with `device` current
+void scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
 }  // namespace
 
 struct gexs_ctx {
@@ -615,6 +654,7 @@ int gexs_create(int device, int profile, gexs_ctx** out) {
         return GEXS_EHIP;
     }
     c->tcount = 0;
+    scratch_init(device);  // the device is current (set_device above)
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -1303,6 +1343,90 @@ int gexs_autotune(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* grid
     c->use_graph = graph;
     if (best) *best = top;
     if (n_report) *n_report = nrep;
+    return GEXS_OK;
+}
+
+// ------------------------------------------------------ no-allocation pass
+int gexs_run_fixed(gexs_ctx* c, const gexs_lines_dev* L, const gexs_tp* tp, const gexs_grid* grid,
+                   double* d_out, int64_t* n_eval) {
+    if (!c || !L || !tp || !grid || !d_out) return GEXS_EINVAL;
+    if (!c->have_k) return c->fail(GEXS_ESTATE, "run_fixed before init_constants");
+    if (c->k.precision != GEXS_FP32C) return c->fail(GEXS_EINVAL, "run_fixed runs the FP32C path only");
+    const int64_t n = L->n;
+    if (n < 0 || (n > 0 && (!L->nu || !L->E || !L->A || !L->g))) return GEXS_EINVAL;
+    const bool voigt = c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD;
+    if (voigt && n > 0 && (!L->gamma || !L->nexp))
+        return c->fail(GEXS_EINVAL, "Voigt profiles need gamma and n per line");
+    if (n > kFixedMaxLines)
+        return c->fail(GEXS_EINVAL, "run_fixed: " + std::to_string((long long)n) +
+                                        " lines exceed its maximum of " +
+                                        std::to_string((long long)kFixedMaxLines));
+    if (grid->npoints <= 0 || !(grid->end > grid->start)) return GEXS_EINVAL;
+    const int span = GX_WAVE * c->ppl;
+    const int64_t ntiles = ((int64_t)grid->npoints + span - 1) / span;
+    if (ntiles > kFixedMaxTiles)
+        return c->fail(GEXS_EINVAL, "run_fixed: " + std::to_string((long long)ntiles) + " tiles of " +
+                                        std::to_string(span) + " points exceed its maximum of " +
+                                        std::to_string((long long)kFixedMaxTiles));
+    if (!(tp->T > 0.0) || !(tp->P >= 0.0) || !(tp->Q > 0.0))
+        return c->fail(GEXS_EINVAL, "run_fixed needs T > 0, P >= 0 and Q > 0");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    char* base = c->device < kMaxDevices ? (char*)g_scratch_ptr[c->device] : nullptr;
+    if (!base) {
+        const hipError_t e = c->device < kMaxDevices ? g_scratch_err[c->device] : hipErrorInvalidDevice;
+        return c->hipfail(e == hipSuccess ? hipErrorInvalidSymbol : e, "hipGetSymbolAddress(scratch)");
+    }
+    TPParams* dtp = (TPParams*)(base + kFixedTabOff);
+    JobDesc* djd = (JobDesc*)(base + kFixedTabOff + 256);
+    Job* djob = (Job*)(base + kFixedTabOff + 512);
+    unsigned long long* cnt = (unsigned long long*)(base + kFixedCntOff);
+    unsigned int* xq = (unsigned int*)cnt;          // 8 queue counters (words 0-3)
+    unsigned long long* scale = cnt + 4;
+    unsigned long long* neval = cnt + 5;
+    int* kmax = (int*)(cnt + 6);
+    LinePos* pos = (LinePos*)(base + kFixedPosOff);
+    VRec* rec = (VRec*)(base + kFixedRecOff);
+    VRecX* recx = (VRecX*)(base + kFixedRecxOff);
+    WorkItem* items = (WorkItem*)(base + kFixedItemsOff);
+    const double dfreq = (grid->end - grid->start) / (double)grid->npoints;  // BaseProfile.cpp:10
+    const TPParams t = make_tp(c, tp->T, tp->P, tp->Q, grid->start, dfreq, 0, grid->npoints);
+    JobDesc d{};
+    d.n = (int32_t)n;
+    d.ntiles = (int32_t)ntiles;
+    d.q_slot = -1;
+    d.groups = 1;
+    Job jb{};
+    jb.pos = pos;
+    jb.rec = rec;
+    jb.recx = recx;
+    jb.out = d_out;
+    jb.n = (int32_t)n;
+    jb.profile = c->profile;
+    jb.accumulate = 0;  // every point of the grid is written once
+    hipStream_t s = c->stream;
+    HIPCHK(c, gxk::write_tables(s, t, d, jb, dtp, djd, djob, cnt, kFixedCntWords));
+    if (n > 0) {
+        HIPCHK(c, gxk::line_pos(s, L->nu, djd, 1, n, dtp, pos));
+        HIPCHK(c, gxk::line_max(s, L->nu, L->E, L->A, L->g, pos, djd, 1, n, dtp, scale));
+        HIPCHK(c, gxk::line_rec(s, L->nu, L->E, L->A, L->g, voigt ? L->gamma : nullptr,
+                                voigt ? L->nexp : nullptr, pos, djd, 1, n, dtp, scale, rec, recx,
+                                kmax, neval));
+    }
+    HIPCHK(c, gxk::tiles(s, pos, djd, 1, (int)ntiles, dtp, span, items, nullptr, nullptr));
+    HIPCHK(c, gxk::segments(s, items, items, (int)ntiles, djd, pos, kmax, c->profile));
+    // launch shapes that need no host-built queue bounds or tile sort: one
+    // slot per tile (grid or dispatch order) or persistent workgroups on one
+    // counter; every shape of one tile width and split gives the same bits
+    const int mode = (c->persistent_blocks > 0 && c->order != 2) ? 1 : (c->order == 0 ? 0 : 3);
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, items, (int)ntiles, djob, scale, xq,
+                                 xq, mode, launch_blocks(c)));
+    if (n_eval) {
+        unsigned long long ne = 0;
+        HIPCHK(c, hipMemcpyAsync(&ne, neval, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        *n_eval = (int64_t)ne;
+    }
     return GEXS_OK;
 }
 

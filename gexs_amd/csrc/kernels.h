@@ -57,6 +57,11 @@ hipError_t partition(hipStream_t s, const double* E, const int* g, int64_t n, co
                      int nT, double* partial, int nblk, double* dQ, const JobDesc* jd = nullptr,
                      int njobs = 0, TPParams* tp = nullptr, unsigned long long* clear = nullptr,
                      int nclear = 0);
+// one job's tables from the host values (kernel arguments) into device
+// memory, and clear[0 .. nclear) zeroed, in one launch
+hipError_t write_tables(hipStream_t s, const TPParams& tp, const JobDesc& jd, const Job& jb,
+                        TPParams* dtp, JobDesc* djd, Job* djob, unsigned long long* clear,
+                        int nclear);
 hipError_t fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v);
 hipError_t fill_f64(hipStream_t s, double* p, int64_t n, double v);
 hipError_t add_f64(hipStream_t s, double* dst, const double* src, int64_t n);

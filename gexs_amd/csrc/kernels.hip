@@ -1359,6 +1359,20 @@ __global__ __launch_bounds__(256) void k_partition_final(const double* __restric
 }
 
 // ------------------------------------------------------------- small kernels
+// One job's tables written from kernel arguments and nclear 64-bit counters
+// zeroed: the set-up of a pass whose tables live in device memory the host
+// cannot stage into.
+__global__ void k_write_tables(TPParams tp, JobDesc jd, Job jb, TPParams* __restrict__ dtp,
+                               JobDesc* __restrict__ djd, Job* __restrict__ djob,
+                               unsigned long long* __restrict__ clear, int nclear) {
+    if (threadIdx.x == 0) {
+        *dtp = tp;
+        *djd = jd;
+        *djob = jb;
+    }
+    for (int i = threadIdx.x; i < nclear; i += blockDim.x) clear[i] = 0ull;
+}
+
 __global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1549,6 +1563,14 @@ hipError_t partition(hipStream_t s, const double* E, const int* g, int64_t n, co
                        nblk);
     hipLaunchKernelGGL(k_partition_final, dim3(nT), dim3(256), 0, s, partial, nblk, dQ, jd, njobs, tp,
                        clear, nclear);
+    return hipGetLastError();
+}
+
+hipError_t write_tables(hipStream_t s, const TPParams& tp, const JobDesc& jd, const Job& jb,
+                        TPParams* dtp, JobDesc* djd, Job* djob, unsigned long long* clear,
+                        int nclear) {
+    hipLaunchKernelGGL(k_write_tables, dim3(1), dim3(64), 0, s, tp, jd, jb, dtp, djd, djob, clear,
+                       nclear);
     return hipGetLastError();
 }
 

@@ -245,6 +245,20 @@ class HipManager:
         self._chk(self._L.gexs_run_batch(self._h, arr, len(tps), C.byref(grid), _ptr(out)))
         return out
 
+    def run_fixed(self, lines_dev, T, P, Q, start, end, npoints, d_out, n_eval=True):
+        """gexs_run_fixed: one cross-section of lines held in device memory
+        (lines_dev = (n, nu, E, A, g, gamma, nexp) device addresses, nu
+        ascending) into the device grid at d_out, with no device allocation.
+        Returns N_eval (waits) or None (asynchronous) when n_eval is False."""
+        L = _lib.LinesDev(*[int(v) if v else None for v in lines_dev])
+        tp = _lib.TP(float(T), float(P), float(Q))
+        grid = _lib.Grid(float(start), float(end), int(npoints), 0)
+        ne = C.c_int64()
+        self._chk(self._L.gexs_run_fixed(self._h, C.byref(L), C.byref(tp), C.byref(grid),
+                                         C.c_void_p(int(d_out)),
+                                         C.byref(ne) if n_eval else None))
+        return ne.value if n_eval else None
+
     # ------------------------------------------------------------- serving
     def serve_start(self, max_inflight=2, mem_budget=4 << 30):
         """Start the continuous-batching server over the resident list."""

@@ -243,6 +243,43 @@ typedef struct gexs_tuning {
 int gexs_autotune(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* grid,
                   gexs_tuning* best, gexs_tuning* report, int report_cap, int* n_report);
 
+/* ------------------------------------------------------ no-allocation pass
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gexs_run_batch for one (T,P) over lines the caller holds
+ * in device memory, writing a device grid the caller owns, with every
+ * intermediate (pass tables, counters, per-line windows and FP32 records,
+ * per-tile work items) in one file-scope device array of the library,
+ * about 62 MiB, which each GPU the library runs on holds until the process
+ * ends.  The call makes no hipMalloc, hipFree or other allocation.
+ *   - lines: the pass's lines (the caller's selection), ascending in nu,
+ *     at most GEXS_FIXED_MAX_LINES; gamma / nexp may be NULL for Doppler and
+ *     Gaussian profiles.
+ *   - grid: at most GEXS_FIXED_MAX_TILES tiles of 64 * points_per_lane
+ *     points; d_out receives its npoints doubles (every point is written).
+ *   - tp: T > 0, P >= 0 and Q > 0 (Q is not computed here).
+ *   - FP32C precision, the ctx's constants and tuning; tile orders that need
+ *     a host-built queue or a tile sort (heaviest first, per-XCD queues) run
+ *     in dispatch order, which gives the same bits.
+ *   - Asynchronous on the ctx stream when n_eval is NULL; otherwise waits and
+ *     stores the pass's N_eval in *n_eval.
+ * Every call on one GPU uses that GPU's array: calls that overlap on one GPU
+ * (from any ctx or thread) race and must be serialised by the caller.
+ * Sizes beyond the maxima are refused with GEXS_EINVAL before any launch. */
+#define GEXS_FIXED_MAX_LINES (1 << 20)
+#define GEXS_FIXED_MAX_TILES (1 << 17)
+typedef struct gexs_lines_dev {
+    int64_t n;
+    const double* nu;     /* cm^-1, ascending */
+    const double* E;      /* lower-state energy E'', cm^-1 */
+    const double* A;      /* Einstein A, s^-1 */
+    const int32_t* g;     /* upper-state degeneracy g' */
+    const double* gamma;  /* Lorentz gamma_ref */
+    const double* nexp;   /* temperature exponent */
+} gexs_lines_dev;
+int gexs_run_fixed(gexs_ctx* ctx, const gexs_lines_dev* lines, const gexs_tp* tp,
+                   const gexs_grid* grid, double* d_out, int64_t* n_eval);
+
 /* ---------------------------------------------------------------- serving
  * Continuous batching over the resident list: requests are admitted by HBM
  * footprint as they arrive and run while others are on the device.
