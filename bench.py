@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="voigt_1e6")
+    ap.add_argument("--profile", default=None,
+                    help="profile of the headline workload (default: the workload's own)")
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0: tuned default)")
     ap.add_argument("--order", type=int, default=1)
     ap.add_argument("--split", type=int, default=1, help="waves per tile (1 or 4)")
@@ -564,7 +566,7 @@ def main():
     from gexs_amd.manager import HipManager
 
     log(f"[bench] generating {a.workload}")
-    r = execute_bench(a, a.workload, log_candidates=True)
+    r = execute_bench(a, a.workload, profile=a.profile, log_candidates=True)
     spec, prof, lst = r["spec"], r["profile"], r["lst"]
     T, P, Q = r["T"], r["P"], r["Q"]
     start, end, N = spec["grid"]

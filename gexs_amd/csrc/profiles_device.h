@@ -240,19 +240,25 @@ __device__ inline float voigt916_f32(float x, float y) {
     return k;
 }
 
-// 11-point Gauss-Hermite quadrature of K (A&S 25.4.46, table 25.10)
+// 11-point Gauss-Hermite quadrature of K (A&S 25.4.46, table 25.10): the
+// +-t_k pair sums to 2 w_k (r + t_k^2) / ((r + t_k^2)^2 - 4 x^2 t_k^2),
+// r = x^2 + y^2.  It is used for r >= 100, where every denominator is at
+// least (r - t_k^2)^2 > 0, and each quotient takes the hardware reciprocal
+// (1 ulp) instead of a correctly rounded division (~10 instructions).
 __device__ __forceinline__ float gh11_f32(float x, float y) {
     const float t2[5] = {0.4313988071478515f, 1.7597536984236963f, 4.104465362828315f,
                          7.746703779542555f, 13.457678352057581f};
-    const float w[5] = {0.42935975235612495f, 0.11722787516770851f, 0.011911395444911507f,
-                        0.00034681946632334544f, 1.4395603937142596e-06f};
-    float r = x * x + y * y;
-    float acc = 0.6547592869145916f / r;
-    float fx2 = 4.0f * x * x;
+    const float w2[5] = {0.8587195047122499f, 0.23445575033541702f, 0.023822790889823014f,
+                         0.0006936389326466909f, 2.879120787428519e-06f};
+    const float x2 = x * x;
+    const float r = fmaf(y, y, x2);
+    const float fx2 = 4.0f * x2;
+    float acc = 0.6547592869145916f * __builtin_amdgcn_rcpf(r);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        float qq = r + t2[k];
-        acc += w[k] * 2.0f * qq / (qq * qq - fx2 * t2[k]);
+        const float qq = r + t2[k];
+        const float den = fmaf(qq, qq, -fx2 * t2[k]);
+        acc = fmaf(w2[k] * qq, __builtin_amdgcn_rcpf(den), acc);
     }
     return acc * y * 0.3183098861837907f;
 }
