@@ -212,6 +212,7 @@ struct gexs_ctx {
     PassBuf pb[2];
     int pbuf = 0;
     hipEvent_t geom = nullptr;  // on `stream` after positions and tiles were (re)built
+    bool geom_new = false;      // geom recorded since the pre-pass stream last waited on it
     int cus = 256;  // compute units of the device
     PosKey pos_key;
     PosKey items_key;
@@ -375,6 +376,7 @@ int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
     c->pos_key = key;
     c->items_key = PosKey();  // tiles depend on positions
     HIPCHK(c, hipEventRecord(c->geom, c->stream));
+    c->geom_new = true;
     return GEXS_OK;
 }
 
@@ -499,6 +501,7 @@ int ensure_items(gexs_ctx* c, int njobs, const TPParams* htp, const JobDesc* hjd
     c->items_njobs = njobs;
     c->n_items = total;
     HIPCHK(c, hipEventRecord(c->geom, c->stream));
+    c->geom_new = true;
     return GEXS_OK;
 }
 
@@ -546,10 +549,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, B.recx.ensure((size_t)std::max(n, 1) * sizeof(VRecX)));
     HIPCHK(c, B.items.ensure((size_t)std::max(c->n_items, 1) * sizeof(WorkItem)));
     HIPCHK(c, c->counter.ensure(256));
-    std::vector<char> h(t.bytes, 0);
-    *(TPParams*)h.data() = tp;
-    *(JobDesc*)(h.data() + t.jd_off) = d;
-    Job& jb = *(Job*)(h.data() + t.jobs_off);
+    Job jb{};
     jb.pos = c->pos.as<LinePos>();
     jb.rec = B.rec.as<VRec>();
     jb.recx = B.recx.as<VRecX>();
@@ -559,11 +559,16 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     jb.scale_slot = 0;
     jb.accumulate = 1;
     hipStream_t P = c->pre;
-    HIPCHK(c, hipStreamWaitEvent(P, c->geom, 0));  // positions and tiles
+    if (c->geom_new) {  // positions and tiles were rebuilt on the main stream
+        HIPCHK(c, hipStreamWaitEvent(P, c->geom, 0));
+        c->geom_new = false;
+    }
     if (B.used) HIPCHK(c, hipStreamWaitEvent(P, B.prof_done, 0));  // buffer set free
     HIPCHK(c, hipEventRecord(c->ev_start, P));
-    HIPCHK(c, c->staging.upload(B.tab.p, h.data(), t.bytes, P));
-    HIPCHK(c, gxk::fill_u64(P, (unsigned long long*)B.small.p, t.zero_u64, 0ull));
+    // the pass's tables go in as kernel arguments of the set-up kernel, which
+    // also clears the pass counters: no host staging copy per pass
+    HIPCHK(c, gxk::write_tables(P, tp, d, jb, t.tp, t.jd, t.jobs, (unsigned long long*)B.small.p,
+                                (int)t.zero_u64));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
     HIPCHK(c, gxk::line_max(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
@@ -584,7 +589,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split,
                                  voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>(), c->n_items, t.jobs,
                                  t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
-                                 launch_blocks(c)));
+                                 launch_blocks(c), &jb));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     HIPCHK(c, hipEventRecord(B.prof_done, s));
     B.used = true;
@@ -1420,7 +1425,7 @@ int gexs_run_fixed(gexs_ctx* c, const gexs_lines_dev* L, const gexs_tp* tp, cons
     // counter; every shape of one tile width and split gives the same bits
     const int mode = (c->persistent_blocks > 0 && c->order != 2) ? 1 : (c->order == 0 ? 0 : 3);
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, items, (int)ntiles, djob, scale, xq,
-                                 xq, mode, launch_blocks(c)));
+                                 xq, mode, launch_blocks(c), &jb));
     if (n_eval) {
         unsigned long long ne = 0;
         HIPCHK(c, hipMemcpyAsync(&ne, neval, 8, hipMemcpyDeviceToHost, s));

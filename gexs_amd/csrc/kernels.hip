@@ -586,7 +586,7 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
 template <int PROF, int PPL, int MASK, bool DEFER = false>
 __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp, const VRec& r,
                                              const VRecX& rx, int w0, int w1) {
-    if (PROF == GX_PROF_VOIGT) {
+    if constexpr (PROF == GX_PROF_VOIGT) {
         const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
         if (kmin >= rx.k15) {
 #pragma unroll
@@ -614,19 +614,15 @@ __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp,
             }
         }
     } else {
+        static_assert(PROF == GX_PROF_VOIGT_QUAD, "bin-Gauss lines run gauss_staged");
 #pragma unroll
         for (int q = 0; q < PPL; ++q) {
             const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
             if (st == 0) continue;
             if (st == 2 || in_window<MASK>(t.jj[q], lp)) {
                 const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
-                float k;
-                if (PROF == GX_PROF_VOIGT_QUAD) {
-                    const float rr = x * x + rx.y * rx.y;
-                    k = rr < 100.0f ? gx::voigt916_f32(x, rx.y) : gx::gh11_f32(x, rx.y);
-                } else {
-                    k = gx::bin_erf_f32(x, rx.y);
-                }
+                const float rr = x * x + rx.y * rx.y;
+                const float k = rr < 100.0f ? gx::voigt916_f32(x, rx.y) : gx::gh11_f32(x, rx.y);
                 t.part[q] = fmaf(r.Kg, k, t.part[q]);
             }
         }
@@ -747,7 +743,7 @@ __device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobR
 //   both edges >= 0:  erf(a) - erf(b) = F(b) - F(a)
 //   both edges <  0:  erf(a) - erf(b) = F(a) - F(b)
 //   a >= 0 > b:       erf(a) - erf(b) = 2 - F(a) - F(b)
-// (no cancellation beyond that of the bin itself, as in bin_erf_f32).
+// (no cancellation beyond that of the bin itself).
 template <int PPL>
 constexpr int gauss_row() { return 8 + (2 * PPL + 3) / 4 * 4; }
 
@@ -757,9 +753,10 @@ __device__ __forceinline__ float wave_shr1(float v, float lane0) {
 }
 
 __device__ __forceinline__ float bin_from_edges(float ea, float Fa, float eb, float Fb) {
-    if (eb >= 0.0f) return Fb - Fa;
-    if (ea < 0.0f) return Fa - Fb;
-    return (2.0f - Fa) - Fb;
+    const float d = Fb - Fa;
+    const float across = (2.0f - Fa) - Fb;
+    const float v = ea < 0.0f ? -d : across;  // selects, not branches
+    return eb >= 0.0f ? d : v;
 }
 
 template <int PPL>
@@ -792,29 +789,38 @@ __device__ __forceinline__ void gauss_staged(TileAcc<PPL>& t, GlobPos gpos, Glob
         for (int q = 0; q < PPL; ++q) {
             const float e = fmaf(rc.cf - (float)(w0 + GX_WAVE * q - 1), rc.A, rc.B) - xc.y;
             row[8 + 2 * q] = e;
-            row[9 + 2 * q] = erfcf(fabsf(e));
+            row[9 + 2 * q] = gx::erfc_nonneg(fabsf(e));
         }
+        // Per line the window test is per lane (VALU compares into a
+        // select); the only scalar decision is whether a slice meets the
+        // window at all.  The partial sums are folded once per chunk of
+        // kChunk lines rather than counted line by line: with a bare scalar
+        // unit per CU shared by four SIMDs, per-line scalar work would bound
+        // this loop.
         const int nb = min(GX_WAVE, b - base);
-        for (int k = 0; k < nb; ++k) {
-            const float* rk = xs + k * R;
-            const float cf = rk[0], A = rk[1], B = rk[2], Kg = rk[3], h = rk[4];
-            LinePos lp;
-            lp.ib = __builtin_amdgcn_readfirstlane(__float_as_int(rk[5]));
-            lp.ie = __builtin_amdgcn_readfirstlane(__float_as_int(rk[6]));
+        if (t.n > 0) t.fold();
+        for (int k0 = 0; k0 < nb; k0 += kChunk) {
+            const int k1 = min(nb, k0 + kChunk);
+            for (int k = k0; k < k1; ++k) {
+                const float* rk = xs + k * R;
+                const float cf = rk[0], A = rk[1], B = rk[2], Kg = rk[3], h = rk[4];
+                const int ib = __builtin_amdgcn_readfirstlane(__float_as_int(rk[5]));
+                const int ie = __builtin_amdgcn_readfirstlane(__float_as_int(rk[6]));
 #pragma unroll
-            for (int q = 0; q < PPL; ++q) {
-                const int st = slice_in_window<kMaskBoth>(w0 + GX_WAVE * q, lp);
-                if (st == 0) continue;
-                const float x = fmaf(cf - t.jf[q], A, B);
-                const float eb = x - h;
-                const float Fb = erfcf(fabsf(eb));
-                const float ea = wave_shr1(eb, rk[8 + 2 * q]);
-                const float Fa = wave_shr1(Fb, rk[9 + 2 * q]);
-                const float v = bin_from_edges(ea, Fa, eb, Fb);
-                t.part[q] = fmaf(Kg, (st == 2 || in_window<kMaskBoth>(t.jj[q], lp)) ? v : 0.0f,
-                                 t.part[q]);
+                for (int q = 0; q < PPL; ++q) {
+                    const int j0 = w0 + GX_WAVE * q;
+                    if (j0 >= ie || j0 + GX_WAVE <= ib) continue;  // slice outside the window
+                    const float x = fmaf(cf - t.jf[q], A, B);
+                    const float eb = x - h;
+                    const float Fb = gx::erfc_nonneg(fabsf(eb));
+                    const float ea = wave_shr1(eb, rk[8 + 2 * q]);
+                    const float Fa = wave_shr1(Fb, rk[9 + 2 * q]);
+                    const float v = bin_from_edges(ea, Fa, eb, Fb);
+                    const bool in = t.jj[q] >= ib && t.jj[q] < ie;
+                    t.part[q] = fmaf(Kg, in ? v : 0.0f, t.part[q]);
+                }
             }
-            t.tick(1);
+            t.fold();
         }
     }
 }
@@ -967,7 +973,10 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         t.n = 0;
     }
     const int w0 = it.w0, w1 = it.w1;
-    if constexpr (PPL == 1) {
+    if constexpr (PROF == GX_PROF_DOPPLER) {
+        gauss_staged<PPL>(t, (GlobPos)jb.pos, (GlobRec)jb.rec, (GlobRecX)jb.recx, ca, cb, w0, lane,
+                          xs);
+    } else if constexpr (PPL == 1) {
         // region-I segments staged, the others by vector loads broadcast
         // per line; the wave's LDS block holds the staged X0 or the W4
         // candidate scan of batch_w4
@@ -976,6 +985,7 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         const GlobRec grec = (GlobRec)jb.rec;
         const GlobRecX grecx = (GlobRecX)jb.recx;
         if (PROF == GX_PROF_VOIGT && it.s1 < it.s4) {
+            static_assert(PROF == GX_PROF_VOIGT || PROF == GX_PROF_VOIGT_QUAD, "Voigt profiles");
             const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
             const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
             // the region-I segments through the staged loop (6 VALU per
@@ -995,7 +1005,7 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         } else {
             lines_batched<PROF, kMaskBoth>(t, gpos, grec, grecx, ca, cb, w0, w1, lane, incl);
         }
-    } else if (PROF == GX_PROF_VOIGT) {
+    } else if constexpr (PROF == GX_PROF_VOIGT) {
         const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
         const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
         if (it.s1 < it.s4) {
@@ -1019,9 +1029,6 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         } else {
             lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
         }
-    } else if (PROF == GX_PROF_DOPPLER) {
-        gauss_staged<PPL>(t, (GlobPos)jb.pos, (GlobRec)jb.rec, (GlobRecX)jb.recx, ca, cb, w0, lane,
-                          xs);
     } else {
         lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
     }
@@ -1074,12 +1081,14 @@ __device__ __forceinline__ void store_tile(const WorkItem& it, const Job& jb, in
         store_point(jb, it.w0 + k, v);
 }
 
-// one wave evaluates the whole item
+// one wave evaluates the whole item; the FP64 scale is read after the line
+// loops (it would otherwise hold two VGPRs through them)
 template <int PROF, int PPL>
-__device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb, double inv_scale,
-                                         int lane, float* xs) {
+__device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb,
+                                         const unsigned long long* scale_bits, int lane, float* xs) {
     TileAcc<PPL> t;
     sum_share<PROF, PPL>(t, it, jb, lane, it.group, max(it.ngroups, 1), xs);
+    const double inv_scale = 1.0 / scale_of(scale_bits[jb.scale_slot]);
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
         store_tile<PPL>(it, jb, GX_WAVE * q + lane,
@@ -1098,10 +1107,11 @@ constexpr int block_threads() { return SPLIT > 4 ? SPLIT * GX_WAVE : GX_WG; }
 // once every wave of the workgroup has left its line loops.
 template <int PROF, int PPL, int SPLIT>
 __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb,
-                                               double inv_scale, int lane, int wv,
-                                               double* red, float* xs) {
+                                               const unsigned long long* scale_bits, int lane,
+                                               int wv, double* red, float* xs) {
     TileAcc<PPL> t;
     sum_share<PROF, PPL>(t, it, jb, lane, it.group * SPLIT + wv, max(it.ngroups, 1) * SPLIT, xs);
+    const double inv_scale = 1.0 / scale_of(scale_bits[jb.scale_slot]);
     __syncthreads();  // the staging rows are free
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
@@ -1126,6 +1136,8 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 }  // namespace
 
+// jobs == nullptr: every item belongs to the one job passed by value (job0),
+// which spares each wave a dependent load of the job table.
 // mode 0: one item per wave (SPLIT 1) or per workgroup (SPLIT 4, 16), blocks in
 // XCD-contiguous order; 3: as 0 in dispatch order (consecutive items on
 // different XCDs); 1: persistent, items pulled from one counter; 2:
@@ -1146,6 +1158,7 @@ constexpr int waves_target() { return PPL >= 8 ? 3 : ((PPL == 4 && SPLIT == 16) 
 template <int PROF, int PPL, int SPLIT>
 __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves_per_eu(waves_target<PPL, SPLIT>()))) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
+                                                         Job job0,
                                                          const unsigned long long* __restrict__ scale_bits,
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
@@ -1167,14 +1180,13 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             const int idx = lb * kWavesPerBlock + wv;
             if (idx >= nitems) return;
             const WorkItem it = items[idx];
-            const Job jb = jobs[it.job];
-            run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane, xs);
+            const Job jb = jobs ? jobs[it.job] : job0;
+            run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
         } else {
             if (lb >= nitems) return;
             const WorkItem it = items[lb];
-            const Job jb = jobs[it.job];
-            run_item_split<PROF, PPL, SPLIT>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane,
-                                             wv, red, xs);
+            const Job jb = jobs ? jobs[it.job] : job0;
+            run_item_split<PROF, PPL, SPLIT>(it, jb, scale_bits, lane, wv, red, xs);
         }
         return;
     }
@@ -1198,12 +1210,11 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             }
             if (idx >= end) break;
             const WorkItem it = items[idx];
-            const Job jb = jobs[it.job];
+            const Job jb = jobs ? jobs[it.job] : job0;
             if (SPLIT == 1)
-                run_item<PROF, PPL>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]), lane, xs);
+                run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
             else
-                run_item_split<PROF, PPL, SPLIT>(it, jb, 1.0 / scale_of(scale_bits[jb.scale_slot]),
-                                                 lane, wv, red, xs);
+                run_item_split<PROF, PPL, SPLIT>(it, jb, scale_bits, lane, wv, red, xs);
         }
     }
 }
@@ -1467,55 +1478,63 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
 
 template <int PROF, int PPL, int SPLIT>
 static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitems,
-                                 const Job* jobs, const unsigned long long* scale_bits,
-                                 unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
+                                 const Job* jobs, const Job& job0,
+                                 const unsigned long long* scale_bits, unsigned int* xq,
+                                 const unsigned int* xqb, int mode, int blocks) {
     if (nitems <= 0) return hipSuccess;
     const int ipb = SPLIT == 1 ? kWavesPerBlock : 1;  // items per block
     const bool slots = mode == 0 || mode == 3;  // one launch slot per item
     const int nb = slots ? blocks_for(nitems, ipb)
                          : std::max(1, std::min(blocks, blocks_for(nitems, ipb)));
     hipLaunchKernelGGL((k_profile_tiles<PROF, PPL, SPLIT>), dim3(nb), dim3(block_threads<SPLIT>()), 0, s, items,
-                       nitems, jobs, scale_bits, xq, xqb, mode);
+                       nitems, jobs, job0, scale_bits, xq, xqb, mode);
     return hipGetLastError();
 }
 
 template <int PROF, int SPLIT>
 static hipError_t launch_ppl(hipStream_t s, int ppl, const WorkItem* items, int nitems,
-                             const Job* jobs, const unsigned long long* scale_bits,
+                             const Job* jobs, const Job& job0, const unsigned long long* scale_bits,
                              unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     switch (ppl) {
-        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
-        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
-        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
-        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
+        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
+        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
+        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <int PROF>
 static hipError_t launch_prof(hipStream_t s, int ppl, int split, const WorkItem* items,
-                              int nitems, const Job* jobs, const unsigned long long* scale_bits,
+                              int nitems, const Job* jobs, const Job& job0,
+                              const unsigned long long* scale_bits,
                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
-    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
-    if (split == 4) return launch_ppl<PROF, 4>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
+    if (split == 4) return launch_ppl<PROF, 4>(s, ppl, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
     if (split == 16)
-        return launch_ppl<PROF, 16>(s, ppl, items, nitems, jobs, scale_bits, xq, xqb, mode, blocks);
+        return launch_ppl<PROF, 16>(s, ppl, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
     return hipErrorInvalidValue;
 }
 
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,
-                         unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
+                         unsigned int* xq, const unsigned int* xqb, int mode, int blocks,
+                         const Job* job_value) {
+    Job job0{};
+    if (job_value) {  // one job, passed to the kernel by value
+        job0 = *job_value;
+        jobs = nullptr;
+    }
     switch (profile) {
         case GX_PROF_VOIGT:
-            return launch_prof<GX_PROF_VOIGT>(s, ppl, split, items, nitems, jobs, scale_bits, xq, xqb, mode,
+            return launch_prof<GX_PROF_VOIGT>(s, ppl, split, items, nitems, jobs, job0, scale_bits, xq, xqb, mode,
                                               blocks);
         case GX_PROF_VOIGT_QUAD:
-            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, scale_bits, xq, xqb,
+            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, job0, scale_bits, xq, xqb,
                                                    mode, blocks);
         case GX_PROF_DOPPLER:
         case GX_PROF_GAUSSIAN:
-            return launch_prof<GX_PROF_DOPPLER>(s, ppl, split, items, nitems, jobs, scale_bits, xq, xqb,
+            return launch_prof<GX_PROF_DOPPLER>(s, ppl, split, items, nitems, jobs, job0, scale_bits, xq, xqb,
                                                 mode, blocks);
         default:
             return hipErrorInvalidValue;
