@@ -156,6 +156,11 @@ void scratch_init(int device) {
 }
 }  // namespace
 
+// buffer sets of the execute path: the pre-pass of pass k writes set
+// k mod kPassSets once the profile kernel of pass k - kPassSets has read it,
+// so it may run during the profile kernels of the kPassSets - 1 passes before
+constexpr int kPassSets = 3;
+
 struct gexs_ctx {
     int device = 0;
     int profile = GEXS_VOIGT;
@@ -201,15 +206,15 @@ struct gexs_ctx {
     const WorkItem* last_items = nullptr;  // work items the last FP32C profile kernel read
 
     // execute path: the per-line pre-pass of a pass runs on `pre` into one of
-    // two buffer sets while the profile kernel of the previous pass runs on
-    // `stream` from the other set
+    // kPassSets buffer sets while the profile kernels of earlier passes run
+    // on `stream` from the others
     hipStream_t pre = nullptr;
     struct PassBuf {
         DevArr tab, small, rec, recx, items;
         hipEvent_t pre_done = nullptr, prof_done = nullptr;
         bool used = false;
     };
-    PassBuf pb[2];
+    PassBuf pb[kPassSets];
     int pbuf = 0;
     hipEvent_t geom = nullptr;  // on `stream` after positions and tiles were (re)built
     bool geom_new = false;      // geom recorded since the pre-pass stream last waited on it
@@ -521,8 +526,8 @@ int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
 }
 
 // The FP32C pass of one execute over the lines and window of c->pos_key into
-// out.  The per-line pre-pass (table upload, counter fill, k_line_max,
-// k_line_rec, k_segments) runs on c->pre into one of two buffer sets, after
+// out.  The per-line pre-pass (tables and counters, k_line_rec,
+// k_segments) runs on c->pre into one of kPassSets buffer sets, after
 // the profile kernel that last used that set; the profile kernel runs on
 // c->stream after it.  The pre-pass of the next execute thus overlaps this
 // pass's profile kernel; every pass computes exactly what it would alone.
@@ -542,7 +547,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     d.q_slot = -1;
     if ((rc = ensure_items(c, 1, &tp, &d)) != GEXS_OK) return rc;
     gexs_ctx::PassBuf& B = c->pb[c->pbuf];
-    c->pbuf ^= 1;
+    c->pbuf = (c->pbuf + 1) % kPassSets;
     PassTables t;
     if ((rc = pass_tables_in(c, B.tab, B.small, 1, &t)) != GEXS_OK) return rc;
     HIPCHK(c, B.rec.ensure((size_t)std::max(n, 1) * sizeof(VRec)));
@@ -571,11 +576,11 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
                                 (int)t.zero_u64));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    HIPCHK(c, gxk::line_max(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale));
+    // one block per CU: the pre-pass runs beside the profile kernels of the
+    // passes before, and few blocks get CU slots as soon as they are issued
     HIPCHK(c, gxk::line_rec(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale, B.rec.as<VRec>(),
-                            B.recx.as<VRecX>(), t.kmax, t.neval));
+                            B.recx.as<VRecX>(), t.kmax, t.neval, c->cus));
     // Voigt items carry this pass's near-centre bounds (a copy per buffer
     // set); the other profiles read the shared items unchanged
     const bool voigt = c->profile == GEXS_VOIGT;
@@ -1097,8 +1102,6 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, const PassTables& t) {
     }
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    HIPCHK(c, gxk::line_max(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale));
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale,
                             c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval));
@@ -1413,7 +1416,6 @@ int gexs_run_fixed(gexs_ctx* c, const gexs_lines_dev* L, const gexs_tp* tp, cons
     HIPCHK(c, gxk::write_tables(s, t, d, jb, dtp, djd, djob, cnt, kFixedCntWords));
     if (n > 0) {
         HIPCHK(c, gxk::line_pos(s, L->nu, djd, 1, n, dtp, pos));
-        HIPCHK(c, gxk::line_max(s, L->nu, L->E, L->A, L->g, pos, djd, 1, n, dtp, scale));
         HIPCHK(c, gxk::line_rec(s, L->nu, L->E, L->A, L->g, voigt ? L->gamma : nullptr,
                                 voigt ? L->nexp : nullptr, pos, djd, 1, n, dtp, scale, rec, recx,
                                 kmax, neval));

@@ -27,6 +27,7 @@
 #define GX_PROF_VOIGT 2
 #define GX_PROF_VOIGT_QUAD 4
 
+
 // points of a wave-tile = 64 * points-per-lane
 #define GX_WAVE 64
 #define GX_WG 256
@@ -44,14 +45,18 @@ struct __attribute__((aligned(16))) LinePos {
 // branch-free loop of the profile kernel.
 // Voigt:       x_j = fma(cf - j, A, B) (= (nu - f_j) / w), y = gammaL / w,
 //              region I:  K*Kg = Kl (s + 1) / (s^2 + H),  s = x^2 + E2,
-//              E2 = y^2 - 1/2, H = 2y^2, Kl = Kg*0.5641896*y, Kg = S / w * scale.
+//              E2 = y^2 - 1/2, H = 2y^2, Kl = Kg*0.5641896*y, Kg = S / w as
+//              the mantissa of S / w = Kg 2^ex (Kg in [0.5, 1), Kl likewise
+//              scaled by 2^-ex); the profile kernel applies 2^(ex - E), E the
+//              pass's largest ex.
 //              (Re of 0.5641896 t / (1/2 + t^2), t = y - ix: with b = 1/2 + y^2
 //              its denominator x^4 + (2y^2 - 1) x^2 + b^2 equals s^2 + 2y^2 and
 //              its numerator Kl (x^2 + b) equals Kl (s + 1).)
 // Voigt-Quad:  cf, A, B, Kg as Voigt.
-// bin-Gauss:   cf, A, B as Voigt with 1/w = gG, Kg = S * 0.5/dfreq * scale.
+// bin-Gauss:   cf, A, B as Voigt with 1/w = gG, Kg 2^ex = S * 0.5/dfreq.
 struct __attribute__((aligned(32))) VRec {
-    float cf, A, B, Kl, E2, H, Kg, pad;
+    float cf, A, B, Kl, E2, H, Kg;
+    int32_t ex;
 };
 
 // per line, per (T,P): what the checked loop needs beyond VRec.
@@ -80,6 +85,17 @@ struct __attribute__((aligned(16))) WorkItem {
     int32_t group, ngroups, slot;
 };
 
+// per-(T,P) physics of one job, FP64
+struct TPParams {
+    double T, P, Q;
+    double mass, t_ref, p_scale;
+    double cutoff;
+    double start, dfreq;
+    double gauss_hwhm;
+    int32_t profile, width_conv, pressure_mode, win_lo;
+    int32_t win_hi, pad0, pad1, pad2;
+};
+
 // one cross-section
 struct __attribute__((aligned(16))) Job {
     const LinePos* pos;  // job's lines, [0, n)
@@ -89,7 +105,7 @@ struct __attribute__((aligned(16))) Job {
     double* part;        // rows of 64 * PPL FP64 partial sums, one per work item (line groups)
     int32_t n;
     int32_t profile;
-    int32_t scale_slot;  // index into the scale array (uint64 bit pattern of max amplitude)
+    int32_t scale_slot;  // index into the pass-exponent array (biased largest amplitude exponent)
     int32_t accumulate;  // 1: out += tile sums (chunks accumulate); 0: out = tile sums
 };
 
@@ -104,17 +120,6 @@ struct JobDesc {
     int32_t ntiles;     // number of work items (wave-tiles)
     int32_t q_slot;     // >= 0: Q(T) is taken from this slot of a device partition pass
     int32_t groups;     // line groups per tile (<= 1: one); items = ntiles * groups
-};
-
-// per-(T,P) physics of one job, FP64
-struct TPParams {
-    double T, P, Q;
-    double mass, t_ref, p_scale;
-    double cutoff;
-    double start, dfreq;
-    double gauss_hwhm;
-    int32_t profile, width_conv, pressure_mode, win_lo;
-    int32_t win_hi, pad0, pad1, pad2;
 };
 
 // plain-FP64 per (line, (T,P)) values of the baseline path

@@ -11,18 +11,16 @@ namespace gxk {
 // (max_tiles) bounds the lines (tiles) of any one job.
 hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njobs,
                     int64_t max_n, const TPParams* tp, LinePos* pos);
-hipError_t line_max(hipStream_t s, const double* nu, const double* E, const double* A,
-                    const int* g, const LinePos* pos, const JobDesc* jd, int njobs, int64_t max_n,
-                    const TPParams* tp, unsigned long long* scale_bits);
-// line_pos and line_max in one launch (positions made for this pass alone)
-hipError_t line_pos_max(hipStream_t s, const double* nu, const double* E, const double* A,
-                        const int* g, const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
-                        LinePos* pos, unsigned long long* scale_bits);
+// FP32 records of every job's lines, with each amplitude as mantissa and
+// exponent; emax[k] (zero before) receives job k's largest biased exponent,
+// neval[k] its N_eval and kmax[k] its largest k15 (each zero before).
+// max_blocks > 0 caps the blocks per job (a pass that runs beside another
+// kernel gets its few blocks dispatched at once); 0: up to 2048.
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
-                    const unsigned long long* scale_bits, VRec* rec, VRecX* recx, int* kmax,
-                    unsigned long long* neval);
+                    unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
+                    unsigned long long* neval, int max_blocks = 0);
 // near-centre bounds of every work item (after line_rec has produced kmax per
 // job): out[i] = in[i] with s2, s3 set (out may equal in); Voigt only
 hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,

@@ -1,8 +1,9 @@
 // kernels.hip -- gfx950 kernels of the line-by-line cross-section path.
 //
 // Per (T,P) the path is
-//   k_line_max      FP64 line amplitudes -> max (power-of-two scale of the FP32 sums)
-//   k_line_rec      FP64 S, gammaL, 1/w, window -> 32-B FP32 record per line, N_eval
+//   k_line_rec      FP64 S, gammaL, 1/w, window -> 32-B FP32 record per line
+//                   (amplitude as mantissa and exponent), N_eval, the largest
+//                   exponent (the FP32 sums run in units of 2^E)
 //   k_segments      per tile, the lines centred near it (Humlicek regions II-IV)
 //   k_profile_tiles a tile of 64*PPL grid points per wave or per workgroup (its
 //                   waves share the tile's nu-sorted lines); lines are staged per
@@ -65,15 +66,6 @@ __device__ __forceinline__ double line_amp(const TPParams& p, double S, double i
     return S * (0.5 / p.dfreq);
 }
 
-__device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double v) {
-    // positive doubles order like their bit patterns
-    if (v > 0.0 && isfinite(v)) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
 
 __device__ __forceinline__ long long wave_sum_ll(long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -83,14 +75,6 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
 // Workgroup reductions (256 threads): the result is valid in thread 0.  One
 // atomic per workgroup instead of one per wave keeps same-address atomics
 // off the critical path of the per-line passes.
-__device__ __forceinline__ double block_max(double v, double* sh) {
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) v = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
-    return v;
-}
-
 __device__ __forceinline__ long long block_sum_ll(long long v, long long* sh) {
     v = wave_sum_ll(v);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
@@ -146,70 +130,73 @@ __global__ __launch_bounds__(256) void k_line_pos(const double* __restrict__ nu,
         pos[d.pos_off + i] = window_of(p, nu[d.l0 + i]);
 }
 
-// ------------------------------------------------------------- k_line_max
-__global__ __launch_bounds__(256) void k_line_max(const double* __restrict__ nu,
-                                                  const double* __restrict__ E,
-                                                  const double* __restrict__ A,
-                                                  const int* __restrict__ g,
-                                                  const LinePos* __restrict__ pos,
-                                                  const JobDesc* __restrict__ jd,
-                                                  const TPParams* __restrict__ tp,
-                                                  unsigned long long* __restrict__ scale_bits) {
-    const int k = blockIdx.y;
-    const TPParams p = tp[k];
-    const JobDesc d = jd[k];
-    double m = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += gridDim.x * blockDim.x) {
-        const LinePos lp = pos[d.pos_off + i];
-        if (lp.ie <= lp.ib) continue;  // only lines that reach the grid set the scale
-        const int64_t l = d.l0 + i;
-        double S = line_strength(p, nu[l], E[l], A[l], g[l]);
-        double amp = line_amp(p, S, inv_width(p, nu[l]));
-        if (isfinite(amp)) m = fmax(m, amp);
-    }
-    __shared__ double sh[4];
-    m = block_max(m, sh);
-    if (threadIdx.x == 0) atomic_max_pos(&scale_bits[k], m);
-}
+// Amplitudes of one pass span many decades (line strengths reach 1e-125 at
+// 2000 K), beyond FP32.  Each record therefore holds its amplitude as an FP32
+// mantissa m in [0.5, 1) times 2^ex (ex in VRec::ex), and the pass keeps the
+// largest ex of the lines that reach the grid, E; the profile kernel stages a
+// line's constants as ldexp(m, ex - E), so the largest amplitude lands in
+// [0.5, 1).  Power-of-two scaling commutes with rounding, so these are the
+// FP32 values of amp * 2^-E (bit for bit, away from FP32 subnormals).  E is
+// kept biased by kExBias in a 64-bit counter, zero before the pass (no line).
+constexpr int kExBias = 4096;
 
-// k_line_pos and k_line_max in one pass (jobs whose positions are computed
-// for this pass alone, as a served batch's)
-__global__ __launch_bounds__(256) void k_line_pos_max(const double* __restrict__ nu,
-                                                      const double* __restrict__ E,
-                                                      const double* __restrict__ A,
-                                                      const int* __restrict__ g,
-                                                      const JobDesc* __restrict__ jd,
-                                                      const TPParams* __restrict__ tp,
-                                                      LinePos* __restrict__ pos,
-                                                      unsigned long long* __restrict__ scale_bits) {
-    const int k = blockIdx.y;
-    const TPParams p = tp[k];
-    const JobDesc d = jd[k];
-    double m = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n; i += gridDim.x * blockDim.x) {
-        const int64_t l = d.l0 + i;
-        const LinePos lp = window_of(p, nu[l]);
-        pos[d.pos_off + i] = lp;
-        if (lp.ie <= lp.ib) continue;
-        double S = line_strength(p, nu[l], E[l], A[l], g[l]);
-        double amp = line_amp(p, S, inv_width(p, nu[l]));
-        if (isfinite(amp)) m = fmax(m, amp);
-    }
-    __shared__ double sh[4];
-    m = block_max(m, sh);
-    if (threadIdx.x == 0) atomic_max_pos(&scale_bits[k], m);
-}
-
-// power-of-two scale so that the largest amplitude maps into [0.5, 1)
-__device__ __forceinline__ double scale_of(unsigned long long bits) {
-    double m = __longlong_as_double((long long)bits);
-    if (!(m > 0.0)) return 1.0;
-    int ex;
-    frexp(m, &ex);
-    return ldexp(1.0, -ex);
+__device__ __forceinline__ int pass_exponent(unsigned long long biased) {
+    return biased == 0ull ? 0 : (int)biased - kExBias;
 }
 
 // ------------------------------------------------------------- k_line_rec
+// FP32 record of one line at one (T,P) (FP64 arithmetic, no contraction):
+// returns the amplitude exponent ex (0 when the amplitude is 0) and sets
+// *neval to the line's window length when S != 0 and the window is not empty,
+// and *k15 (Voigt) to the distance beyond which its points are in region I.
+__device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double nu, double E,
+                                           double A, int g, const double* gamma,
+                                           const double* nexp, int64_t i, const LinePos& lp,
+                                           VRec& r, VRecX& x, long long* neval) {
+#pragma clang fp contract(off)
+    const double v = nu == 0.0 ? 1e-6 : nu;
+    const double S = line_strength(p, nu, E, A, g);
+    const double iw = inv_width(p, nu);
+    const bool reaches = lp.ie > lp.ib;
+    *neval = (S != 0.0 && reaches) ? (long long)(lp.ie - lp.ib) : 0;
+    // offset of the line from its nearest grid point, FP64 (f_c = start + c*dfreq)
+    const double fc = p.start + (double)lp.c * p.dfreq;
+    const double delta = v - fc;
+    const double Af = p.dfreq * iw;
+    const double Bf = delta * iw;
+    // a line whose window misses the grid never adds to it: amplitude 0
+    double amp = reaches ? line_amp(p, S, iw) : 0.0;
+    int ex = 0;
+    if (isfinite(amp) && amp > 0.0) {
+        frexp(amp, &ex);
+        amp = ldexp(amp, -ex);  // exact: the mantissa in [0.5, 1)
+    } else {
+        amp = 0.0;
+    }
+    r.cf = lp.cf;
+    r.A = (float)Af;
+    r.B = (float)Bf;
+    r.Kg = (float)amp;
+    if (voigt) {
+        const double y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
+        r.Kl = (float)(amp * 0.5641896 * y);
+        r.E2 = (float)(y * y - 0.5);
+        r.H = (float)(2.0 * y * y);
+        x.y = (float)y;
+        // |x| >= |c - j| A - |B|; region I needs |x| + y >= 15 (with margin)
+        const double kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
+        x.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e8 ? 100000000 : (int)ceil(kk));
+    } else {
+        r.Kl = 0.0f;
+        r.E2 = 0.0f;
+        r.H = 0.0f;
+        x.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
+        x.k15 = 0;
+    }
+    r.ex = ex;
+    return amp > 0.0 ? ex : 0;
+}
+
 __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
                                                   const double* __restrict__ E,
                                                   const double* __restrict__ A,
@@ -219,67 +206,40 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
                                                   const LinePos* __restrict__ pos,
                                                   const JobDesc* __restrict__ jd,
                                                   const TPParams* __restrict__ tp,
-                                                  const unsigned long long* __restrict__ scale_bits,
+                                                  unsigned long long* __restrict__ emax,
                                                   VRec* __restrict__ rec, VRecX* __restrict__ recx,
                                                   int* __restrict__ kmax,
                                                   unsigned long long* __restrict__ neval) {
-#pragma clang fp contract(off)
     const int k = blockIdx.y;
     const TPParams p = tp[k];
     const JobDesc d = jd[k];
-    const double scale = scale_of(scale_bits[k]);
     const bool voigt = p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD;
     long long ne = 0;
     int km = 0;
+    int em = 0;  // largest biased exponent of a line reaching the grid (0: none)
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < d.n; ii += gridDim.x * blockDim.x) {
         const int64_t i = d.l0 + ii;
-        double v = nu[i] == 0.0 ? 1e-6 : nu[i];
-        double S = line_strength(p, nu[i], E[i], A[i], g[i]);
-        double iw = inv_width(p, nu[i]);
         const LinePos lp = pos[d.pos_off + ii];
-        const bool reaches = lp.ie > lp.ib;
-        if (S != 0.0 && reaches) ne += lp.ie - lp.ib;
-        // offset of the line from its nearest grid point, FP64 (f_c = start + c*dfreq)
-        const double fc = p.start + (double)lp.c * p.dfreq;
-        const double delta = v - fc;
-        const double Af = p.dfreq * iw;
-        const double Bf = delta * iw;
-        double amp = line_amp(p, S, iw) * scale;
-        if (!isfinite(amp)) amp = 0.0;
         VRec r;
         VRecX x;
-        r.cf = lp.cf;
-        r.A = (float)Af;
-        r.B = (float)Bf;
-        r.Kg = (float)amp;
-        if (voigt) {
-            const double y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
-            r.Kl = (float)(amp * 0.5641896 * y);
-            r.E2 = (float)(y * y - 0.5);
-            r.H = (float)(2.0 * y * y);
-            x.y = (float)y;
-            // |x| >= |c - j| A - |B|; region I needs |x| + y >= 15 (with margin)
-            const double kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
-            x.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e8 ? 100000000 : (int)ceil(kk));
-            if (reaches) km = max(km, x.k15);
-        } else {
-            r.Kl = 0.0f;
-            r.E2 = 0.0f;
-            r.H = 0.0f;
-            x.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
-            x.k15 = 0;
-        }
-        r.pad = 0.0f;
+        long long nl;
+        line_record(p, voigt, nu[i], E[i], A[i], g[i], gamma, nexp, i, lp, r, x, &nl);
+        ne += nl;
+        if (r.Kg > 0.0f) em = max(em, r.ex + kExBias);
+        if (voigt && lp.ie > lp.ib) km = max(km, x.k15);
         rec[d.rec_off + ii] = r;
         recx[d.recx_off + ii] = x;
     }
     __shared__ long long sh_ne[4];
     __shared__ int sh_km[4];
+    __shared__ int sh_em[4];
     ne = block_sum_ll(ne, sh_ne);
     km = block_max_i(km, sh_km);
+    em = block_max_i(em, sh_em);
     if (threadIdx.x == 0) {
         if (neval != nullptr && ne != 0) atomicAdd(&neval[k], (unsigned long long)ne);
         if (kmax != nullptr && km > 0) atomicMax(&kmax[k], km);
+        if (em > 0) atomicMax(&emax[k], (unsigned long long)em);
     }
 }
 
@@ -450,6 +410,7 @@ struct TileAcc {
     float sum[PPL], cmp[PPL], part[PPL], jf[PPL];
     int jj[PPL];
     int n = 0;  // lines in part since the last fold
+    int E = 0;  // the pass's largest amplitude exponent: constants are staged times 2^(ex - E)
     __device__ __forceinline__ void fold() {
 #pragma unroll
         for (int q = 0; q < PPL; ++q) {
@@ -535,7 +496,7 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
         vn = grec[min(base + GX_WAVE + lane, b - 1)];
         float* row = xs + lane * R;
         row[0] = vc.A;
-        row[1] = vc.Kl;
+        row[1] = ldexpf(vc.Kl, vc.ex - t.E);
         row[2] = vc.E2;
         row[3] = vc.H;
 #pragma unroll
@@ -630,6 +591,13 @@ __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp,
     t.tick(1);
 }
 
+// a record with its amplitudes scaled to the pass: Kl, Kg times 2^(ex - E)
+__device__ __forceinline__ VRec to_pass(VRec r, int E) {
+    r.Kl = ldexpf(r.Kl, r.ex - E);
+    r.Kg = ldexpf(r.Kg, r.ex - E);
+    return r;
+}
+
 // lines [a, b) through checked_line, records by scalar loads, two lines at
 // a time, the next two loading while these are evaluated
 template <int PROF, int PPL, int MASK>
@@ -645,8 +613,8 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
             const VRec s0 = rec[i + 2], s1 = rec[i + 3];
             const VRecX y0 = recx[i + 2], y1 = recx[i + 3];
             __builtin_amdgcn_sched_barrier(0);
-            checked_line<PROF, PPL, MASK>(t, l0, r0, x0, w0, w1);
-            checked_line<PROF, PPL, MASK>(t, l1, r1, x1, w0, w1);
+            checked_line<PROF, PPL, MASK>(t, l0, to_pass(r0, t.E), x0, w0, w1);
+            checked_line<PROF, PPL, MASK>(t, l1, to_pass(r1, t.E), x1, w0, w1);
             l0 = m0;
             l1 = m1;
             r0 = s0;
@@ -654,11 +622,12 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
             x0 = y0;
             x1 = y1;
         }
-        checked_line<PROF, PPL, MASK>(t, l0, r0, x0, w0, w1);
-        checked_line<PROF, PPL, MASK>(t, l1, r1, x1, w0, w1);
+        checked_line<PROF, PPL, MASK>(t, l0, to_pass(r0, t.E), x0, w0, w1);
+        checked_line<PROF, PPL, MASK>(t, l1, to_pass(r1, t.E), x1, w0, w1);
         i += 2;
     }
-    for (; i < b; ++i) checked_line<PROF, PPL, MASK>(t, pos[i], rec[i], recx[i], w0, w1);
+    for (; i < b; ++i)
+        checked_line<PROF, PPL, MASK>(t, pos[i], to_pass(rec[i], t.E), recx[i], w0, w1);
 }
 
 // Voigt lines centred near the tile (kMaskNone) staged like the region-I
@@ -700,10 +669,10 @@ __device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobR
         row[0] = rc.cf;
         row[1] = rc.A;
         row[2] = rc.B;
-        row[3] = rc.Kl;
+        row[3] = ldexpf(rc.Kl, rc.ex - t.E);
         row[4] = rc.E2;
         row[5] = rc.H;
-        row[6] = rc.Kg;
+        row[6] = ldexpf(rc.Kg, rc.ex - t.E);
         row[7] = xc.y;
         row[8] = __int_as_float(pc.c);
         row[9] = __int_as_float(xc.k15);
@@ -781,7 +750,7 @@ __device__ __forceinline__ void gauss_staged(TileAcc<PPL>& t, GlobPos gpos, Glob
         row[0] = rc.cf;
         row[1] = rc.A;
         row[2] = rc.B;
-        row[3] = rc.Kg;
+        row[3] = ldexpf(rc.Kg, rc.ex - t.E);
         row[4] = xc.y;
         row[5] = __int_as_float(pc.ib);
         row[6] = __int_as_float(pc.ie);
@@ -918,6 +887,7 @@ __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRe
     }
     for (int base = a; base < b; base += GX_WAVE) {
         const int nb = min(GX_WAVE, b - base);
+        rc = to_pass(rc, t.E);  // lane k's line, amplitudes scaled to the pass
         {
             // unconditional (the last batch reloads line b - 1), so every
             // iteration has the same loads in flight and the wait before the
@@ -1081,14 +1051,15 @@ __device__ __forceinline__ void store_tile(const WorkItem& it, const Job& jb, in
         store_point(jb, it.w0 + k, v);
 }
 
-// one wave evaluates the whole item; the FP64 scale is read after the line
-// loops (it would otherwise hold two VGPRs through them)
+// one wave evaluates the whole item; the sums are in units of 2^E (the
+// pass exponent, a scalar), restored in FP64 at the store
 template <int PROF, int PPL>
 __device__ __forceinline__ void run_item(const WorkItem& it, const Job& jb,
                                          const unsigned long long* scale_bits, int lane, float* xs) {
     TileAcc<PPL> t;
+    t.E = pass_exponent(scale_bits[jb.scale_slot]);
     sum_share<PROF, PPL>(t, it, jb, lane, it.group, max(it.ngroups, 1), xs);
-    const double inv_scale = 1.0 / scale_of(scale_bits[jb.scale_slot]);
+    const double inv_scale = ldexp(1.0, t.E);
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
         store_tile<PPL>(it, jb, GX_WAVE * q + lane,
@@ -1110,18 +1081,22 @@ __device__ __forceinline__ void run_item_split(const WorkItem& it, const Job& jb
                                                const unsigned long long* scale_bits, int lane,
                                                int wv, double* red, float* xs) {
     TileAcc<PPL> t;
+    t.E = pass_exponent(scale_bits[jb.scale_slot]);
     sum_share<PROF, PPL>(t, it, jb, lane, it.group * SPLIT + wv, max(it.ngroups, 1) * SPLIT, xs);
-    const double inv_scale = 1.0 / scale_of(scale_bits[jb.scale_slot]);
+    // each wave's FP64 partial in absolute units (a power-of-two scaling in
+    // FP64 is exact, so the sum equals the one scaled after adding)
+    const double to_abs = ldexp(1.0, t.E);
     __syncthreads();  // the staging rows are free
 #pragma unroll
     for (int q = 0; q < PPL; ++q)
-        red[(size_t)wv * GX_WAVE * PPL + GX_WAVE * q + lane] = (double)t.sum[q] - (double)t.cmp[q];
+        red[(size_t)wv * GX_WAVE * PPL + GX_WAVE * q + lane] =
+            ((double)t.sum[q] - (double)t.cmp[q]) * to_abs;
     __syncthreads();
     for (int k = threadIdx.x; k < GX_WAVE * PPL; k += block_threads<SPLIT>()) {
         double v = red[k];
 #pragma unroll
         for (int w = 1; w < SPLIT; ++w) v += red[(size_t)w * GX_WAVE * PPL + k];
-        store_tile<PPL>(it, jb, k, v * inv_scale);
+        store_tile<PPL>(it, jb, k, v);
     }
     __syncthreads();  // red (the staging rows) is reused by the next item
 }
@@ -1412,35 +1387,15 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
     return hipGetLastError();
 }
 
-hipError_t line_max(hipStream_t s, const double* nu, const double* E, const double* A,
-                    const int* g, const LinePos* pos, const JobDesc* jd, int njobs, int64_t max_n,
-                    const TPParams* tp, unsigned long long* scale_bits) {
-    if (njobs <= 0 || max_n <= 0) return hipSuccess;
-    int gx = std::min(blocks_for(max_n, 256), 1024);
-    hipLaunchKernelGGL(k_line_max, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, pos, jd, tp,
-                       scale_bits);
-    return hipGetLastError();
-}
-
-hipError_t line_pos_max(hipStream_t s, const double* nu, const double* E, const double* A,
-                        const int* g, const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
-                        LinePos* pos, unsigned long long* scale_bits) {
-    if (njobs <= 0 || max_n <= 0) return hipSuccess;
-    int gx = std::min(blocks_for(max_n, 256), 1024);
-    hipLaunchKernelGGL(k_line_pos_max, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, jd, tp, pos,
-                       scale_bits);
-    return hipGetLastError();
-}
-
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
-                    const unsigned long long* scale_bits, VRec* rec, VRecX* recx, int* kmax,
-                    unsigned long long* neval) {
+                    unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
+                    unsigned long long* neval, int max_blocks) {
     if (njobs <= 0 || max_n <= 0) return hipSuccess;
-    int gx = std::min(blocks_for(max_n, 256), 2048);
+    int gx = std::min(blocks_for(max_n, 256), max_blocks > 0 ? max_blocks : 2048);
     hipLaunchKernelGGL(k_line_rec, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, gamma, nexp, pos,
-                       jd, tp, scale_bits, rec, recx, kmax, neval);
+                       jd, tp, emax, rec, recx, kmax, neval);
     return hipGetLastError();
 }
 

@@ -465,8 +465,7 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         if ((e = gxk::partition(st, v_.sE, v_.sg, v_.n_states, d_T, nreq, d_part, kQBlocks, d_Q, d_jd, nreq,
                                 d_tp, d_scale, 3 * kMaxBatch)) != hipSuccess)
             goto fail;
-        if ((e = gxk::line_pos_max(st, v_.nu, v_.E, v_.A, v_.g, d_jd, nreq, max_n, d_tp, (LinePos*)base,
-                                   d_scale)) != hipSuccess)
+        if ((e = gxk::line_pos(st, v_.nu, d_jd, nreq, max_n, d_tp, (LinePos*)base)) != hipSuccess)
             goto fail;
         if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, (const LinePos*)base, d_jd,
                                nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval)) !=
