@@ -24,7 +24,7 @@ EXPORTS = [
     "gexs_transfer_freq", "gexs_transfer_lines", "gexs_transfer_lines_device", "gexs_execute",
     "gexs_transfer_results", "gexs_ready", "gexs_sync", "gexs_last_error", "gexs_last_stats",
     "gexs_timing", "gexs_line_windows", "gexs_pass_lines", "gexs_pass_items", "gexs_set_tuning", "gexs_load_list", "gexs_line_count",
-    "gexs_partition", "gexs_run_batch", "gexs_batch_device_out",
+    "gexs_partition", "gexs_run_batch", "gexs_batch_device_out", "gexs_stream_xsec",
     "gexs_set_launch", "gexs_autotune", "gexs_run_fixed",
     "gexs_serve_start", "gexs_submit", "gexs_poll", "gexs_wait", "gexs_release",
     "gexs_serve_stop",
@@ -74,6 +74,14 @@ class Tuning(C.Structure):
     _fields_ = [("points_per_lane", C.c_int32), ("order", C.c_int32),
                 ("persistent_blocks", C.c_int32), ("waves_per_tile", C.c_int32),
                 ("kernel_ms", C.c_double)]
+
+
+class StreamStats(C.Structure):
+    _fields_ = [("n_transitions", C.c_int64), ("n_lines", C.c_int64), ("n_eval", C.c_int64),
+                ("chunk", C.c_int64), ("device_bytes", C.c_int64), ("n_chunks", C.c_int32),
+                ("reserved", C.c_int32), ("wall_ms", C.c_double), ("device_ms", C.c_double),
+                ("stage_ms", C.c_double), ("copy_ms", C.c_double), ("sort_ms", C.c_double),
+                ("pass_ms", C.c_double)]
 
 
 class LinesDev(C.Structure):
@@ -135,6 +143,8 @@ def load():
         "gexs_partition": (i32, [P, P, i32, P]),
         "gexs_run_batch": (i32, [P, C.POINTER(TP), i32, C.POINTER(Grid), P]),
         "gexs_batch_device_out": (i32, [P, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+        "gexs_stream_xsec": (i32, [P, C.POINTER(States), C.POINTER(Transitions), C.POINTER(TP),
+                                   i32, C.POINTER(Grid), i64, P, C.POINTER(StreamStats)]),
         "gexs_set_launch": (i32, [P, i32, i32]),
         "gexs_autotune": (i32, [P, C.POINTER(TP), i32, C.POINTER(Grid), C.POINTER(Tuning),
                                 C.POINTER(Tuning), i32, C.POINTER(i32)]),

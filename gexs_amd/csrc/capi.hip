@@ -115,6 +115,58 @@ struct PosKey {
     }
 };
 
+void swap_arr(DevArr& a, DevArr& b) {
+    std::swap(a.p, b.p);
+    std::swap(a.bytes, b.bytes);
+}
+
+// Buffers of gexs_stream_xsec, kept (and grown) between calls.  Two sets of
+// chunk buffers: while the pass of chunk k reads sorted set k mod 2, chunk
+// k + 1 is copied into raw set (k + 1) mod 2, sorted and gathered into the
+// other sorted set, and chunk k + 2 is staged on the host.
+struct StreamBufs {
+    hipStream_t cp = nullptr;  // host -> device copies
+    hipEvent_t h2d_done[2] = {nullptr, nullptr};   // raw set written, staging set free
+    hipEvent_t gathered[2] = {nullptr, nullptr};   // sorted set written, raw set free
+    hipEvent_t pass_done[2] = {nullptr, nullptr};  // sorted set free
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::vector<hipEvent_t> tev;  // per chunk: copy, sort, pass (begin, end each)
+    void* h[2] = {nullptr, nullptr};  // pinned staging, one chunk's columns each
+    size_t h_bytes = 0;
+    unsigned long long* h_small = nullptr;  // pinned read-back slot
+    DevArr raw[2];                          // device copy of one chunk's columns
+    DevArr nu[2], E[2], A[2], gamma[2], nexp[2], g[2];  // a chunk's selected lines, by nu
+    DevArr keys, vals, dropped, st_E, st_g, q_T, q_part, q_Q;
+    gxk::SortBuffers sb;
+    hipError_t init() {
+        hipError_t e = hipStreamCreateWithFlags(&cp, hipStreamNonBlocking);
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+            e = hipEventCreateWithFlags(&h2d_done[i], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&gathered[i], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&pass_done[i], hipEventDisableTiming);
+        }
+        if (e == hipSuccess) e = hipEventCreate(&t0);
+        if (e == hipSuccess) e = hipEventCreate(&t1);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&h_small, 64, hipHostMallocDefault);
+        return e;
+    }
+    ~StreamBufs() {
+        if (cp) (void)hipStreamSynchronize(cp);
+        for (int i = 0; i < 2; ++i) {
+            if (h[i]) (void)hipHostFree(h[i]);
+            if (h2d_done[i]) (void)hipEventDestroy(h2d_done[i]);
+            if (gathered[i]) (void)hipEventDestroy(gathered[i]);
+            if (pass_done[i]) (void)hipEventDestroy(pass_done[i]);
+        }
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        if (h_small) (void)hipHostFree(h_small);
+        gxk::sort_buffers_free(sb);
+        if (cp) (void)hipStreamDestroy(cp);
+    }
+};
+
 // ------------------------------------------------------ no-allocation pass
 // gexs_run_fixed keeps every intermediate of its pass in one file-scope
 // device array (each GPU the library runs on holds its own copy until the
@@ -243,6 +295,7 @@ struct gexs_ctx {
     DevArr batch_out;
 
     std::unique_ptr<gxs::Server> server;
+    std::unique_ptr<StreamBufs> strm;  // gexs_stream_xsec
     Staging staging;
     DevArr pos_jd, pos_tp;
 
@@ -693,6 +746,8 @@ int gexs_destroy(gexs_ctx* c) {
     (void)hipSetDevice(c->device);
     c->server.reset();
     (void)hipStreamSynchronize(c->stream);
+    if (c->pre) (void)hipStreamSynchronize(c->pre);
+    c->strm.reset();
     for (double* p : c->results) (void)hipFree(p);
     c->results.clear();
     gxk::sort_buffers_free(c->sb);
@@ -1288,6 +1343,472 @@ int gexs_batch_device_out(gexs_ctx* c, const double** out, int64_t* count) {
     if (!c || !out) return GEXS_EINVAL;
     *out = c->batch_out.as<double>();
     if (count) *count = (int64_t)(c->batch_out.bytes / 8);
+    return GEXS_OK;
+}
+
+// ------------------------------------------------------- streamed host list
+// Column layout of one chunk (pinned staging and device copy alike):
+// upper | lower | A | gamma | n, each at a 256-B aligned offset, for `cap`
+// transitions.
+struct ChunkLayout {
+    size_t up, lo, A, gam, nex, bytes;
+    int asz, gsz, nsz;
+};
+
+static ChunkLayout chunk_layout(const gexs_transitions* tr, bool has_gamma, int64_t cap) {
+    ChunkLayout L{};
+    L.asz = tr->A64 ? 8 : 4;
+    L.gsz = has_gamma ? (tr->gamma64 ? 8 : 4) : 0;
+    L.nsz = has_gamma ? (tr->nexp64 ? 8 : 4) : 0;
+    const size_t n = (size_t)cap;
+    L.up = 0;
+    L.lo = al256(n * 4);
+    L.A = L.lo + al256(n * 4);
+    L.gam = L.A + al256(n * L.asz);
+    L.nex = L.gam + al256(n * L.gsz);
+    L.bytes = L.nex + al256(n * L.nsz);
+    return L;
+}
+
+// Copies transitions [a, a + m) of tr into the staging columns at h
+// (threads share the range) and returns the largest state index among them.
+static uint32_t stage_chunk(const gexs_transitions* tr, bool has_gamma, const ChunkLayout& L, char* h,
+                            int64_t a, int64_t m) {
+    const int64_t per_thread = 1 << 20;
+    int nthr = (int)std::min<int64_t>((m + per_thread - 1) / per_thread,
+                                      std::max(1u, std::min(8u, std::thread::hardware_concurrency())));
+    nthr = std::max(nthr, 1);
+    std::vector<uint32_t> mx((size_t)nthr, 0u);
+    auto work = [&](int t) {
+        const int64_t b = a + m * t / nthr, e = a + m * (t + 1) / nthr, o = b - a;
+        uint32_t hi = 0;
+        uint32_t* up = (uint32_t*)(h + L.up) + o;
+        uint32_t* lo = (uint32_t*)(h + L.lo) + o;
+        for (int64_t i = b; i < e; ++i) {
+            const uint32_t u = tr->upper[i], l = tr->lower[i];
+            up[i - b] = u;
+            lo[i - b] = l;
+            hi = std::max(hi, std::max(u, l));
+        }
+        mx[(size_t)t] = hi;
+        auto col = [&](size_t off, int sz, const void* p32, const void* p64) {
+            const void* src = sz == 8 ? p64 : p32;
+            std::memcpy(h + off + (size_t)o * sz, (const char*)src + (size_t)b * sz, (size_t)(e - b) * sz);
+        };
+        col(L.A, L.asz, tr->A, tr->A64);
+        if (has_gamma) {
+            col(L.gam, L.gsz, tr->gamma, tr->gamma64);
+            col(L.nex, L.nsz, tr->nexp, tr->nexp64);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nthr; ++t) {
+        try {
+            th.emplace_back(work, t);
+        } catch (...) {
+            work(t);  // no thread: this one does the share
+        }
+    }
+    work(0);
+    for (auto& x : th) x.join();
+    uint32_t hi = 0;
+    for (uint32_t v : mx) hi = std::max(hi, v);
+    return hi;
+}
+
+// Joins the staging thread on every way out of gexs_stream_xsec.
+struct Joiner {
+    std::thread t;
+    ~Joiner() {
+        if (t.joinable()) t.join();
+    }
+};
+
+// The current lines of c are those of sorted chunk set `set` while one
+// exists (the pass launchers read them at enqueue time).
+struct ChunkLines {
+    gexs_ctx* c;
+    StreamBufs* S;
+    int set;
+    int64_t n_saved;
+    bool gamma_saved;
+    ChunkLines(gexs_ctx* c_, StreamBufs* S_, int set_, int64_t n, bool has_gamma)
+        : c(c_), S(S_), set(set_), n_saved(c_->n), gamma_saved(c_->has_gamma) {
+        swap_all();
+        c->n = n;
+        c->has_gamma = has_gamma;
+        c->gen++;
+    }
+    ~ChunkLines() {
+        swap_all();
+        c->n = n_saved;
+        c->has_gamma = gamma_saved;
+        c->gen++;  // caches built on the chunk's lines are stale
+    }
+    void swap_all() {
+        swap_arr(c->nu, S->nu[set]);
+        swap_arr(c->E, S->E[set]);
+        swap_arr(c->A, S->A[set]);
+        swap_arr(c->gamma, S->gamma[set]);
+        swap_arr(c->nexp, S->nexp[set]);
+        swap_arr(c->g, S->g[set]);
+    }
+};
+
+// The pass of one chunk (its n selected lines are the current lines) on
+// c->stream: every (T,P) adds into its grid of c->batch_out.  FP32C: the
+// counters before the N_eval words are cleared, N_eval adds up over the chunks.
+static int stream_chunk_pass(gexs_ctx* c, const std::vector<TPParams>& htp, int64_t n, int np) {
+    const int ntp = (int)htp.size();
+    hipStream_t s = c->stream;
+    int rc;
+    if ((rc = ensure_pos(c, htp[0], 0, n)) != GEXS_OK) return rc;
+    if (c->k.precision == GEXS_FP64) {
+        for (int j = 0; j < ntp; ++j)
+            if ((rc = run_fp64(c, htp[(size_t)j], c->batch_out.as<double>() + (size_t)j * np, false, false)) !=
+                GEXS_OK)
+                return rc;
+        return GEXS_OK;
+    }
+    const int per = tiles_per_job(c);
+    PassTables t;
+    if ((rc = pass_tables(c, ntp, &t)) != GEXS_OK) return rc;
+    std::vector<char> host(t.bytes);
+    TPParams* htab = (TPParams*)host.data();
+    JobDesc* hjd = (JobDesc*)(host.data() + t.jd_off);
+    Job* hjobs = (Job*)(host.data() + t.jobs_off);
+    for (int j = 0; j < ntp; ++j) {
+        htab[j] = htp[(size_t)j];
+        JobDesc d{};
+        d.l0 = 0;
+        d.pos_off = 0;
+        d.rec_off = (int64_t)j * n;
+        d.recx_off = (int64_t)j * n;
+        d.n = (int32_t)n;
+        d.items_off = j * per;
+        d.ntiles = per;
+        d.q_slot = -1;
+        hjd[j] = d;
+        Job jb{};
+        jb.pos = c->pos.as<LinePos>();
+        jb.rec = c->rec.as<VRec>() + (size_t)j * n;
+        jb.recx = c->recx.as<VRecX>() + (size_t)j * n;
+        jb.out = c->batch_out.as<double>() + (size_t)j * np;
+        jb.n = (int32_t)n;
+        jb.profile = c->profile;
+        jb.scale_slot = j;
+        jb.accumulate = 1;  // the chunks add up
+        hjobs[j] = jb;
+    }
+    HIPCHK(c, c->staging.upload(c->tab.p, host.data(), t.bytes, s));
+    if ((rc = ensure_items(c, ntp, htab, hjd)) != GEXS_OK) return rc;
+    HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)c->small.p, 4 + (int64_t)ntp, 0ull));  // xq | scale
+    HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)t.kmax, ntp, 0ull));
+    const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
+    const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(), gam,
+                            nex, c->pos.as<LinePos>(), t.jd, ntp, n, t.tp, t.scale, c->rec.as<VRec>(),
+                            c->recx.as<VRecX>(), t.kmax, t.neval));
+    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->items.as<WorkItem>(), c->n_items, t.jd,
+                            c->pos.as<LinePos>(), t.kmax, c->profile));
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items, t.jobs,
+                                 t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
+                                 launch_blocks(c)));
+    c->last.n_tiles = c->n_items;
+    c->last_items = c->items.as<WorkItem>();
+    return GEXS_OK;
+}
+
+int gexs_stream_xsec(gexs_ctx* c, const gexs_states* st, const gexs_transitions* tr, const gexs_tp* tp,
+                     int ntp, const gexs_grid* grid, int64_t mem_budget, double* out,
+                     gexs_stream_stats* stats) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    if (!c || !st || !tr || !tp || ntp <= 0 || !grid || grid->npoints <= 0 || !(grid->end > grid->start) ||
+        mem_budget <= 0)
+        return GEXS_EINVAL;
+    if (st->n <= 0 || !st->E || !st->g || tr->n < 0) return GEXS_EINVAL;
+    if (tr->n > 0 && (!tr->upper || !tr->lower || (!tr->A && !tr->A64))) return GEXS_EINVAL;
+    if (!c->have_k) return c->fail(GEXS_ESTATE, "stream_xsec before init_constants");
+    if (c->server) return c->fail(GEXS_ESTATE, "stream_xsec while serving");
+    if (st->n > 0xFFFFFFFFll) return c->fail(GEXS_EINVAL, "more than 2^32 states");
+    const bool has_gamma = (tr->gamma != nullptr || tr->gamma64 != nullptr) &&
+                           (tr->nexp != nullptr || tr->nexp64 != nullptr);
+    if ((c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD) && !has_gamma)
+        return c->fail(GEXS_EINVAL, "Voigt profiles need gamma and n per transition");
+    for (int i = 0; i < ntp; ++i)
+        if (!(tp[i].T > 0.0) || !(tp[i].P >= 0.0)) return c->fail(GEXS_EINVAL, "T must be positive, P >= 0");
+    const double cutoff = effective_cutoff(c->k, tp[0].P);
+    for (int i = 1; i < ntp; ++i)
+        if (effective_cutoff(c->k, tp[i].P) != cutoff)
+            return c->fail(GEXS_EINVAL, "a batch needs one cutoff (set gexs_constants.cutoff > 0)");
+    int rc;
+    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->pre));
+    const bool f64 = c->k.precision == GEXS_FP64;
+    const int np = grid->npoints;
+    const double dfreq = (grid->end - grid->start) / (double)np;  // BaseProfile.cpp:10
+    const double slo = grid->start - c->k.half_width, shi = grid->end + c->k.half_width;
+    const int64_t N = tr->n;
+
+    // chunk size under the budget: per transition two staging/raw sets, the
+    // sort keys, two sorted sets and the pass's windows and records
+    const ChunkLayout L1 = chunk_layout(tr, has_gamma, 1);
+    const int64_t raw_b = 8 + L1.asz + L1.gsz + L1.nsz;
+    const int64_t sorted_b = 28 + (has_gamma ? 16 : 0);
+    const int64_t sort_b = 25;  // keys, values, their second buffers, digit counts
+    const int64_t pass_b = (int64_t)sizeof(LinePos) +
+                           (f64 ? (int64_t)sizeof(LineF64) : (int64_t)ntp * (int64_t)(sizeof(VRec) + sizeof(VRecX)));
+    const int64_t per_tr = 2 * raw_b + 2 * sorted_b + sort_b + pass_b;
+    const int64_t tiles = (int64_t)((np + GX_WAVE * c->ppl - 1) / (GX_WAVE * c->ppl)) * ntp;
+    const int64_t fixed = st->n * 12 + (int64_t)ntp * np * 8 + tiles * (int64_t)(2 * sizeof(WorkItem) + 17) +
+                          (int64_t)ntp * 8 * 256 + (4 << 20);
+    // at least 4096 transitions per chunk (or the whole list, if shorter)
+    const int64_t min_chunk = std::min<int64_t>(std::max<int64_t>(N, 1), 4096);
+    if (mem_budget < fixed + per_tr * min_chunk)
+        return c->fail(GEXS_ENOMEM, "mem_budget below " + std::to_string(fixed + per_tr * min_chunk) +
+                                        " bytes (" + std::to_string(min_chunk) + " transitions per chunk)");
+    int64_t cap = (mem_budget - fixed) / per_tr;
+    cap = std::min<int64_t>(cap, std::max<int64_t>(N, 1));
+    cap = std::min<int64_t>(cap, 0x7FFFFFFF);
+    const int64_t nchunks = N == 0 ? 0 : (N + cap - 1) / cap;
+    const ChunkLayout L = chunk_layout(tr, has_gamma, cap);
+
+    // buffers, all sized before the first chunk (nothing is freed while a
+    // chunk is in flight)
+    if (!c->strm) {
+        c->strm.reset(new (std::nothrow) StreamBufs());
+        if (!c->strm) return GEXS_ENOMEM;
+        hipError_t e = c->strm->init();
+        if (e != hipSuccess) {
+            c->strm.reset();
+            return c->hipfail(e, "stream buffers");
+        }
+    }
+    StreamBufs* S = c->strm.get();
+    HIPCHK(c, hipStreamSynchronize(S->cp));
+    hipStream_t s = c->stream, P = c->pre, cp = S->cp;
+    HIPCHK(c, S->st_E.ensure((size_t)st->n * 8));
+    HIPCHK(c, S->st_g.ensure((size_t)st->n * 4));
+    HIPCHK(c, hipMemcpyAsync(S->st_E.p, st->E, (size_t)st->n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(S->st_g.p, st->g, (size_t)st->n * 4, hipMemcpyHostToDevice, s));
+    const size_t cb = (size_t)cap;
+    for (int i = 0; i < 2; ++i) {
+        HIPCHK(c, S->raw[i].ensure(L.bytes));
+        HIPCHK(c, S->nu[i].ensure(cb * 8));
+        HIPCHK(c, S->E[i].ensure(cb * 8));
+        HIPCHK(c, S->A[i].ensure(cb * 8));
+        HIPCHK(c, S->g[i].ensure(cb * 4));
+        if (has_gamma) {
+            HIPCHK(c, S->gamma[i].ensure(cb * 8));
+            HIPCHK(c, S->nexp[i].ensure(cb * 8));
+        }
+        if (S->h_bytes < L.bytes) {
+            if (S->h[i]) (void)hipHostFree(S->h[i]);
+            S->h[i] = nullptr;
+            HIPCHK(c, hipHostMalloc(&S->h[i], L.bytes, hipHostMallocDefault));
+        }
+    }
+    S->h_bytes = std::max(S->h_bytes, L.bytes);
+    HIPCHK(c, S->keys.ensure(cb * 8));
+    HIPCHK(c, S->vals.ensure(cb * 4));
+    HIPCHK(c, S->dropped.ensure(64));
+    HIPCHK(c, gxk::sort_buffers_alloc(S->sb, (int64_t)cb, 8));
+    HIPCHK(c, c->pos.ensure(cb * sizeof(LinePos)));
+    if (f64) {
+        HIPCHK(c, c->lf64.ensure(cb * sizeof(LineF64)));
+        HIPCHK(c, c->neval.ensure(8));
+        HIPCHK(c, gxk::fill_u64(s, c->neval.as<unsigned long long>(), 1, 0ull));
+    } else {
+        HIPCHK(c, c->rec.ensure(cb * (size_t)ntp * sizeof(VRec)));
+        HIPCHK(c, c->recx.ensure(cb * (size_t)ntp * sizeof(VRecX)));
+        HIPCHK(c, c->counter.ensure(256));
+        PassTables t;
+        if ((rc = pass_tables(c, ntp, &t)) != GEXS_OK) return rc;
+        HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)c->small.p, t.zero_u64, 0ull));
+    }
+    HIPCHK(c, c->batch_out.ensure((size_t)np * ntp * 8));
+    HIPCHK(c, gxk::fill_f64(s, c->batch_out.as<double>(), (int64_t)np * ntp, 0.0));
+
+    // (T,P) tables; Q(T) from the streamed states where not given
+    std::vector<TPParams> htp((size_t)ntp);
+    {
+        std::vector<double> needT;
+        for (int i = 0; i < ntp; ++i)
+            if (!(tp[i].Q > 0.0)) needT.push_back(tp[i].T);
+        std::vector<double> q(needT.size());
+        if (!needT.empty()) {
+            const int nq = (int)needT.size(), nblk = 512;
+            HIPCHK(c, S->q_T.ensure((size_t)nq * 8));
+            HIPCHK(c, S->q_part.ensure((size_t)nq * nblk * 8));
+            HIPCHK(c, S->q_Q.ensure((size_t)nq * 8));
+            HIPCHK(c, hipMemcpyAsync(S->q_T.p, needT.data(), (size_t)nq * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(c, gxk::partition(s, S->st_E.as<double>(), S->st_g.as<int32_t>(), st->n, S->q_T.as<double>(), nq,
+                                     S->q_part.as<double>(), nblk, S->q_Q.as<double>()));
+            HIPCHK(c, hipMemcpyAsync(q.data(), S->q_Q.p, (size_t)nq * 8, hipMemcpyDeviceToHost, s));
+        }
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int i = 0, w = 0; i < ntp; ++i)
+            htp[(size_t)i] = make_tp(c, tp[i].T, tp[i].P, tp[i].Q > 0.0 ? tp[i].Q : q[(size_t)w++], grid->start,
+                                     dfreq, 0, np);
+    }
+
+    while ((int64_t)S->tev.size() < 6 * nchunks) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        S->tev.push_back(e);
+    }
+    auto ev = [&](int64_t k, int i) { return S->tev[(size_t)(6 * k + i)]; };
+
+    // the pipeline
+    c->last = gexs_stats{};
+    double stage_ms = 0.0;
+    int64_t n_lines = 0;
+    bool h2d_used[2] = {false, false};
+    int stage_rc = GEXS_OK;
+    Joiner ld;
+    auto start_stage = [&](int64_t k) {
+        const int set = (int)(k & 1);
+        const bool wait = h2d_used[set];
+        // host copy of chunk k into staging set k mod 2 (free once the DMA of
+        // chunk k - 2 has read it), then its DMA into raw set k mod 2 (free
+        // once the gather of chunk k - 2 has read it), enqueued from here
+        auto job = [&, k, set, wait]() {
+            if (hipSetDevice(c->device) != hipSuccess ||
+                (wait && hipEventSynchronize(S->h2d_done[set]) != hipSuccess)) {
+                stage_rc = GEXS_EHIP;
+                return;
+            }
+            const auto h0 = std::chrono::steady_clock::now();
+            const int64_t a = k * cap, m = std::min(cap, N - a);
+            const uint32_t hi = stage_chunk(tr, has_gamma, L, (char*)S->h[set], a, m);
+            stage_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+            if ((int64_t)hi >= st->n) {  // checked before the chunk reaches the device
+                stage_rc = GEXS_EINVAL;
+                return;
+            }
+            char* dev = (char*)S->raw[set].p;
+            const char* h = (const char*)S->h[set];
+            bool ok = k < 2 || hipStreamWaitEvent(cp, S->gathered[set], 0) == hipSuccess;
+            if (k == 0) ok = ok && hipEventRecord(S->t0, cp) == hipSuccess;
+            ok = ok && hipEventRecord(ev(k, 0), cp) == hipSuccess;
+            auto put = [&](size_t off, int sz) {
+                ok = ok && hipMemcpyAsync(dev + off, h + off, (size_t)m * sz, hipMemcpyHostToDevice, cp) == hipSuccess;
+            };
+            put(L.up, 4);
+            put(L.lo, 4);
+            put(L.A, L.asz);
+            if (has_gamma) {
+                put(L.gam, L.gsz);
+                put(L.nex, L.nsz);
+            }
+            ok = ok && hipEventRecord(ev(k, 1), cp) == hipSuccess;
+            ok = ok && hipEventRecord(S->h2d_done[set], cp) == hipSuccess;
+            h2d_used[set] = true;
+            stage_rc = ok ? GEXS_OK : GEXS_EHIP;
+        };
+        try {
+            ld.t = std::thread(job);
+        } catch (...) {
+            job();
+        }
+    };
+    if (nchunks > 0) start_stage(0);
+    else HIPCHK(c, hipEventRecord(S->t0, s));
+    for (int64_t k = 0; k < nchunks; ++k) {
+        const int set = (int)(k & 1);
+        const int64_t m = std::min(cap, N - k * cap);
+        if (ld.t.joinable()) ld.t.join();
+        if (stage_rc == GEXS_EINVAL) return c->fail(GEXS_EINVAL, "transition index outside the state table");
+        if (stage_rc != GEXS_OK) return c->fail(stage_rc, "staging or copying a chunk");
+        char* dev = (char*)S->raw[set].p;
+        // chunk k + 1 is staged and copied meanwhile
+        if (k + 1 < nchunks) start_stage(k + 1);
+        // keys, sort and selection of chunk k (pre stream)
+        const uint32_t* up = (const uint32_t*)(dev + L.up);
+        const uint32_t* lo = (const uint32_t*)(dev + L.lo);
+        HIPCHK(c, hipStreamWaitEvent(P, S->h2d_done[set], 0));
+        HIPCHK(c, hipEventRecord(ev(k, 2), P));
+        HIPCHK(c, gxk::fill_u64(P, S->dropped.as<unsigned long long>(), 2, 0ull));
+        HIPCHK(c, gxk::chunk_keys(P, S->st_E.as<double>(), up, lo, m, slo, shi, S->keys.as<uint64_t>(),
+                                  S->vals.as<uint32_t>(), S->dropped.as<unsigned long long>()));
+        // the selected count and the digits the sort passes over: the one
+        // wait of the chunk (the sort and gather then run unattended)
+        HIPCHK(c, hipMemcpyAsync(S->h_small, S->dropped.p, 16, hipMemcpyDeviceToHost, P));
+        HIPCHK(c, hipStreamSynchronize(P));
+        const int64_t kept = m - (int64_t)S->h_small[0];
+        const unsigned long long diff = S->h_small[1];
+        HIPCHK(c, gxk::radix_sort_u64(P, S->keys.as<uint64_t>(), S->vals.as<uint32_t>(), m, S->sb, 0, 64, &diff));
+        // the selected lines, by nu, into sorted set k mod 2 once pass k - 2 has read it
+        if (k >= 2) HIPCHK(c, hipStreamWaitEvent(P, S->pass_done[set], 0));
+        gxk::ChunkCols cols{};
+        if (L.asz == 8) cols.A64 = (const double*)(dev + L.A); else cols.A32 = (const float*)(dev + L.A);
+        if (has_gamma) {
+            if (L.gsz == 8) cols.gam64 = (const double*)(dev + L.gam); else cols.gam32 = (const float*)(dev + L.gam);
+            if (L.nsz == 8) cols.nex64 = (const double*)(dev + L.nex); else cols.nex32 = (const float*)(dev + L.nex);
+        }
+        HIPCHK(c, gxk::chunk_gather(P, S->vals.as<uint32_t>(), kept, up, lo, S->st_E.as<double>(),
+                                    S->st_g.as<int32_t>(), cols, S->nu[set].as<double>(), S->E[set].as<double>(),
+                                    S->g[set].as<int32_t>(), S->A[set].as<double>(),
+                                    has_gamma ? S->gamma[set].as<double>() : nullptr,
+                                    has_gamma ? S->nexp[set].as<double>() : nullptr));
+        HIPCHK(c, hipEventRecord(ev(k, 3), P));
+        HIPCHK(c, hipEventRecord(S->gathered[set], P));
+        // the pass of chunk k (main stream)
+        HIPCHK(c, hipStreamWaitEvent(s, S->gathered[set], 0));
+        HIPCHK(c, hipEventRecord(ev(k, 4), s));
+        if (kept > 0) {
+            ChunkLines cur(c, S, set, kept, has_gamma);
+            if ((rc = stream_chunk_pass(c, htp, kept, np)) != GEXS_OK) return rc;
+        }
+        HIPCHK(c, hipEventRecord(ev(k, 5), s));
+        HIPCHK(c, hipEventRecord(S->pass_done[set], s));
+        n_lines += kept;
+    }
+    HIPCHK(c, hipEventRecord(S->t1, s));
+    if (out) HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, (size_t)np * ntp * 8, hipMemcpyDefault, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    int64_t ne = 0;
+    if (f64) {
+        unsigned long long v = 0;
+        HIPCHK(c, hipMemcpy(&v, c->neval.p, 8, hipMemcpyDeviceToHost));
+        ne = (int64_t)v;
+    } else {
+        PassTables t;
+        if ((rc = pass_tables(c, ntp, &t)) != GEXS_OK) return rc;
+        std::vector<unsigned long long> v((size_t)ntp, 0ull);
+        HIPCHK(c, hipMemcpy(v.data(), t.neval, v.size() * 8, hipMemcpyDeviceToHost));
+        for (auto x : v) ne += (int64_t)x;
+    }
+    float dms = 0.f;
+    if (hipEventElapsedTime(&dms, S->t0, S->t1) != hipSuccess) dms = 0.f;
+    double stage_sum[3] = {0.0, 0.0, 0.0};  // copy, sort, pass
+    for (int64_t k = 0; k < nchunks; ++k)
+        for (int i = 0; i < 3; ++i) {
+            float x = 0.f;
+            if (hipEventElapsedTime(&x, ev(k, 2 * i), ev(k, 2 * i + 1)) == hipSuccess) stage_sum[i] += x;
+        }
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    c->last.n_eval = ne;
+    c->last.n_lines = n_lines;
+    c->last.kernel_ms = dms;
+    c->last.total_ms = wall;
+    c->last_neval_pending = false;
+    if (stats) {
+        *stats = gexs_stream_stats{};
+        stats->n_transitions = N;
+        stats->n_lines = n_lines;
+        stats->n_eval = ne;
+        stats->chunk = cap;
+        stats->device_bytes = fixed + per_tr * cap;
+        stats->n_chunks = (int32_t)nchunks;
+        stats->wall_ms = wall;
+        stats->device_ms = dms;
+        stats->stage_ms = stage_ms;
+        stats->copy_ms = stage_sum[0];
+        stats->sort_ms = stage_sum[1];
+        stats->pass_ms = stage_sum[2];
+    }
     return GEXS_OK;
 }
 

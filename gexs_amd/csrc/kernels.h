@@ -79,10 +79,13 @@ struct SortBuffers {
     int key_bytes = 0;
 };
 size_t sort_scratch_bytes(int64_t n, int key_bytes);
-// Sorts keys/vals in place (results end in keys/vals).  Blocks on the stream
-// once per pass to skip passes whose digit is constant.
+// Sorts keys/vals in place (results end in keys/vals), skipping the passes
+// whose digit is the same in every key.  Those digits come from diff (the OR
+// of key ^ keys[0] over the keys) when given; otherwise one pass over the
+// keys finds them and the call blocks on the stream once, before the passes.
 hipError_t radix_sort_u64(hipStream_t s, uint64_t* keys, uint32_t* vals, int64_t n,
-                          SortBuffers& buf, int begin_bit, int end_bit);
+                          SortBuffers& buf, int begin_bit, int end_bit,
+                          const unsigned long long* diff = nullptr);
 hipError_t radix_sort_u32(hipStream_t s, uint32_t* keys, uint32_t* vals, int64_t n,
                           SortBuffers& buf, int begin_bit, int end_bit);
 hipError_t sort_buffers_alloc(SortBuffers& buf, int64_t n, int key_bytes);
@@ -105,5 +108,28 @@ hipError_t assemble_lines(hipStream_t s, const double* sE, const int32_t* sg,
                           const uint32_t* up, const uint32_t* lo, int64_t n, double* nu,
                           double* E, int32_t* g);
 hipError_t widen_f32(hipStream_t s, const float* in, int64_t n, double* out);
+
+// ---- one chunk of a streamed list (gexs_stream_xsec)
+// keys of the chunk's transitions by nu = E_u - E_l; those outside
+// [nu_lo, nu_hi] or at nu == 0 sort last.  out[0] (zero before) counts them,
+// out[1] (zero before) receives the OR of key ^ key[0] (radix_sort_u64's diff)
+hipError_t chunk_keys(hipStream_t s, const double* sE, const uint32_t* up, const uint32_t* lo,
+                      int64_t n, double nu_lo, double nu_hi, uint64_t* keys, uint32_t* vals,
+                      unsigned long long* out);
+// the chunk's A, gamma, n columns: per column one of the FP32 / FP64 arrays
+// (gamma and n both NULL for a Doppler-only list)
+struct ChunkCols {
+    const float* A32;
+    const double* A64;
+    const float* gam32;
+    const double* gam64;
+    const float* nex32;
+    const double* nex64;
+};
+// lines [0, n) of the sorted chunk: line i = transition perm[i] (gamma and
+// nexp NULL: not written)
+hipError_t chunk_gather(hipStream_t s, const uint32_t* perm, int64_t n, const uint32_t* up,
+                        const uint32_t* lo, const double* sE, const int32_t* sg, const ChunkCols& in,
+                        double* nu, double* E, int32_t* g, double* A, double* gamma, double* nexp);
 
 }  // namespace gxk

@@ -124,6 +124,20 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* __restrict__
     }
 }
 
+// out |= key ^ keys[0] over the keys: a digit that is zero in it is the same
+// in every key (one atomic per wave)
+template <typename K>
+__global__ __launch_bounds__(256) void k_rs_diff(const K* __restrict__ keys, int64_t n,
+                                                 unsigned long long* __restrict__ out) {
+    const K ref = keys[0];
+    unsigned long long x = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x |= (unsigned long long)(keys[i] ^ ref);
+    for (int o = GX_WAVE / 2; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+    if ((threadIdx.x & (GX_WAVE - 1)) == 0 && x) atomicOr(out, x);
+}
+
 // With drop_zero, lines at nu == 0 get the largest key (they sort last) and
 // are counted, so the caller can cut them off the end.
 __global__ void k_nu_keys(const double* __restrict__ nu, int64_t n, uint64_t* __restrict__ keys,
@@ -189,6 +203,78 @@ __global__ void k_widen(const float* __restrict__ in, int64_t n, double* __restr
         out[i] = (double)in[i];
 }
 
+// One chunk of a streamed list: nu = E_u - E_l of each transition (as
+// k_assemble), its order-preserving key and the identity permutation.  A
+// transition outside the selection [lo, hi] or at nu == 0 takes the largest
+// key, so it sorts behind every selected one.  out[0] += the number of such
+// transitions, out[1] |= key ^ (key of transition 0) over the chunk (the
+// digits the sort must pass over); one atomic each per wave.
+__device__ __forceinline__ uint64_t chunk_key(const double* __restrict__ sE, uint32_t u, uint32_t l,
+                                              double nu_lo, double nu_hi, bool& drop) {
+    const double v = sE[u] - sE[l];
+    drop = !(v >= nu_lo && v <= nu_hi) || v == 0.0;
+    uint64_t k = (uint64_t)__double_as_longlong(v);
+    k = (k >> 63) ? ~k : (k | 0x8000000000000000ull);
+    return drop ? ~0ull : k;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_keys(const double* __restrict__ sE,
+                                                    const uint32_t* __restrict__ up,
+                                                    const uint32_t* __restrict__ lo, int64_t n,
+                                                    double nu_lo, double nu_hi,
+                                                    uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals,
+                                                    unsigned long long* __restrict__ out) {
+    bool d0;
+    const uint64_t ref = chunk_key(sE, up[0], lo[0], nu_lo, nu_hi, d0);
+    unsigned long long drops = 0, x = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        bool drop;
+        const uint64_t k = chunk_key(sE, up[i], lo[i], nu_lo, nu_hi, drop);
+        keys[i] = k;
+        vals[i] = (uint32_t)i;
+        drops += drop ? 1ull : 0ull;
+        x |= k ^ ref;
+    }
+    for (int o = GX_WAVE / 2; o > 0; o >>= 1) {
+        drops += __shfl_xor(drops, o);
+        x |= __shfl_xor(x, o);
+    }
+    if ((threadIdx.x & (GX_WAVE - 1)) == 0) {
+        if (drops) atomicAdd(&out[0], drops);
+        if (x) atomicOr(&out[1], x);
+    }
+}
+
+// The sorted lines of a streamed chunk: line i is transition perm[i], with
+// nu, E'' = E_l and g' = g_u from the states (as k_assemble) and A, gamma, n
+// from the chunk's FP32 (x32) or FP64 (x64) arrays, widened to FP64.
+__device__ __forceinline__ double pick(const float* a32, const double* a64, uint32_t p) {
+    return a64 ? a64[p] : (double)a32[p];
+}
+__global__ __launch_bounds__(256) void k_chunk_gather(
+    const uint32_t* __restrict__ perm, int64_t n, const uint32_t* __restrict__ up,
+    const uint32_t* __restrict__ lo, const double* __restrict__ sE, const int32_t* __restrict__ sg,
+    const float* A32, const double* A64, const float* gam32, const double* gam64, const float* nex32,
+    const double* nex64, double* __restrict__ nu, double* __restrict__ E, int32_t* __restrict__ g,
+    double* __restrict__ A, double* __restrict__ gamma, double* __restrict__ nexp) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = perm[i];
+        const uint32_t u = up[p], l = lo[p];
+        const double eu = sE[u], el = sE[l];
+        nu[i] = eu - el;
+        E[i] = el;
+        g[i] = sg[u];
+        A[i] = pick(A32, A64, p);
+        if (gamma) {
+            gamma[i] = pick(gam32, gam64, p);
+            nexp[i] = pick(nex32, nex64, p);
+        }
+    }
+}
+
 namespace gxk {
 
 static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
@@ -224,25 +310,33 @@ void sort_buffers_free(SortBuffers& buf) {
 
 template <typename K>
 static hipError_t radix_sort(hipStream_t s, K* keys, uint32_t* vals, int64_t n, SortBuffers& buf,
-                             int begin_bit, int end_bit) {
+                             int begin_bit, int end_bit, const unsigned long long* diff_in) {
     if (n <= 1) return hipSuccess;
     hipError_t e = sort_buffers_alloc(buf, n, (int)sizeof(K));
     if (e != hipSuccess) return e;
     const int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
+    // the digits that vary: given, or found by one pass over the keys (the
+    // stream blocks once, here, and never between the passes)
+    unsigned long long diff;
+    if (diff_in) {
+        diff = *diff_in;
+    } else {
+        unsigned long long* d = (unsigned long long*)buf.totals;
+        if ((e = fill_u64(s, d, 1, 0ull)) != hipSuccess) return e;
+        hipLaunchKernelGGL((k_rs_diff<K>), dim3(grid_for(n)), dim3(256), 0, s, keys, n, d);
+        if ((e = hipMemcpyAsync(buf.totals_host, d, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        diff = *(const unsigned long long*)buf.totals_host;
+    }
     K* kin = keys;
     uint32_t* vin = vals;
     K* kout = (K*)buf.keys_alt;
     uint32_t* vout = buf.vals_alt;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        if (((diff >> shift) & 0xFFull) == 0) continue;  // every key has the same digit: order unchanged
         hipLaunchKernelGGL((k_rs_hist<K>), dim3(nblk), dim3(RS_THREADS), 0, s, kin, n, shift,
                            buf.hist, nblk);
         hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, s, buf.hist, nblk, buf.totals);
-        if ((e = hipMemcpyAsync(buf.totals_host, buf.totals, 256 * 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-        bool constant = false;
-        for (int d = 0; d < 256; ++d)
-            if ((int64_t)buf.totals_host[d] == n) constant = true;
-        if (constant) continue;  // every key has the same digit: order unchanged
         hipLaunchKernelGGL((k_rs_scatter<K>), dim3(nblk), dim3(RS_THREADS), 0, s, kin, vin, kout,
                            vout, n, shift, buf.hist, nblk, buf.totals);
         std::swap(kin, kout);
@@ -257,13 +351,14 @@ static hipError_t radix_sort(hipStream_t s, K* keys, uint32_t* vals, int64_t n, 
 }
 
 hipError_t radix_sort_u64(hipStream_t s, uint64_t* keys, uint32_t* vals, int64_t n,
-                          SortBuffers& buf, int begin_bit, int end_bit) {
-    return radix_sort<uint64_t>(s, keys, vals, n, buf, begin_bit, end_bit);
+                          SortBuffers& buf, int begin_bit, int end_bit,
+                          const unsigned long long* diff) {
+    return radix_sort<uint64_t>(s, keys, vals, n, buf, begin_bit, end_bit, diff);
 }
 
 hipError_t radix_sort_u32(hipStream_t s, uint32_t* keys, uint32_t* vals, int64_t n,
                           SortBuffers& buf, int begin_bit, int end_bit) {
-    return radix_sort<uint32_t>(s, keys, vals, n, buf, begin_bit, end_bit);
+    return radix_sort<uint32_t>(s, keys, vals, n, buf, begin_bit, end_bit, nullptr);
 }
 
 hipError_t make_nu_keys(hipStream_t s, const double* nu, int64_t n, uint64_t* keys,
@@ -305,6 +400,25 @@ hipError_t assemble_lines(hipStream_t s, const double* sE, const int32_t* sg,
 hipError_t widen_f32(hipStream_t s, const float* in, int64_t n, double* out) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_widen, dim3(grid_for(n)), dim3(256), 0, s, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t chunk_keys(hipStream_t s, const double* sE, const uint32_t* up, const uint32_t* lo,
+                      int64_t n, double nu_lo, double nu_hi, uint64_t* keys, uint32_t* vals,
+                      unsigned long long* out) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_keys, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                       s, sE, up, lo, n, nu_lo, nu_hi, keys, vals, out);
+    return hipGetLastError();
+}
+
+hipError_t chunk_gather(hipStream_t s, const uint32_t* perm, int64_t n, const uint32_t* up,
+                        const uint32_t* lo, const double* sE, const int32_t* sg, const ChunkCols& in,
+                        double* nu, double* E, int32_t* g, double* A, double* gamma, double* nexp) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_gather, dim3(grid_for(n)), dim3(256), 0, s, perm, n, up, lo, sE, sg,
+                       in.A32, in.A64, in.gam32, in.gam64, in.nex32, in.nex64, nu, E, g, A, gamma,
+                       nexp);
     return hipGetLastError();
 }
 

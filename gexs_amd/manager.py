@@ -171,25 +171,54 @@ class HipManager:
         """Upload a synth.LineList (states + transitions) and sort it on the device.
         A, gamma and n go up in the list's own precision (FP32 or FP64); the
         device holds them in FP64 either way."""
-        E = np.ascontiguousarray(lst.E, dtype=np.float64)
-        g = np.ascontiguousarray(lst.g, dtype=np.int32)
-        up = np.ascontiguousarray(lst.upper, dtype=np.uint32)
-        lo = np.ascontiguousarray(lst.lower, dtype=np.uint32)
-        st = _lib.States(E.shape[0], E.ctypes.data, g.ctypes.data)
-        f64 = lst.A.dtype == np.float64
-        dt = np.float64 if f64 else np.float32
-        A = np.ascontiguousarray(lst.A, dtype=dt)
-        ga = np.ascontiguousarray(lst.gamma, dtype=dt)
-        nx = np.ascontiguousarray(lst.nexp, dtype=dt)
-        arrs = [A.ctypes.data, ga.ctypes.data, nx.ctypes.data]
-        none = [None, None, None]
-        tr = _lib.Transitions(up.shape[0], up.ctypes.data, lo.ctypes.data,
-                              *(none + arrs if f64 else arrs + none))
+        st, tr, keep = self._list_structs(lst)
         self._chk(self._L.gexs_load_list(self._h, C.byref(st), C.byref(tr)))
+        del keep
         n = C.c_int64()
         self._chk(self._L.gexs_line_count(self._h, C.byref(n)))
         self._n_lines = n.value
         return n.value
+
+    @staticmethod
+    def _list_structs(lst):
+        """(States, Transitions, arrays kept alive) of a synth.LineList; A,
+        gamma and n go in the list's own precision."""
+        E = np.ascontiguousarray(lst.E, dtype=np.float64)
+        g = np.ascontiguousarray(lst.g, dtype=np.int32)
+        up = np.ascontiguousarray(lst.upper, dtype=np.uint32)
+        lo = np.ascontiguousarray(lst.lower, dtype=np.uint32)
+        f64 = lst.A.dtype == np.float64
+        dt = np.float64 if f64 else np.float32
+        cols = [np.ascontiguousarray(a, dtype=dt) if a is not None else None
+                for a in (lst.A, lst.gamma, lst.nexp)]
+        ptrs = [a.ctypes.data if a is not None else None for a in cols]
+        none = [None, None, None]
+        st = _lib.States(E.shape[0], E.ctypes.data, g.ctypes.data)
+        tr = _lib.Transitions(up.shape[0], up.ctypes.data, lo.ctypes.data,
+                              *(none + ptrs if f64 else ptrs + none))
+        return st, tr, (E, g, up, lo, cols)
+
+    def stream_xsec(self, lst, tps, start, end, npoints, mem_budget, out=None):
+        """gexs_stream_xsec: cross-sections at [(T, P[, Q]), ...] of a list that
+        stays in host memory, streamed through the device in chunks that fit
+        mem_budget bytes.  Returns (grids, stats): grids a (len(tps), npoints)
+        float64 array (``out`` if given; out="device" leaves them in the
+        library's device buffer and returns None)."""
+        st, tr, keep = self._list_structs(lst)
+        arr = (_lib.TP * len(tps))()
+        for i, t in enumerate(tps):
+            arr[i] = _lib.TP(float(t[0]), float(t[1]), float(t[2]) if len(t) > 2 else 0.0)
+        grid = _lib.Grid(float(start), float(end), int(npoints), 0)
+        stats = _lib.StreamStats()
+        dev = isinstance(out, str) and out == "device"
+        if not dev and out is None:
+            out = np.empty((len(tps), int(npoints)), dtype=np.float64)
+        self._chk(self._L.gexs_stream_xsec(self._h, C.byref(st), C.byref(tr), arr, len(tps),
+                                           C.byref(grid), int(mem_budget),
+                                           None if dev else _ptr(out), C.byref(stats)))
+        del keep
+        info = {k: getattr(stats, k) for k, _ in _lib.StreamStats._fields_ if k != "reserved"}
+        return (None if dev else out), info
 
     def set_launch(self, persistent_blocks=0, use_graph=False):
         self._chk(self._L.gexs_set_launch(self._h, int(persistent_blocks), 1 if use_graph else 0))

@@ -219,6 +219,48 @@ int gexs_run_batch(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* g
  * the buffer's capacity in doubles; valid until the next run_batch or
  * gexs_destroy. */
 int gexs_batch_device_out(gexs_ctx* ctx, const double** out, int64_t* count);
+/* ----------------------------------------------------- streamed host list
+ * Cross-sections of a list that stays in host memory, streamed through the
+ * device in chunks, as the reference driver feeds a manager
+ * (Profiles/BaseProfile.cpp:120-200: transitions read in file order, a chunk
+ * cut when it holds the manager's capacity, :174-181, which
+ * Managers/Accel/GPU/GPUManager.cpp:98-101 derives from free device memory;
+ * then TransferVectors + ExecuteCrossSection per (T,P), accumulated into the
+ * grids).  Here:
+ *   - chunks of `chunk` transitions in list order, the largest that fits
+ *     mem_budget bytes of device memory together with the states, the grids
+ *     and the pass buffers (stats->device_bytes);
+ *   - a chunk is copied (host -> pinned staging on a host thread, then DMA on
+ *     a copy stream) while the chunk before it is sorted and the one before
+ *     that computed; nu = E_u - E_l, the selection start - half_width <= nu
+ *     <= end + half_width (nu != 0) and the sort by nu run on the device;
+ *   - each chunk's pass adds into the ntp grids in the ctx's device buffer
+ *     (gexs_batch_device_out); out (host or device pointer, may be NULL)
+ *     receives them at the end, ntp*npoints doubles row-major by (T,P).
+ * Q <= 0 in tp: computed on the device from st.  The ctx's precision,
+ * constants and tuning apply (persistent launch and tile orders as in
+ * gexs_run_batch).  Synchronous.  Indices outside the state table stop the
+ * call with GEXS_EINVAL (the grids then hold the chunks before it); the
+ * resident list of gexs_load_list, if any, is left as it was. */
+typedef struct gexs_stream_stats {
+    int64_t n_transitions;  /* transitions streamed */
+    int64_t n_lines;        /* of them selected */
+    int64_t n_eval;         /* profile evaluations, summed over the (T,P) */
+    int64_t chunk;          /* transitions per chunk */
+    int64_t device_bytes;   /* device memory the call's buffers take (<= mem_budget) */
+    int32_t n_chunks;
+    int32_t reserved;
+    double wall_ms;         /* the call, host clock */
+    double device_ms;       /* first copy to last pass, HIP events */
+    /* per stage, summed over the chunks (the stages overlap, so the sums may
+     * exceed device_ms): host copies into the pinned staging (host clock),
+     * their DMA to the device, keys + sort + gather, passes (HIP events) */
+    double stage_ms, copy_ms, sort_ms, pass_ms;
+} gexs_stream_stats;
+int gexs_stream_xsec(gexs_ctx* ctx, const gexs_states* st, const gexs_transitions* tr,
+                     const gexs_tp* tp, int ntp, const gexs_grid* grid, int64_t mem_budget,
+                     double* out, gexs_stream_stats* stats);
+
 /* Launch shape of batched passes: persistent_blocks > 0 runs the profile
  * kernel as that many workgroups pulling tiles from a counter (0: one wave
  * per tile); use_graph != 0 captures a batch's device work into a HIP graph
