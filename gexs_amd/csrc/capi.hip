@@ -634,9 +634,9 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, gxk::line_rec(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale, B.rec.as<VRec>(),
                             B.recx.as<VRecX>(), t.kmax, t.neval, c->cus));
-    // Voigt items carry this pass's near-centre bounds (a copy per buffer
-    // set); the other profiles read the shared items unchanged
-    const bool voigt = c->profile == GEXS_VOIGT;
+    // Voigt and Voigt-Quad items carry this pass's near-centre bounds (a
+    // copy per buffer set); the bin-Gauss profiles read the shared items
+    const bool voigt = c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD;
     if (voigt)
         HIPCHK(c, gxk::segments(P, c->items.as<WorkItem>(), B.items.as<WorkItem>(), c->n_items, t.jd,
                                 c->pos.as<LinePos>(), t.kmax, c->profile));
@@ -647,7 +647,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split,
                                  voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>(), c->n_items, t.jobs,
                                  t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
-                                 launch_blocks(c), &jb));
+                                 launch_blocks(c), true));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
     HIPCHK(c, hipEventRecord(B.prof_done, s));
     B.used = true;
@@ -1948,7 +1948,7 @@ int gexs_run_fixed(gexs_ctx* c, const gexs_lines_dev* L, const gexs_tp* tp, cons
     // counter; every shape of one tile width and split gives the same bits
     const int mode = (c->persistent_blocks > 0 && c->order != 2) ? 1 : (c->order == 0 ? 0 : 3);
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, items, (int)ntiles, djob, scale, xq,
-                                 xq, mode, launch_blocks(c), &jb));
+                                 xq, mode, launch_blocks(c), true));
     if (n_eval) {
         unsigned long long ne = 0;
         HIPCHK(c, hipMemcpyAsync(&ne, neval, 8, hipMemcpyDeviceToHost, s));

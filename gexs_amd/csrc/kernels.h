@@ -39,12 +39,11 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
 // dispatch order); 1: `blocks` persistent workgroups on one counter (xq[0]);
 // 2: `blocks` persistent workgroups on per-XCD queues (xq = 8 counters, zero
 // at launch; xqb = 9 item bounds)
-// job_value != nullptr: all items belong to that one job, passed by value
-// (jobs is then not read)
+// single_job: all items belong to jobs[0] (read without waiting for the item)
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,
                          unsigned int* xq, const unsigned int* xqb, int mode, int blocks,
-                         const Job* job_value = nullptr);
+                         bool single_job = false);
 hipError_t line_f64(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, int n,
                     const LinePos* pos, const TPParams* tp, LineF64* out,

@@ -179,12 +179,21 @@ __device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double
     r.Kg = (float)amp;
     if (voigt) {
         const double y = (gamma != nullptr) ? gamma_l(p, gamma[i], nexp[i]) * iw : 0.0;
-        r.Kl = (float)(amp * 0.5641896 * y);
+        const bool quad = p.profile == GX_PROF_VOIGT_QUAD;
+        // Voigt: Humlicek's region-I constant; Voigt-Quad: 1/sqrt(pi)
+        r.Kl = (float)(amp * (quad ? 0.5641895835477563 : 0.5641896) * y);
         r.E2 = (float)(y * y - 0.5);
         r.H = (float)(2.0 * y * y);
         x.y = (float)y;
-        // |x| >= |c - j| A - |B|; region I needs |x| + y >= 15 (with margin)
-        const double kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
+        // |x| >= |c - j| A - |B|; region I needs |x| + y >= 15 (Voigt) or
+        // x^2 + y^2 >= kQuadR2 (Voigt-Quad), with margin
+        double kk;
+        if (quad) {
+            const double xm2 = (double)gx::kQuadR2 * (1.0 + 1e-5) - y * y;
+            kk = xm2 > 0.0 ? (sqrt(xm2) + fabs(Bf)) / Af : 0.0;
+        } else {
+            kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
+        }
         x.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e8 ? 100000000 : (int)ceil(kk));
     } else {
         r.Kl = 0.0f;
@@ -539,53 +548,54 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
     }
 }
 
+// Points outside the profile kernel's region I, and their value.  Voigt:
+// Humlicek's regions II-IV (|x| + y < 15).  Voigt-Quad: x^2 + y^2 < kQuadR2,
+// the Zaghloul-Ali core and the Gauss-Hermite wings (gx::quad_inner).  Beyond,
+// both profiles take the region-I rational of the line's record.
+template <int PROF>
+__device__ __forceinline__ bool inner_point(float x, float y) {
+    if constexpr (PROF == GX_PROF_VOIGT_QUAD) return fmaf(x, x, y * y) < gx::kQuadR2;
+    return x + y < 15.0f;
+}
+template <int PROF>
+__device__ __forceinline__ float inner_value(float x, float y) {
+    if constexpr (PROF == GX_PROF_VOIGT_QUAD) return gx::quad_inner(x, y);
+    return gx::w4_f32_inner(x, y);
+}
+
 // one line with per-line decisions: window masks per lane (only the edges
 // MASK names; slices wholly outside the window are skipped, slices wholly
-// inside are unmasked), region I inline, regions II-IV per lane
-// DEFER (Voigt): points outside region I add nothing here; their W4 values
-// come from batch_w4.
+// inside are unmasked), region I inline, the inner points per lane
+// DEFER: inner points add nothing here; their values come from batch_w4.
 template <int PROF, int PPL, int MASK, bool DEFER = false>
 __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp, const VRec& r,
                                              const VRecX& rx, int w0, int w1) {
-    if constexpr (PROF == GX_PROF_VOIGT) {
-        const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
-        if (kmin >= rx.k15) {
-#pragma unroll
-            for (int q = 0; q < PPL; ++q) {
-                const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
-                if (st == 0) continue;
-                const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
-                if (st == 2)
-                    t.part[q] += v;
-                else
-                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < PPL; ++q) {
-                const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
-                if (st == 0) continue;
-                const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
-                float v = region1(x, r.Kl, r.E2, r.H);
-                if (x + rx.y < 15.0f) v = DEFER ? 0.0f : r.Kg * gx::w4_f32_inner(x, rx.y);
-                if (st == 2)
-                    t.part[q] += v;
-                else
-                    t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
-            }
-        }
-    } else {
-        static_assert(PROF == GX_PROF_VOIGT_QUAD, "bin-Gauss lines run gauss_staged");
+    static_assert(PROF == GX_PROF_VOIGT || PROF == GX_PROF_VOIGT_QUAD,
+                  "bin-Gauss lines run gauss_staged");
+    const int kmin = lp.c < w0 ? (w0 - lp.c) : (lp.c >= w1 ? (lp.c - w1 + 1) : 0);
+    if (kmin >= rx.k15) {
 #pragma unroll
         for (int q = 0; q < PPL; ++q) {
             const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
             if (st == 0) continue;
-            if (st == 2 || in_window<MASK>(t.jj[q], lp)) {
-                const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
-                const float rr = x * x + rx.y * rx.y;
-                const float k = rr < 100.0f ? gx::voigt916_f32(x, rx.y) : gx::gh11_f32(x, rx.y);
-                t.part[q] = fmaf(r.Kg, k, t.part[q]);
-            }
+            const float v = region1(fmaf(r.cf - t.jf[q], r.A, r.B), r.Kl, r.E2, r.H);
+            if (st == 2)
+                t.part[q] += v;
+            else
+                t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) {
+            const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+            if (st == 0) continue;
+            const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
+            float v = region1(x, r.Kl, r.E2, r.H);
+            if (inner_point<PROF>(x, rx.y)) v = DEFER ? 0.0f : r.Kg * inner_value<PROF>(x, rx.y);
+            if (st == 2)
+                t.part[q] += v;
+            else
+                t.part[q] += in_window<MASK>(t.jj[q], lp) ? v : 0.0f;
         }
     }
     t.tick(1);
@@ -630,7 +640,7 @@ __device__ __forceinline__ void lines_checked(TileAcc<PPL>& t, ConstPos pos, Con
         checked_line<PROF, PPL, MASK>(t, pos[i], to_pass(rec[i], t.E), recx[i], w0, w1);
 }
 
-// Voigt lines centred near the tile (kMaskNone) staged like the region-I
+// Lines centred near the tile (kMaskNone) staged like the region-I
 // loop: per batch of 64 lines lane k loads line k's LinePos, VRec and VRecX
 // with vector loads and writes a 12-word row (cf, A, B, Kl | E2, H, Kg, y |
 // c, k15); each line then reads its row with three broadcast ds_read_b128,
@@ -648,7 +658,7 @@ constexpr int wave_row() {
     return stage_row<PPL>() > kNearRow ? stage_row<PPL>() : kNearRow;
 }
 
-template <int PPL>
+template <int PROF, int PPL>
 __device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobRec grec,
                                             GlobRecX grecx, int a, int b, int w0, int w1,
                                             int lane, float* xs) {
@@ -693,7 +703,7 @@ __device__ __forceinline__ void near_staged(TileAcc<PPL>& t, GlobPos gpos, GlobR
             LinePos lp;
             lp.c = __builtin_amdgcn_readfirstlane(__float_as_int(rk[8]));
             lp.ib = lp.ie = 0;  // the windows cover the tile (kMaskNone)
-            checked_line<GX_PROF_VOIGT, PPL, kMaskNone>(t, lp, r, rx, w0, w1);
+            checked_line<PROF, PPL, kMaskNone>(t, lp, r, rx, w0, w1);
         }
     }
 }
@@ -809,7 +819,7 @@ __device__ __forceinline__ int bcast(int v, int k) { return __builtin_amdgcn_rea
 // repeats the region test of checked_line bit for bit and evaluates W4;
 // the values then go to the lanes owning the points line by line in batch
 // order (ds_bpermute), so each point sums them in a fixed order.
-template <int MASK>
+template <int PROF, int MASK>
 __device__ __forceinline__ void batch_w4(TileAcc<1>& t, const VRec& rc, const LinePos& pc,
                                          const VRecX& xc, int nb, int w0, int w1, int lane,
                                          int* incl) {
@@ -846,7 +856,7 @@ __device__ __forceinline__ void batch_w4(TileAcc<1>& t, const VRec& rc, const Li
         if (pp < total) {
             const int j = jlk + (pp - exk);
             const float x = fabsf(fmaf(cf - (float)j, A, B));
-            if (x + y < 15.0f) v = Kg * gx::w4_f32_inner(x, y);
+            if (inner_point<PROF>(x, y)) v = Kg * inner_value<PROF>(x, y);
         }
         const int last = min(total - p0, GX_WAVE) - 1;
         const int k0 = __builtin_amdgcn_readlane(k, 0), k1 = __builtin_amdgcn_readlane(k, last);
@@ -874,7 +884,7 @@ template <int PROF, int MASK>
 __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRec rec,
                                               GlobRecX recx, int a, int b, int w0, int w1,
                                               int lane, int* incl) {
-    constexpr bool kDefer = PROF == GX_PROF_VOIGT;
+    constexpr bool kDefer = true;
     if (a >= b) return;
     LinePos pc, pn;
     VRec rc, rn;
@@ -915,7 +925,7 @@ __device__ __forceinline__ void lines_batched(TileAcc<1>& t, GlobPos pos, GlobRe
             rx.k15 = bcast(xc.k15, k);
             checked_line<PROF, 1, MASK, kDefer>(t, lp, r, rx, w0, w1);
         }
-        if (kDefer) batch_w4<MASK>(t, rc, pc, xc, nb, w0, w1, lane, incl);
+        if (kDefer) batch_w4<PROF, MASK>(t, rc, pc, xc, nb, w0, w1, lane, incl);
         rc = rn;
         pc = pn;
         xc = xn;
@@ -954,8 +964,8 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         const GlobPos gpos = (GlobPos)jb.pos;
         const GlobRec grec = (GlobRec)jb.rec;
         const GlobRecX grecx = (GlobRecX)jb.recx;
-        if (PROF == GX_PROF_VOIGT && it.s1 < it.s4) {
-            static_assert(PROF == GX_PROF_VOIGT || PROF == GX_PROF_VOIGT_QUAD, "Voigt profiles");
+        static_assert(PROF == GX_PROF_VOIGT || PROF == GX_PROF_VOIGT_QUAD, "Voigt profiles");
+        if (it.s1 < it.s4) {
             const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
             const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
             // the region-I segments through the staged loop (6 VALU per
@@ -975,7 +985,7 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         } else {
             lines_batched<PROF, kMaskBoth>(t, gpos, grec, grecx, ca, cb, w0, w1, lane, incl);
         }
-    } else if constexpr (PROF == GX_PROF_VOIGT) {
+    } else {
         const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
         const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
         if (it.s1 < it.s4) {
@@ -990,7 +1000,8 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             else
                 lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
             voigt_staged<PPL, kMaskNone>(t, grec, pos, s1, s2, w0, lane, xs);
-            near_staged<PPL>(t, (GlobPos)jb.pos, grec, (GlobRecX)jb.recx, s2, s3, w0, w1, lane, xs);
+            near_staged<PROF, PPL>(t, (GlobPos)jb.pos, grec, (GlobRecX)jb.recx, s2, s3, w0, w1, lane,
+                                   xs);
             voigt_staged<PPL, kMaskNone>(t, grec, pos, s3, s4, w0, lane, xs);
             if (it.s3 < it.s4)
                 voigt_staged<PPL, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
@@ -999,8 +1010,6 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         } else {
             lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
         }
-    } else {
-        lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
     }
     t.fold();
 }
@@ -1029,7 +1038,7 @@ __device__ __forceinline__ void sum_share(TileAcc<PPL>& t, const WorkItem& it, c
                                           int lane, int k, int K, float* xs) {
     const int lo = it.lo, hi = max(it.hi, it.lo);
     const int m2 = min(max(it.s2, lo), hi), m3 = min(max(it.s3, lo), hi);
-    const bool three = PROF == GX_PROF_VOIGT && K > 1 && it.s1 < it.s4 && m3 - m2 >= K;
+    const bool three = PROF != GX_PROF_DOPPLER && K > 1 && it.s1 < it.s4 && m3 - m2 >= K;
     const int np = three ? 3 : 1;
     // one loop, so the kernel holds one copy of sum_lines
 #pragma nounroll
@@ -1111,8 +1120,10 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 }  // namespace
 
-// jobs == nullptr: every item belongs to the one job passed by value (job0),
-// which spares each wave a dependent load of the job table.
+// single != 0: every item belongs to jobs[0], which the waves then load
+// without waiting for their item (no dependent load).  The choice is an index
+// select: a select between a loaded Job and one passed by value puts the Job's
+// fields in VGPRs and spills the line loops' registers.
 // mode 0: one item per wave (SPLIT 1) or per workgroup (SPLIT 4, 16), blocks in
 // XCD-contiguous order; 3: as 0 in dispatch order (consecutive items on
 // different XCDs); 1: persistent, items pulled from one counter; 2:
@@ -1133,7 +1144,7 @@ constexpr int waves_target() { return PPL >= 8 ? 3 : ((PPL == 4 && SPLIT == 16) 
 template <int PROF, int PPL, int SPLIT>
 __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves_per_eu(waves_target<PPL, SPLIT>()))) void k_profile_tiles(const WorkItem* __restrict__ items,
                                                          int nitems, const Job* __restrict__ jobs,
-                                                         Job job0,
+                                                         int single,
                                                          const unsigned long long* __restrict__ scale_bits,
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
@@ -1155,12 +1166,12 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             const int idx = lb * kWavesPerBlock + wv;
             if (idx >= nitems) return;
             const WorkItem it = items[idx];
-            const Job jb = jobs ? jobs[it.job] : job0;
+            const Job jb = jobs[single ? 0 : it.job];
             run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
         } else {
             if (lb >= nitems) return;
             const WorkItem it = items[lb];
-            const Job jb = jobs ? jobs[it.job] : job0;
+            const Job jb = jobs[single ? 0 : it.job];
             run_item_split<PROF, PPL, SPLIT>(it, jb, scale_bits, lane, wv, red, xs);
         }
         return;
@@ -1185,7 +1196,7 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             }
             if (idx >= end) break;
             const WorkItem it = items[idx];
-            const Job jb = jobs ? jobs[it.job] : job0;
+            const Job jb = jobs[single ? 0 : it.job];
             if (SPLIT == 1)
                 run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
             else
@@ -1401,7 +1412,8 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
 
 hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,
                     const JobDesc* jd, const LinePos* pos, const int* kmax, int profile) {
-    if (nitems <= 0 || profile != GX_PROF_VOIGT) return hipSuccess;  // only Voigt reads them
+    // only the Voigt profiles read them
+    if (nitems <= 0 || (profile != GX_PROF_VOIGT && profile != GX_PROF_VOIGT_QUAD)) return hipSuccess;
     hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 256)), dim3(256), 0, s, in, out, nitems,
                        jd, pos, kmax);
     return hipGetLastError();
@@ -1433,7 +1445,7 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
 
 template <int PROF, int PPL, int SPLIT>
 static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitems,
-                                 const Job* jobs, const Job& job0,
+                                 const Job* jobs, int single,
                                  const unsigned long long* scale_bits, unsigned int* xq,
                                  const unsigned int* xqb, int mode, int blocks) {
     if (nitems <= 0) return hipSuccess;
@@ -1442,54 +1454,50 @@ static hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitem
     const int nb = slots ? blocks_for(nitems, ipb)
                          : std::max(1, std::min(blocks, blocks_for(nitems, ipb)));
     hipLaunchKernelGGL((k_profile_tiles<PROF, PPL, SPLIT>), dim3(nb), dim3(block_threads<SPLIT>()), 0, s, items,
-                       nitems, jobs, job0, scale_bits, xq, xqb, mode);
+                       nitems, jobs, single, scale_bits, xq, xqb, mode);
     return hipGetLastError();
 }
 
 template <int PROF, int SPLIT>
 static hipError_t launch_ppl(hipStream_t s, int ppl, const WorkItem* items, int nitems,
-                             const Job* jobs, const Job& job0, const unsigned long long* scale_bits,
+                             const Job* jobs, int single, const unsigned long long* scale_bits,
                              unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     switch (ppl) {
-        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
-        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
-        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
-        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
+        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
+        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
+        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
+        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <int PROF>
 static hipError_t launch_prof(hipStream_t s, int ppl, int split, const WorkItem* items,
-                              int nitems, const Job* jobs, const Job& job0,
+                              int nitems, const Job* jobs, int single,
                               const unsigned long long* scale_bits,
                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
-    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
-    if (split == 4) return launch_ppl<PROF, 4>(s, ppl, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
+    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
+    if (split == 4) return launch_ppl<PROF, 4>(s, ppl, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
     if (split == 16)
-        return launch_ppl<PROF, 16>(s, ppl, items, nitems, jobs, job0, scale_bits, xq, xqb, mode, blocks);
+        return launch_ppl<PROF, 16>(s, ppl, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
     return hipErrorInvalidValue;
 }
 
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,
                          unsigned int* xq, const unsigned int* xqb, int mode, int blocks,
-                         const Job* job_value) {
-    Job job0{};
-    if (job_value) {  // one job, passed to the kernel by value
-        job0 = *job_value;
-        jobs = nullptr;
-    }
+                         bool single_job) {
+    const int single = single_job ? 1 : 0;
     switch (profile) {
         case GX_PROF_VOIGT:
-            return launch_prof<GX_PROF_VOIGT>(s, ppl, split, items, nitems, jobs, job0, scale_bits, xq, xqb, mode,
+            return launch_prof<GX_PROF_VOIGT>(s, ppl, split, items, nitems, jobs, single, scale_bits, xq, xqb, mode,
                                               blocks);
         case GX_PROF_VOIGT_QUAD:
-            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, job0, scale_bits, xq, xqb,
+            return launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, single, scale_bits, xq, xqb,
                                                    mode, blocks);
         case GX_PROF_DOPPLER:
         case GX_PROF_GAUSSIAN:
-            return launch_prof<GX_PROF_DOPPLER>(s, ppl, split, items, nitems, jobs, job0, scale_bits, xq, xqb,
+            return launch_prof<GX_PROF_DOPPLER>(s, ppl, split, items, nitems, jobs, single, scale_bits, xq, xqb,
                                                 mode, blocks);
         default:
             return hipErrorInvalidValue;

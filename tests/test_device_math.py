@@ -47,3 +47,54 @@ def test_erfc_nonneg_accuracy():
     bad[5] *= 1.001
     worse = erfc_nonneg_f32(z, bad).astype(np.float64)
     assert np.max(np.abs(worse - ref)) > 1e-6
+
+
+def quad_r2():
+    src = open(SRC).read()
+    return float(re.search(r"constexpr float kQuadR2 = ([0-9.e]+)f;", src).group(1))
+
+
+def test_voigt_quad_far_rule_against_the_11_point_sum(O):
+    """Voigt-Quad beyond the Zaghloul-Ali core (x^2 + y^2 >= 100): the oracle
+    sums the reference's 11-point Gauss-Hermite rule in FP64.  The kernel takes
+    the 4-point rule (Humlicek's region-II pairs) up to kQuadR2 and the one-term
+    continued fraction (the region-I rational with 1/sqrt(pi)) beyond.  Both
+    stay within 1.2e-6 of the oracle's value relative to it, also with the
+    4-point pairs in FP32; the region-I rational at x^2 + y^2 >= 900 (a lower
+    threshold) does not."""
+    r2 = quad_r2()
+    assert r2 == 1600.0
+    rng = np.random.default_rng(21)
+    n = 4000
+    y = np.concatenate([10 ** rng.uniform(-6, 1.6, n), rng.uniform(0, 45, n)])
+    rr = np.concatenate([rng.uniform(100.0, r2, n), 10 ** rng.uniform(2, np.log10(r2), n)])
+    keep = rr > y * y
+    y, rr = y[keep], rr[keep]
+    x = np.sqrt(rr - y * y)
+    ref = np.array([O.gh11(a, b) for a, b in zip(x, y)])
+
+    t, w = np.polynomial.hermite.hermgauss(4)
+    a2 = np.sort(t[t > 0] ** 2)
+    c = 2 * w[t > 0][np.argsort(t[t > 0])] / np.pi
+
+    def gh4(x, y, dt):
+        x, y = dt(x), dt(y)
+        x2 = x * x
+        b = (x2 + y * y)[:, None] + a2.astype(dt)
+        d = b * b - dt(4) * a2.astype(dt) * x2[:, None]
+        return (y * np.sum(c.astype(dt) * b / d, axis=1)).astype(np.float64)
+
+    assert np.max(np.abs(gh4(x, y, np.float64) - ref) / ref) < 2e-7
+    assert np.max(np.abs(gh4(x, y, np.float32) - ref) / ref) < 1.2e-6
+
+    def rational(x, y):
+        s = x * x + y * y - 0.5
+        return y * (s + 1.0) / (s * s + 2 * y * y) / np.sqrt(np.pi)
+
+    yf = np.concatenate([10 ** rng.uniform(-6, 3, n), rng.uniform(0, 60, n)])
+    xf = np.sqrt(np.maximum(r2 - yf * yf, 0.0)) + 10 ** rng.uniform(-6, 4, 2 * n)
+    far = np.array([O.gh11(a, b) for a, b in zip(xf, yf)])
+    assert np.max(np.abs(rational(xf, yf) - far) / far) < 1.2e-6
+    xl = np.sqrt(np.maximum(900.0 - yf * yf, 0.0)) + 10 ** rng.uniform(-6, 0, 2 * n)
+    low = np.array([O.gh11(a, b) for a, b in zip(xl, yf)])
+    assert np.max(np.abs(rational(xl, yf) - low) / low) > 2e-6

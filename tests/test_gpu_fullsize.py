@@ -72,6 +72,38 @@ def test_voigt_1e6_full_grid(c1_full, path, ppl, split, order):
     assert err <= 1e-4 and band <= 1e-6
 
 
+@pytest.mark.parametrize("ppl,split,order", [(2, 4, 3), (1, 4, 3)])
+def test_voigt_quad_1e6_full_grid(O, ppl, split, order):
+    """The 10^6-line workload with the Voigt-Quad profile (the bench's second
+    profile line) over the whole grid, at the launch shapes its autotuner picks."""
+    spec = synth.WORKLOADS["voigt_1e6"]
+    L = synth.workload_list("voigt_1e6")
+    start, end, N = spec["grid"]
+    dnu = (end - start) / N
+    T, P = spec["T"], spec["P"]
+    nu, E, A, g, gam, nex = L.lines()
+    Q = O.partition(L.E, L.g, T)
+    p = O.params("voigt_quad", T=T, P=P, Q=Q, mass=spec["mass"], cutoff=25.0, p_scale=P_SCALE)
+    ref, ne = O.cross_section(p, start, dnu, N, nu, E, A, g, gam, nex, nthreads=NTHREADS)
+    m = HipManager("voigt_quad")
+    m.set_tuning(ppl, order, split)
+    m.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, 296.0, P_SCALE, cutoff=25.0)
+    n = m.load_list(L)
+    m.InitializeVectors(N, 1)
+    m.TransferFreq(start + np.arange(N, dtype=np.float64) * dnu)
+    m.ExecuteCrossSection(N, n, 0, T, P, Q)
+    out = np.zeros(N)
+    m.TransferResults(None, out, N, 0)
+    ne_gpu = m.stats()["n_eval"]
+    m.close()
+    assert ne_gpu == ne
+    err = np.max(np.abs(out - ref)) / ref.max()
+    band = abs(out.sum() - ref.sum()) / ref.sum()
+    log(f"voigt_1e6 voigt-quad full grid ppl={ppl} split={split} order={order}: N_eval {ne:.4e} "
+        f"peak-rel {err:.3e} band {band:.3e}")
+    assert err <= 1e-4 and band <= 1e-6
+
+
 def sample_points(rng, N, start, dnu, nu_strong, windows=3, nrand=10_000, wwidth=10.0):
     pts = [rng.integers(0, N, nrand)]
     for v in nu_strong:
