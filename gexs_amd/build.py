@@ -8,6 +8,7 @@ explicit ``#pragma clang fp contract(off)`` regions, not on global flags.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -19,7 +20,8 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libgexs_hip.so")
 PEAK_LIB = os.path.join(HERE, "libgexs_peak.so")
 
-SOURCES = ["kernels.hip", "sort.hip", "serve.hip", "capi.hip"]
+SOURCES = ["kernels.hip", "tiles_voigt.hip", "tiles_quad.hip", "tiles_gauss.hip", "sort.hip",
+           "serve.hip", "capi.hip"]
 HOST_SOURCES = ["HipManager.cpp", "host/readers.cpp", "host/cross_section.cpp"]
 CLI = os.path.join(HERE, "gexs_xsec")
 ARCH = os.environ.get("GEXS_OFFLOAD_ARCH", "gfx950")
@@ -38,37 +40,38 @@ def flags():
             "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 
+def _compile(cmd, what, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"{what} failed")
+
+
 def build(verbose: bool = False) -> str:
     cc = hipcc()
-    objs = []
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
+    jobs = []  # (command, description, object)
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(bdir, src.replace(".hip", ".o"))
-        cmd = [cc, *flags(), "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed on {src}")
-        objs.append(obj)
+        jobs.append(([cc, *flags(), "-c", path, "-o", obj], f"hipcc on {src}", obj))
     cxx = shutil.which("g++") or "g++"
     hflags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"),
               "-I", CSRC]
     for src in HOST_SOURCES:
         obj = os.path.join(bdir, os.path.basename(src).replace(".cpp", ".o"))
-        cmd = [cxx, *hflags, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"g++ failed on {src}")
-        objs.append(obj)
+        jobs.append(([cxx, *hflags, "-c", os.path.join(CSRC, src), "-o", obj], f"g++ on {src}",
+                     obj))
+    # the translation units are independent: compile them side by side
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+        for f in [ex.submit(_compile, cmd, what, verbose) for cmd, what, _ in jobs]:
+            f.result()
+    objs = [obj for _, _, obj in jobs]
     tmp = LIB + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread",
            "-l:libbz2.so.1"]

@@ -264,6 +264,9 @@ struct gexs_ctx {
     struct PassBuf {
         DevArr tab, small, rec, recx, items;
         hipEvent_t pre_done = nullptr, prof_done = nullptr;
+        // after the set's last profile kernel: prof_done, or that execute's
+        // timing event (a later record of it on `stream` marks a later point)
+        hipEvent_t done = nullptr;
         bool used = false;
     };
     PassBuf pb[kPassSets];
@@ -621,7 +624,13 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
         HIPCHK(c, hipStreamWaitEvent(P, c->geom, 0));
         c->geom_new = false;
     }
-    if (B.used) HIPCHK(c, hipStreamWaitEvent(P, B.prof_done, 0));  // buffer set free
+    if (B.used) {  // buffer set free: a device-side wait only if its last kernel still runs
+        const hipError_t q = hipEventQuery(B.done);
+        if (q == hipErrorNotReady)
+            HIPCHK(c, hipStreamWaitEvent(P, B.done, 0));
+        else if (q != hipSuccess)
+            HIPCHK(c, q);
+    }
     HIPCHK(c, hipEventRecord(c->ev_start, P));
     // the pass's tables go in as kernel arguments of the set-up kernel, which
     // also clears the pass counters: no host staging copy per pass
@@ -648,8 +657,13 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
                                  voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>(), c->n_items, t.jobs,
                                  t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
                                  launch_blocks(c), true));
-    if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, s));
-    HIPCHK(c, hipEventRecord(B.prof_done, s));
+    if (timed) {
+        HIPCHK(c, hipEventRecord(c->ev_k1, s));
+        B.done = c->ev_k1;
+    } else {
+        HIPCHK(c, hipEventRecord(B.prof_done, s));
+        B.done = B.prof_done;
+    }
     B.used = true;
     c->last.n_tiles = c->n_items;
     c->last_neval = t.neval;
@@ -718,6 +732,12 @@ int gexs_create(int device, int profile, gexs_ctx** out) {
     }
     c->tcount = 0;
     scratch_init(device);  // the device is current (set_device above)
+    // the Gaussian profiles' erfc table of this device (rewritten with the
+    // same values by every context created on it)
+    if (gxk::init_gauss_tables(c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        gexs_destroy(c);
+        return GEXS_EHIP;
+    }
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)

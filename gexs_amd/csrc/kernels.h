@@ -39,6 +39,20 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
 // dispatch order); 1: `blocks` persistent workgroups on one counter (xq[0]);
 // 2: `blocks` persistent workgroups on per-XCD queues (xq = 8 counters, zero
 // at launch; xqb = 9 item bounds)
+// fills the bin-averaged Gaussian's erfc table on the current device (once
+// per device before its first Gaussian profile kernel; idempotent)
+hipError_t init_gauss_tables(hipStream_t s);
+// the profile kernel of one profile (tiles_voigt.hip, tiles_quad.hip,
+// tiles_gauss.hip); single != 0: all items belong to jobs[0]
+hipError_t profile_tiles_voigt(hipStream_t s, int ppl, int split, const WorkItem* items, int nitems,
+                               const Job* jobs, int single, const unsigned long long* scale_bits,
+                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
+hipError_t profile_tiles_quad(hipStream_t s, int ppl, int split, const WorkItem* items, int nitems,
+                              const Job* jobs, int single, const unsigned long long* scale_bits,
+                              unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
+hipError_t profile_tiles_gauss(hipStream_t s, int ppl, int split, const WorkItem* items, int nitems,
+                               const Job* jobs, int single, const unsigned long long* scale_bits,
+                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
 // single_job: all items belong to jobs[0] (read without waiting for the item)
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,

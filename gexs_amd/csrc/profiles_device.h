@@ -274,29 +274,24 @@ __device__ __forceinline__ double gh11_f64(double x, double y) {
     return acc * y / 3.14159265358979323846;
 }
 
-// erfc(z) for z >= 0 in FP32, without branches: erfc(z) = t exp(p(u) - z^2),
-// t = 1 / (1 + z/2) in (0, 1], u = 2t - 1, where p is the degree-9
-// polynomial in u fitted by least squares, at 600 Chebyshev nodes of t, to
-// log(erfc(z) e^{z^2} / t) evaluated in 30-digit arithmetic (a smooth function
-// of t, tending to -log(2 sqrt(pi)) as z -> infinity).  The fit is within 1e-7
-// relative of erfc; evaluated in FP32 the result is within 4e-7 absolute of
-// erfc (the roundings of t and of the exponential near erfc ~ 1) and 2.3e-6
-// relative up to z = 4, beyond which erfc < 2e-8.  About 15 VALU against ~50
-// for erfcf.
-__device__ __forceinline__ float erfc_nonneg(float z) {
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.0f));
-    const float u = fmaf(2.0f, t, -1.0f);
-    float p = 3.33735893946141e-4f;
-    p = fmaf(p, u, -2.0790912094525993e-4f);
-    p = fmaf(p, u, -2.0487348083406687e-3f);
-    p = fmaf(p, u, 1.776564633473754e-3f);
-    p = fmaf(p, u, 8.703812025487423e-3f);
-    p = fmaf(p, u, -9.8737683147192e-3f);
-    p = fmaf(p, u, -4.6874888241291046e-2f);
-    p = fmaf(p, u, 4.734310507774353e-2f);
-    p = fmaf(p, u, 6.726422309875488e-1f);
-    p = fmaf(p, u, -6.71794056892395e-1f);
-    return t * __expf(fmaf(-z, z, p));
+// erfc of the bin-averaged Gaussian as kErfcPieces cubic pieces over
+// [-8, 8), kErfcPerUnit per unit: piece k, e = k / kErfcPerUnit - 8 + f /
+// kErfcPerUnit with f in [0, 1), is the cubic Hermite interpolant of erfc
+// and its derivative -2/sqrt(pi) exp(-e^2) at the piece's ends, kept as FP32
+// coefficients (a0, a1, a2, a3) of f (gxk::init_gauss_tables computes them
+// in FP64 on the device).  Interpolation error: the largest fourth
+// derivative of erfc (5.8) / 384 / kErfcPerUnit^4 = 1.4e-8; the FP32
+// evaluation adds ~2 ulp of erfc.
+constexpr int kErfcPieces = 512;
+constexpr float kErfcPerUnit = 32.0f;
+
+__device__ __forceinline__ double4 erfc_piece(int k) {
+    const double e0 = (double)k / (double)kErfcPerUnit - 8.0;
+    const double e1 = (double)(k + 1) / (double)kErfcPerUnit - 8.0;
+    const double H0 = erfc(e0), H1 = erfc(e1);
+    const double s = -1.1283791670955126 / (double)kErfcPerUnit;  // -2/sqrt(pi) per piece
+    const double D0 = s * exp(-e0 * e0), D1 = s * exp(-e1 * e1);
+    return make_double4(H0, D0, 3.0 * (H1 - H0) - 2.0 * D0 - D1, 2.0 * (H0 - H1) + D0 + D1);
 }
 
 }  // namespace gx

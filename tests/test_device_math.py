@@ -5,48 +5,75 @@ import os
 import re
 
 import numpy as np
-from scipy.special import erfc
+from scipy.special import erf, erfc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "gexs_amd", "csrc", "profiles_device.h")
 
 
-def erfc_nonneg_coefficients():
+def erfc_table_constants():
     src = open(SRC).read()
-    body = src[src.index("float erfc_nonneg(float z)"):]
-    body = body[:body.index("}")]
-    lead = re.search(r"float p = ([-0-9.e]+)f;", body).group(1)
-    rest = re.findall(r"p = fmaf\(p, u, ([-0-9.e]+)f\);", body)
-    return [float(lead)] + [float(c) for c in rest]  # highest degree first
+    n = int(re.search(r"constexpr int kErfcPieces = (\d+);", src).group(1))
+    per = float(re.search(r"constexpr float kErfcPerUnit = ([0-9.]+)f;", src).group(1))
+    return n, per
 
 
-def erfc_nonneg_f32(z, coef):
+def erfc_pieces(n, per, swap=False):
+    """FP32 coefficients (a0, a1, a2, a3) of each piece: the cubic Hermite
+    interpolant of erfc and its derivative at the piece's ends."""
+    k = np.arange(n, dtype=np.float64)
+    e0, e1 = k / per - 8.0, (k + 1) / per - 8.0
+    H0, H1 = erfc(e0), erfc(e1)
+    s = -2.0 / np.sqrt(np.pi) / per
+    D0, D1 = s * np.exp(-e0 * e0), s * np.exp(-e1 * e1)
+    a2 = 3.0 * (H1 - H0) - 2.0 * D0 - D1
+    a3 = 2.0 * (H0 - H1) + D0 + D1
+    if swap:
+        a2, a3 = a3, a2
+    return np.stack([H0, D0, a2, a3], axis=1).astype(np.float32)
+
+
+def fma32(a, b, c):
+    # the float32 product is exact in float64; one rounding to float32 after
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(np.float32)
+
+
+def erfc_lookup_f32(e, tab, n, per):
     f = np.float32
-    z = z.astype(f)
-    t = f(1) / (f(0.5) * z + f(1))
-    u = f(2) * t - f(1)
-    p = np.full_like(z, f(coef[0]))
-    for c in coef[1:]:
-        p = p * u + f(c)
-    return t * np.exp(p - z * z)
+    e = e.astype(f)
+    v = np.clip(e * f(per), f(-n / 2), f(n / 2) - f(1.5e-5))  # exact scaling
+    fl = np.floor(v)
+    fr = (v - fl).astype(f)
+    c = tab[fl.astype(np.int64) + n // 2]
+    return fma32(fma32(fma32(c[:, 3], fr, c[:, 2]), fr, c[:, 1]), fr, c[:, 0])
 
 
-def test_erfc_nonneg_accuracy():
-    coef = erfc_nonneg_coefficients()
-    assert len(coef) == 10
-    z = np.concatenate([np.linspace(0.0, 12.0, 300001), 10 ** np.linspace(-9, -1, 3000)])
-    got = erfc_nonneg_f32(z, coef).astype(np.float64)
-    ref = erfc(z)
-    assert np.max(np.abs(got - ref)) < 5e-7
-    small = z <= 4.0
-    assert np.max(np.abs(got - ref)[small] / ref[small]) < 3e-6
-    assert erfc_nonneg_f32(np.array([0.0]), coef)[0] == np.float32(1.0) or \
-        abs(erfc_nonneg_f32(np.array([0.0]), coef)[0] - 1.0) < 4e-7
-    # a coefficient off in its fourth digit is visible
-    bad = list(coef)
-    bad[5] *= 1.001
-    worse = erfc_nonneg_f32(z, bad).astype(np.float64)
-    assert np.max(np.abs(worse - ref)) > 1e-6
+def test_erfc_table_of_the_bin_averaged_gaussian():
+    """The Gaussian profiles' erfc: cubic pieces over [-8, 8), looked up in
+    FP32 as the kernel does (constants read from the source).  Within 2e-7
+    of erfc everywhere (values up to 2, FP32 spacing 2.4e-7 there), 3e-8 where
+    erfc < 0.25; the bin of a point, erfc(far edge) - erfc(near edge), within
+    5e-7 of erf(near) - erf(far); swapped cubic coefficients fail."""
+    n, per = erfc_table_constants()
+    assert (n, per) == (512, 32.0)
+    tab = erfc_pieces(n, per)
+    e = np.concatenate([np.linspace(-12.0, 12.0, 600001), 10 ** np.linspace(-9, 1, 3000),
+                        -(10 ** np.linspace(-9, 1, 3000))])
+    got = erfc_lookup_f32(e, tab, n, per).astype(np.float64)
+    ref = erfc(e.astype(np.float32).astype(np.float64))
+    err = np.abs(got - ref)
+    assert np.max(err) < 2e-7
+    assert np.max(err[ref < 0.25]) < 3e-8
+    rng = np.random.default_rng(22)
+    A = 10 ** rng.uniform(-2, 1.5, 200000)     # points per width
+    eb = rng.uniform(-7, 7, 200000)            # far edge of a point
+    ea = eb + A                                # near edge: one bin up
+    v = (erfc_lookup_f32(eb, tab, n, per).astype(np.float64)
+         - erfc_lookup_f32(ea, tab, n, per).astype(np.float64))
+    exact = erf(ea.astype(np.float32).astype(np.float64)) - erf(eb.astype(np.float32).astype(np.float64))
+    assert np.max(np.abs(v - exact)) < 5e-7
+    bad = erfc_pieces(n, per, swap=True)
+    assert np.max(np.abs(erfc_lookup_f32(e, bad, n, per).astype(np.float64) - ref)) > 1e-6
 
 
 def quad_r2():

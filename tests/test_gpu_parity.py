@@ -159,30 +159,52 @@ def test_windows_round_half_boundaries(O):
     assert np.all(ib == (k + 1).astype(np.int32))  # half away from zero
 
 
-def test_doppler_full_config_vs_oracle(O):
+@pytest.fixture(scope="module")
+def c0_full(O):
     L = synth.workload_list("doppler_2e5")
     lines = L.lines()
     T = 1000.0
     Q = O.partition(L.E, L.g, T)
     ref, ne, _ = run_oracle(O, "doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0)
-    for prec, tol, btol in (("fp32c", 1e-4, 1e-6), ("fp64", 1e-10, 1e-10)):
-        out, st, _ = run_manager("doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0,
-                                 precision=prec)
-        assert st["n_eval"] == ne
-        check(out, ref, tol, btol, f"doppler {prec}")
+    return lines, T, Q, ref, ne
 
 
-def test_gaussian_vs_oracle(O, c1_small):
+# the bin-averaged Gaussian at every tile width: 64-point tiles (the shape the
+# bench's autotuner picks) run the window-edge segments with one-sided masks
+# and the covering lines unmasked; 512-point tiles are wider than the
+# 200-point windows, so every line is masked at both edges
+@pytest.mark.parametrize("ppl,split,order", [(1, 1, 1), (1, 4, 3), (2, 4, 3), (4, 1, 1),
+                                             (8, 16, 1)])
+def test_doppler_full_config_vs_oracle(c0_full, ppl, split, order):
+    lines, T, Q, ref, ne = c0_full
+    out, st, _ = run_manager("doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0,
+                             ppl=ppl, split=split, order=order)
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-4, 1e-6, f"doppler fp32c ppl={ppl} split={split} order={order}")
+
+
+def test_doppler_fp64_vs_oracle(c0_full):
+    lines, T, Q, ref, ne = c0_full
+    out, st, _ = run_manager("doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0,
+                             precision="fp64")
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-10, 1e-10, "doppler fp64")
+
+
+@pytest.mark.parametrize("ppl,split", [(4, 1), (1, 1), (2, 16)])
+def test_gaussian_vs_oracle(O, c1_small, ppl, split):
     L, lines = c1_small
     T, hwhm = 296.0, 0.05
     Q = O.partition(L.E, L.g, T)
     ref, ne, _ = run_oracle(O, "gaussian", lines, T, 1.0, Q, 0.0, 10000.0, 200_000, 0.5,
                             hwhm=hwhm)
-    for prec, tol, btol in (("fp32c", 1e-4, 1e-6), ("fp64", 1e-10, 1e-10)):
+    precs = (("fp32c", 1e-4, 1e-6), ("fp64", 1e-10, 1e-10)) if ppl == 4 else \
+        (("fp32c", 1e-4, 1e-6),)
+    for prec, tol, btol in precs:
         out, st, _ = run_manager("gaussian", lines, T, 1.0, Q, 0.0, 10000.0, 200_000, 0.5,
-                                 precision=prec, hwhm=hwhm)
+                                 precision=prec, hwhm=hwhm, ppl=ppl, split=split)
         assert st["n_eval"] == ne
-        check(out, ref, tol, btol, f"gaussian {prec}")
+        check(out, ref, tol, btol, f"gaussian {prec} ppl={ppl} split={split}")
 
 
 @pytest.fixture(scope="module")
