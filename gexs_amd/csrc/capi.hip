@@ -1036,8 +1036,9 @@ int gexs_pass_items(gexs_ctx* c, int64_t cap, int32_t* out, int64_t* n) {
     HIPCHK(c, hipMemcpy(h.data(), c->last_items, (size_t)m * sizeof(WorkItem), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < m; ++i) {
         const WorkItem& w = h[(size_t)i];
-        const int32_t v[10] = {w.lo, w.hi, w.w0, w.w1, w.s1, w.s2, w.s3, w.s4, w.job, w.group};
-        std::memcpy(out + 10 * i, v, sizeof v);
+        const int32_t v[12] = {w.lo, w.hi, w.w0, w.w1, w.s1, w.s2,    w.s3,
+                               w.s4, w.job, w.group, w.f2, w.f3};
+        std::memcpy(out + 12 * i, v, sizeof v);
     }
     return GEXS_OK;
 }

@@ -69,9 +69,10 @@ struct __attribute__((aligned(8))) VRecX {
 
 // One wave-tile of one job: grid points [w0, w1) and the lines [lo, hi)
 // whose windows meet them, split into segments
-//   [lo, s1) window ends inside the tile, [s1, s2) region I over the whole
-//   tile (no checks), [s2, s3) line centres near the tile (checked),
-//   [s3, s4) region I, [s4, hi) window starts inside the tile.
+//   [lo, s1) window ends inside the tile, [s1, f2) far wings over the whole
+//   tile (Kl / (s - 1), no checks), [f2, s2) region I over the whole tile (no
+//   checks), [s2, s3) line centres near the tile (checked), [s3, f3) region
+//   I, [f3, s4) far wings, [s4, hi) window starts inside the tile.
 // s1 = s4 = hi when no line covers the whole tile (every line checked).
 // With ngroups > 1 the tile's lines are cut into ngroups contiguous groups,
 // one work item each (group = 0 .. ngroups - 1); such an item writes its FP64
@@ -83,6 +84,7 @@ struct __attribute__((aligned(16))) WorkItem {
     int32_t job;
     int32_t s1, s2, s3, s4;
     int32_t group, ngroups, slot;
+    int32_t f2, f3;
 };
 
 // per-(T,P) physics of one job, FP64

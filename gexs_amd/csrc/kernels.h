@@ -13,7 +13,8 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
                     int64_t max_n, const TPParams* tp, LinePos* pos);
 // FP32 records of every job's lines, with each amplitude as mantissa and
 // exponent; emax[k] (zero before) receives job k's largest biased exponent,
-// neval[k] its N_eval and kmax[k] its largest k15 (each zero before).
+// neval[k] its N_eval, kmax[2k] its largest k15 and kmax[2k + 1] its largest
+// kfar (each zero before).
 // max_blocks > 0 caps the blocks per job (a pass that runs beside another
 // kernel gets its few blocks dispatched at once); 0: up to 2048.
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,

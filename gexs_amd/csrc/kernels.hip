@@ -136,12 +136,16 @@ __global__ __launch_bounds__(256) void k_line_pos(const double* __restrict__ nu,
 // ------------------------------------------------------------- k_line_rec
 // FP32 record of one line at one (T,P) (FP64 arithmetic, no contraction):
 // returns the amplitude exponent ex (0 when the amplitude is 0) and sets
-// *neval to the line's window length when S != 0 and the window is not empty,
-// and *k15 (Voigt) to the distance beyond which its points are in region I.
+// *neval to the line's window length when S != 0 and the window is not empty;
+// for the Voigt profiles x.k15 is the distance (in grid points) beyond which
+// the line's points are in region I and *kfar the one beyond which the
+// one-term wing form Kl / (s - 1) applies: region I is
+// Kl (s + 1) / (s^2 + H) = Kl / (s - 1 + (H + 1) / (s + 1)), and the last
+// term changes the value by at most 2^-23 relative once s^2 >= (H + 1) 2^23.
 __device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double nu, double E,
                                            double A, int g, const double* gamma,
                                            const double* nexp, int64_t i, const LinePos& lp,
-                                           VRec& r, VRecX& x, long long* neval) {
+                                           VRec& r, VRecX& x, long long* neval, int* kfar) {
 #pragma clang fp contract(off)
     const double v = nu == 0.0 ? 1e-6 : nu;
     const double S = line_strength(p, nu, E, A, g);
@@ -184,12 +188,17 @@ __device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double
             kk = (15.0 * (1.0 + 1e-5) - y + fabs(Bf)) / Af;
         }
         x.k15 = kk <= 0.0 ? 0 : (kk >= 1.0e8 ? 100000000 : (int)ceil(kk));
+        const double sf = sqrt((2.0 * y * y + 1.0) * 8388608.0);  // s >= sf: one-term form
+        const double xf2 = sf - (y * y - 0.5);
+        const double kf = xf2 > 0.0 ? (sqrt(xf2) * (1.0 + 1e-5) + fabs(Bf)) / Af : 0.0;
+        *kfar = max(x.k15, kf <= 0.0 ? 0 : (kf >= 1.0e8 ? 100000000 : (int)ceil(kf)));
     } else {
         r.Kl = 0.0f;
         r.E2 = 0.0f;
         r.H = 0.0f;
         x.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
         x.k15 = 0;
+        *kfar = 0;
     }
     r.ex = ex;
     return amp > 0.0 ? ex : 0;
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
     const JobDesc d = jd[k];
     const bool voigt = p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD;
     long long ne = 0;
-    int km = 0;
+    int km = 0, kf = 0;
     int em = 0;  // largest biased exponent of a line reaching the grid (0: none)
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < d.n; ii += gridDim.x * blockDim.x) {
         const int64_t i = d.l0 + ii;
@@ -221,22 +230,29 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
         VRec r;
         VRecX x;
         long long nl;
-        line_record(p, voigt, nu[i], E[i], A[i], g[i], gamma, nexp, i, lp, r, x, &nl);
+        int kfl;
+        line_record(p, voigt, nu[i], E[i], A[i], g[i], gamma, nexp, i, lp, r, x, &nl, &kfl);
         ne += nl;
         if (r.Kg > 0.0f) em = max(em, r.ex + kExBias);
-        if (voigt && lp.ie > lp.ib) km = max(km, x.k15);
+        if (voigt && lp.ie > lp.ib) {
+            km = max(km, x.k15);
+            kf = max(kf, kfl);
+        }
         rec[d.rec_off + ii] = r;
         recx[d.recx_off + ii] = x;
     }
     __shared__ long long sh_ne[4];
     __shared__ int sh_km[4];
+    __shared__ int sh_kf[4];
     __shared__ int sh_em[4];
     ne = block_sum_ll(ne, sh_ne);
     km = block_max_i(km, sh_km);
+    kf = block_max_i(kf, sh_kf);
     em = block_max_i(em, sh_em);
     if (threadIdx.x == 0) {
         if (neval != nullptr && ne != 0) atomicAdd(&neval[k], (unsigned long long)ne);
-        if (kmax != nullptr && km > 0) atomicMax(&kmax[k], km);
+        if (kmax != nullptr && km > 0) atomicMax(&kmax[2 * k], km);
+        if (kmax != nullptr && kf > 0) atomicMax(&kmax[2 * k + 1], kf);
         if (em > 0) atomicMax(&emax[k], (unsigned long long)em);
     }
 }
@@ -300,6 +316,8 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.s1 = lf < rf ? lf : hi;
     it.s4 = lf < rf ? rf : hi;
     it.s2 = it.s3 = it.s4;  // [s1, s4) all region I until k_segments runs
+    it.f2 = it.s1;
+    it.f3 = it.s4;
     const int G = max(d.groups, 1);
     it.ngroups = G;
     for (int g = 0; g < G; ++g) {
@@ -315,11 +333,21 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     }
 }
 
-// Near-centre segment of every work item: among the lines [s1, s4) that cover
-// the whole tile (set by k_tiles), those whose centre c lies within kmax of
-// the tile, [s2, s3), may have points outside Humlicek region I.  kmax = max
-// k15 over the job's lines (T,P-dependent, from k_line_rec).
-// out may equal in (in place) or be a per-pass copy.
+// Near-centre and far segments of every work item: among the lines
+// [s1, s4) that cover the whole tile (set by k_tiles), those whose centre c
+// lies within K = kmax[2 job] of the tile, [s2, s3), may have points outside
+// region I, and those whose centre lies KF = kmax[2 job + 1] or more from it,
+// [s1, f2) and [f3, s4), take the one-term wing form at every point (K and KF
+// are the largest k15 and kfar of the job's lines, T,P-dependent, from
+// k_line_rec; KF >= K).  out may equal in (in place) or be a per-pass copy.
+__device__ __forceinline__ int first_c_at_least(const LinePos* jp, int a, int b, long long v) {
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if ((long long)jp[m].c >= v) b = m; else a = m + 1;
+    }
+    return a;
+}
+
 __global__ __launch_bounds__(256) void k_segments(const WorkItem* in, WorkItem* out, int nitems,
                                                   const JobDesc* __restrict__ jd,
                                                   const LinePos* __restrict__ pos,
@@ -327,26 +355,19 @@ __global__ __launch_bounds__(256) void k_segments(const WorkItem* in, WorkItem* 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nitems) return;
     WorkItem it = in[t];
-    int s2 = it.s4, s3 = it.s4;
+    int f2 = it.s1, s2 = it.s4, s3 = it.s4, f3 = it.s4;
     if (it.s1 < it.s4) {
         const LinePos* jp = pos + jd[it.job].pos_off;
-        const int K = kmax[it.job];
-        const long long cl = (long long)it.w0 - K, ch = (long long)it.w1 + K;
-        int a = it.s1, b = it.s4;
-        while (a < b) {  // first c >= w0 - K
-            const int m = (a + b) >> 1;
-            if ((long long)jp[m].c >= cl) b = m; else a = m + 1;
-        }
-        s2 = a;
-        b = it.s4;
-        while (a < b) {  // first c >= w1 + K
-            const int m = (a + b) >> 1;
-            if ((long long)jp[m].c >= ch) b = m; else a = m + 1;
-        }
-        s3 = a;
+        const int K = kmax[2 * it.job], KF = kmax[2 * it.job + 1];
+        f2 = first_c_at_least(jp, it.s1, it.s4, (long long)it.w0 - KF);
+        s2 = first_c_at_least(jp, f2, it.s4, (long long)it.w0 - K);
+        s3 = first_c_at_least(jp, s2, it.s4, (long long)it.w1 + K);
+        f3 = first_c_at_least(jp, s3, it.s4, (long long)it.w1 + KF);
     }
+    it.f2 = f2;
     it.s2 = s2;
     it.s3 = s3;
+    it.f3 = f3;
     out[t] = it;
 }
 

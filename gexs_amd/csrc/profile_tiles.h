@@ -80,6 +80,13 @@ __device__ __forceinline__ float region1(float x, float Kl, float E2, float H) {
     return num * __builtin_amdgcn_rcpf(den);
 }
 
+// The one-term wing value Kl / (s - 1), s - 1 = x^2 + (E2 - 1), of a line
+// whose whole tile lies beyond its kfar: region I without the term that is
+// below 2^-23 there (4 VALU against 6).
+__device__ __forceinline__ float far1(float x, float Kl, float E2m1) {
+    return Kl * __builtin_amdgcn_rcpf(fmaf(x, x, E2m1));
+}
+
 constexpr int kChunk = 32;  // lines per uncompensated FP32 partial sum (at most kChunk + 3)
 
 template <int PPL>
@@ -144,7 +151,7 @@ __device__ __forceinline__ int slice_in_window(int j0, const LinePos& lp) {
 template <int PPL>
 constexpr int stage_row() { return 4 + (PPL + 3) / 4 * 4; }
 
-template <int PPL, int MASK>
+template <int PPL, int MASK, bool FAR>
 __device__ __forceinline__ void staged_masked(TileAcc<PPL>& t, const float* row,
                                               const LinePos& lp, float nl, int w0) {
     const float A = row[0], Kl = row[1], E2 = row[2], H = row[3];
@@ -152,13 +159,16 @@ __device__ __forceinline__ void staged_masked(TileAcc<PPL>& t, const float* row,
     for (int q = 0; q < PPL; ++q) {
         const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
         if (st == 0) continue;
-        const float v = region1(fmaf(nl, A, row[4 + q]), Kl, E2, H);
+        const float x = fmaf(nl, A, row[4 + q]);
+        const float v = FAR ? far1(x, Kl, E2) : region1(x, Kl, E2, H);
         t.part[q] += (st == 2 || in_window<MASK>(t.jj[q], lp)) ? v : 0.0f;
     }
     t.tick(1);
 }
 
-template <int PPL, int MASK>
+// FAR: the lines' whole tile lies beyond their kfar; the row holds E2 - 1 in
+// place of E2 and the one-term form is evaluated.
+template <int PPL, int MASK, bool FAR = false>
 __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, ConstPos pos, int a,
                                              int b, int w0, int lane, float* xs) {
     constexpr int R = stage_row<PPL>();
@@ -174,7 +184,7 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
         float* row = xs + lane * R;
         row[0] = vc.A;
         row[1] = ldexpf(vc.Kl, vc.ex - t.E);
-        row[2] = vc.E2;
+        row[2] = FAR ? vc.E2 - 1.0f : vc.E2;
         row[3] = vc.H;
 #pragma unroll
         for (int q = 0; q < PPL; ++q) row[4 + q] = fmaf(vc.cf - (float)(w0 + GX_WAVE * q), vc.A, vc.B);
@@ -187,22 +197,22 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
                 for (; k + 8 <= nb; k += 4) {
                     const LinePos q0 = pb[k + 4], q1 = pb[k + 5], q2 = pb[k + 6], q3 = pb[k + 7];
                     __builtin_amdgcn_sched_barrier(0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 0) * R, p0, nl, w0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 1) * R, p1, nl, w0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 2) * R, p2, nl, w0);
-                    staged_masked<PPL, MASK>(t, xs + (k + 3) * R, p3, nl, w0);
+                    staged_masked<PPL, MASK, FAR>(t, xs + (k + 0) * R, p0, nl, w0);
+                    staged_masked<PPL, MASK, FAR>(t, xs + (k + 1) * R, p1, nl, w0);
+                    staged_masked<PPL, MASK, FAR>(t, xs + (k + 2) * R, p2, nl, w0);
+                    staged_masked<PPL, MASK, FAR>(t, xs + (k + 3) * R, p3, nl, w0);
                     p0 = q0;
                     p1 = q1;
                     p2 = q2;
                     p3 = q3;
                 }
-                staged_masked<PPL, MASK>(t, xs + (k + 0) * R, p0, nl, w0);
-                staged_masked<PPL, MASK>(t, xs + (k + 1) * R, p1, nl, w0);
-                staged_masked<PPL, MASK>(t, xs + (k + 2) * R, p2, nl, w0);
-                staged_masked<PPL, MASK>(t, xs + (k + 3) * R, p3, nl, w0);
+                staged_masked<PPL, MASK, FAR>(t, xs + (k + 0) * R, p0, nl, w0);
+                staged_masked<PPL, MASK, FAR>(t, xs + (k + 1) * R, p1, nl, w0);
+                staged_masked<PPL, MASK, FAR>(t, xs + (k + 2) * R, p2, nl, w0);
+                staged_masked<PPL, MASK, FAR>(t, xs + (k + 3) * R, p3, nl, w0);
                 k += 4;
             }
-            for (; k < nb; ++k) staged_masked<PPL, MASK>(t, xs + k * R, pb[k], nl, w0);
+            for (; k < nb; ++k) staged_masked<PPL, MASK, FAR>(t, xs + k * R, pb[k], nl, w0);
             continue;
         }
 #pragma unroll 4
@@ -210,7 +220,10 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
             const float* rk = xs + k * R;
             const float A = rk[0], Kl = rk[1], E2 = rk[2], H = rk[3];
 #pragma unroll
-            for (int q = 0; q < PPL; ++q) t.part[q] += region1(fmaf(nl, A, rk[4 + q]), Kl, E2, H);
+            for (int q = 0; q < PPL; ++q) {
+                const float x = fmaf(nl, A, rk[4 + q]);
+                t.part[q] += FAR ? far1(x, Kl, E2) : region1(x, Kl, E2, H);
+            }
         }
         t.fold();
     }
@@ -656,17 +669,24 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
         if (it.s1 < it.s4) {
             const int s1 = min(max(it.s1, ca), cb), s2 = min(max(it.s2, ca), cb);
             const int s3 = min(max(it.s3, ca), cb), s4 = min(max(it.s4, ca), cb);
-            // the region-I segments through the staged loop (6 VALU per
-            // line against 14 with broadcasts), the rest batched
+            // the region-I and far segments through the staged loop (6 or 4
+            // VALU per line against 14 with broadcasts), the rest batched
+            const int f2 = min(max(it.f2, ca), cb), f3 = min(max(it.f3, ca), cb);
             const ConstPos pos = (ConstPos)jb.pos;
-            if (it.s2 > it.s1)
+            if (it.f2 > it.s1)
+                voigt_staged<1, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
+            else if (it.s2 > it.s1)
                 voigt_staged<1, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskHi>(t, gpos, grec, grecx, ca, s1, w0, w1, lane, incl);
-            voigt_staged<1, kMaskNone>(t, grec, pos, s1, s2, w0, lane, xs);
+            voigt_staged<1, kMaskNone, true>(t, grec, pos, s1, f2, w0, lane, xs);
+            voigt_staged<1, kMaskNone>(t, grec, pos, f2, s2, w0, lane, xs);
             lines_batched<PROF, kMaskNone>(t, gpos, grec, grecx, s2, s3, w0, w1, lane, incl);
-            voigt_staged<1, kMaskNone>(t, grec, pos, s3, s4, w0, lane, xs);
-            if (it.s3 < it.s4)
+            voigt_staged<1, kMaskNone>(t, grec, pos, s3, f3, w0, lane, xs);
+            voigt_staged<1, kMaskNone, true>(t, grec, pos, f3, s4, w0, lane, xs);
+            if (it.f3 < it.s4)
+                voigt_staged<1, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
+            else if (it.s3 < it.s4)
                 voigt_staged<1, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskLo>(t, gpos, grec, grecx, s4, cb, w0, w1, lane, incl);
@@ -680,18 +700,26 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             // [lo, s1): ib <= w0 < ie < w1; [s1, s4): ib <= w0, ie >= w1;
             // [s4, hi): w0 < ib < w1 <= ie.
             // ν order makes c non-decreasing: when the first whole-tile line
-            // is far below the tile (s2 > s1) so are all of [lo, s1); when
-            // the last one is far above (s3 < s4) so are all of [s4, hi)
+            // is far below the tile (s2 > s1, or beyond kfar: f2 > s1) so
+            // are all of [lo, s1); when the last one is far above (s3 < s4,
+            // f3 < s4) so are all of [s4, hi)
+            const int f2 = min(max(it.f2, ca), cb), f3 = min(max(it.f3, ca), cb);
             const GlobRec grec = (GlobRec)jb.rec;
-            if (it.s2 > it.s1)
+            if (it.f2 > it.s1)
+                voigt_staged<PPL, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
+            else if (it.s2 > it.s1)
                 voigt_staged<PPL, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
-            voigt_staged<PPL, kMaskNone>(t, grec, pos, s1, s2, w0, lane, xs);
+            voigt_staged<PPL, kMaskNone, true>(t, grec, pos, s1, f2, w0, lane, xs);
+            voigt_staged<PPL, kMaskNone>(t, grec, pos, f2, s2, w0, lane, xs);
             near_staged<PROF, PPL>(t, (GlobPos)jb.pos, grec, (GlobRecX)jb.recx, s2, s3, w0, w1, lane,
                                    xs);
-            voigt_staged<PPL, kMaskNone>(t, grec, pos, s3, s4, w0, lane, xs);
-            if (it.s3 < it.s4)
+            voigt_staged<PPL, kMaskNone>(t, grec, pos, s3, f3, w0, lane, xs);
+            voigt_staged<PPL, kMaskNone, true>(t, grec, pos, f3, s4, w0, lane, xs);
+            if (it.f3 < it.s4)
+                voigt_staged<PPL, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
+            else if (it.s3 < it.s4)
                 voigt_staged<PPL, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);

@@ -125,6 +125,29 @@ def test_region1_s_form_equals_humlicek_region1(O):
         assert np.max(np.abs(s_form(x, y, e2, hh) - h) / h) > 1e-6
 
 
+def test_far_wing_form_within_2_pow_23_of_region1(O):
+    """Where s^2 >= (2y^2 + 1) 2^23 (s = x^2 + y^2 - 1/2) the profile kernel
+    evaluates region I as 0.5641896 y / (s - 1): region I is
+    0.5641896 y / (s - 1 + (2y^2 + 1) / (s + 1)) exactly, so dropping the last
+    term changes it by at most 2^-23 relative.  Checked against the oracle's
+    W4 beyond that distance; at a 2^20 threshold the error is 8x larger."""
+    rng = np.random.default_rng(17)
+    y = 10 ** rng.uniform(-6, 3, 4000)
+
+    def x_at(thr):
+        sf = np.sqrt((2 * y * y + 1) * thr)
+        return np.sqrt(np.maximum(sf - (y * y - 0.5), 0.0))
+
+    def far(x, y):
+        return 0.5641896 * y / (x * x + y * y - 1.5)
+
+    for thr, lo, hi in ((2.0 ** 23, 0.0, 1.2e-7), (2.0 ** 20, 5e-7, 1.0)):
+        x = x_at(thr) * (1 + rng.uniform(0, 1e-3, 4000))
+        h = np.array([O.humlic(a, b) for a, b in zip(x, y)])
+        err = np.max(np.abs(far(x, y) - h) / h)
+        assert lo <= err <= hi, (thr, err)
+
+
 def test_region2_gauss_hermite_pairs_equal_humlicek_region2(O):
     """The profile kernel evaluates Humlicek region II (5.5 <= |x| + y < 15) as
     the 4-point Gauss-Hermite sum in +-a pairs,
