@@ -131,13 +131,15 @@ def cpu_baseline(lst, spec, T, P, Q, target_s):
 
 
 def traffic_from_profile(workload):
-    """HBM bytes per launch of the profile kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
+    """The committed rocprofv3 figures of this workload's profile kernel (HBM
+    bytes per launch from FETCH_SIZE / WRITE_SIZE, VALU instructions per
+    launch from SQ_INSTS_VALU, and the launch shape they were taken at), or
+    None if absent."""
     f = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
     if not os.path.exists(f):
         return None
     with open(f) as fh:
-        return json.load(fh).get("hbm_bytes_per_launch")
+        return json.load(fh)
 
 
 _LISTS = {}
@@ -559,15 +561,30 @@ def execute_bench(a, workload, profile=None, steps=None, warm=None, fp64_steps=N
             "tuning": tuning, "used": used, "tiles": st["n_tiles"], "fp64": fp64}
 
 
-def roofline_of(r, traffic=None, probe=None):
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4  # wave64 FP32 VALU instructions/s: 1024 SIMDs, one per 4 cycles
+
+
+def roofline_of(r, prof=None, probe=None):
     """The profile kernel's FP32 VALU rate against the MI355X FP32 vector peak
     (frac_unpacked: against half of it, the rate without packed FP32
-    instructions, which this build does not emit)."""
-    return {"bound": "valu", "compute": "fp32 VALU", "achieved": r["achieved"] / 1e12,
-            "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": r["achieved"] / PEAK_FP32,
-            "frac_unpacked": r["achieved"] / (PEAK_FP32 / 2), "traffic": traffic,
-            "kernel": f"k_profile_tiles<{r['profile'].upper()}>", "valu_probe": probe,
-            "kernel_ms_avg": r["k_avg"], "flop_per_eval": r["flops"]}
+    instructions, which this build does not emit).  With the committed
+    counters of the same launch shape (prof), the HBM bytes per launch
+    (traffic) and the fraction of the VALU issue rate the kernel reaches at
+    this run's kernel time."""
+    out = {"bound": "valu", "compute": "fp32 VALU", "achieved": r["achieved"] / 1e12,
+           "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": r["achieved"] / PEAK_FP32,
+           "frac_unpacked": r["achieved"] / (PEAK_FP32 / 2),
+           "traffic": prof.get("hbm_bytes_per_launch") if prof else None,
+           "kernel": f"k_profile_tiles<{r['profile'].upper()}>", "valu_probe": probe,
+           "kernel_ms_avg": r["k_avg"], "flop_per_eval": r["flops"]}
+    if prof and prof.get("valu_insts_per_launch"):
+        same = all(prof["launch"].get(k) == r["used"].get(k)
+                   for k in ("points_per_lane", "waves_per_tile"))
+        out["counters_launch"] = prof["launch"]
+        out["valu_insts_per_eval"] = prof["valu_insts_per_launch"] / (r["n_eval"] / 64)
+        if same:
+            out["valu_issue_frac"] = prof["valu_insts_per_launch"] / (r["k_avg"] * 1e-3) / VALU_ISSUE_PEAK
+    return out
 
 
 def side_workload(a, workload, profile=None):

@@ -43,8 +43,9 @@ def c1_full(O):
 # (path, points per lane, waves per tile, tile order): the batch path at its
 # default shape, and the execute path (ExecuteCrossSection over the resident
 # list) at the shape the autotuner picks on this workload and the bench times
-@pytest.mark.parametrize("path,ppl,split,order", [("batch", 4, 1, 1), ("execute", 2, 4, 3),
-                                                  ("execute", 2, 1, 1), ("execute", 8, 16, 1)])
+@pytest.mark.parametrize("path,ppl,split,order", [("batch", 4, 1, 1), ("execute", 4, 4, 3),
+                                                  ("execute", 2, 4, 3), ("execute", 2, 1, 1),
+                                                  ("execute", 8, 16, 1)])
 def test_voigt_1e6_full_grid(c1_full, path, ppl, split, order):
     spec, L, Q, ref, ne = c1_full
     start, end, N = spec["grid"]
@@ -72,7 +73,7 @@ def test_voigt_1e6_full_grid(c1_full, path, ppl, split, order):
     assert err <= 1e-4 and band <= 1e-6
 
 
-@pytest.mark.parametrize("ppl,split,order", [(2, 4, 3), (1, 4, 3)])
+@pytest.mark.parametrize("ppl,split,order", [(4, 4, 1), (2, 4, 3), (1, 4, 3)])
 def test_voigt_quad_1e6_full_grid(O, ppl, split, order):
     """The 10^6-line workload with the Voigt-Quad profile (the bench's second
     profile line) over the whole grid, at the launch shapes its autotuner picks."""
