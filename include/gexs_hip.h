@@ -293,7 +293,7 @@ int gexs_autotune(gexs_ctx* ctx, const gexs_tp* tp, int ntp, const gexs_grid* gr
  * in device memory, writing a device grid the caller owns, with every
  * intermediate (pass tables, counters, per-line windows and FP32 records,
  * per-tile work items) in one file-scope device array of the library,
- * about 62 MiB, which each GPU the library runs on holds until the process
+ * about 64 MiB, which each GPU the library runs on holds until the process
  * ends.  The call makes no hipMalloc, hipFree or other allocation.
  *   - lines: the pass's lines (the caller's selection), ascending in nu,
  *     at most GEXS_FIXED_MAX_LINES; gamma / nexp may be NULL for Doppler and
