@@ -643,18 +643,17 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, gxk::line_rec(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale, B.rec.as<VRec>(),
                             B.recx.as<VRecX>(), t.kmax, t.neval, c->cus));
-    // Voigt and Voigt-Quad items carry this pass's near-centre bounds (a
-    // copy per buffer set); the bin-Gauss profiles read the shared items
-    const bool voigt = c->profile == GEXS_VOIGT || c->profile == GEXS_VOIGT_QUAD;
-    if (voigt)
-        HIPCHK(c, gxk::segments(P, c->items.as<WorkItem>(), B.items.as<WorkItem>(), c->n_items, t.jd,
-                                c->pos.as<LinePos>(), t.kmax, c->profile));
+    // the items carry this pass's segment bounds (a copy per buffer set):
+    // near-centre and far lines (Voigt profiles) or the lines within reach
+    // of each tile (bin-Gauss), from this pass's kmax
+    HIPCHK(c, gxk::segments(P, c->items.as<WorkItem>(), B.items.as<WorkItem>(), c->n_items, t.jd,
+                            c->pos.as<LinePos>(), t.kmax, c->profile));
     HIPCHK(c, hipEventRecord(B.pre_done, P));
     hipStream_t s = c->stream;
     HIPCHK(c, hipStreamWaitEvent(s, B.pre_done, 0));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split,
-                                 voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>(), c->n_items, t.jobs,
+                                 B.items.as<WorkItem>(), c->n_items, t.jobs,
                                  t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
                                  launch_blocks(c), true));
     if (timed) {
@@ -667,7 +666,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     B.used = true;
     c->last.n_tiles = c->n_items;
     c->last_neval = t.neval;
-    c->last_items = voigt ? B.items.as<WorkItem>() : c->items.as<WorkItem>();
+    c->last_items = B.items.as<WorkItem>();
     return GEXS_OK;
 }
 

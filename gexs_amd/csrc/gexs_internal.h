@@ -61,7 +61,8 @@ struct __attribute__((aligned(32))) VRec {
 
 // per line, per (T,P): what the checked loop needs beyond VRec.
 // Voigt: y and k15 = smallest |c - j| with |x| + y >= 15 for certain.
-// bin-Gauss: y = H, half the bin width in units of the width.
+// bin-Gauss: y = H, half the bin width in units of the width, and k15 the
+// line's reach: it adds exactly zero at points j with |c - j| >= k15.
 struct __attribute__((aligned(8))) VRecX {
     float y;
     int32_t k15;
@@ -74,6 +75,9 @@ struct __attribute__((aligned(8))) VRecX {
 //   checks), [s2, s3) line centres near the tile (checked), [s3, f3) region
 //   I, [f3, s4) far wings, [s4, hi) window starts inside the tile.
 // s1 = s4 = hi when no line covers the whole tile (every line checked).
+// bin-Gauss: [f2, f3) are the lines of [lo, hi) centred within reach of the
+// tile (the job's largest reach); s2 = s3 = s4 and the kernel sums
+// [f2, f3) only, masked by s1 and s4.
 // With ngroups > 1 the tile's lines are cut into ngroups contiguous groups,
 // one work item each (group = 0 .. ngroups - 1); such an item writes its FP64
 // partial sums to row `slot` (tile * ngroups + group) of its job's Job::part

@@ -13,8 +13,9 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
                     int64_t max_n, const TPParams* tp, LinePos* pos);
 // FP32 records of every job's lines, with each amplitude as mantissa and
 // exponent; emax[k] (zero before) receives job k's largest biased exponent,
-// neval[k] its N_eval, kmax[2k] its largest k15 and kmax[2k + 1] its largest
-// kfar (each zero before).
+// neval[k] its N_eval, kmax[2k] its largest k15 (Voigt profiles) and
+// kmax[2k + 1] its largest kfar (Voigt) or reach (bin-Gauss), each zero
+// before.
 // max_blocks > 0 caps the blocks per job (a pass that runs beside another
 // kernel gets its few blocks dispatched at once); 0: up to 2048.
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
@@ -22,8 +23,9 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
                     unsigned long long* neval, int max_blocks = 0);
-// near-centre bounds of every work item (after line_rec has produced kmax per
-// job): out[i] = in[i] with s2, s3 set (out may equal in); Voigt only
+// segment bounds of every work item (after line_rec has produced kmax per
+// job): out[i] = in[i] with f2, s2, s3, f3 set (out may equal in); the
+// bin-Gauss profiles set f2, f3 only (the lines within reach of the tile)
 hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,
                     const JobDesc* jd, const LinePos* pos, const int* kmax, int profile);
 // jobs job0 .. job0 + njobs - 1 of jd / tp, all with tiles of span points

@@ -142,6 +142,11 @@ __global__ __launch_bounds__(256) void k_line_pos(const double* __restrict__ nu,
 // one-term wing form Kl / (s - 1) applies: region I is
 // Kl (s + 1) / (s^2 + H) = Kl / (s - 1 + (H + 1) / (s + 1)), and the last
 // term changes the value by at most 2^-23 relative once s^2 >= (H + 1) 2^23.
+// For the bin-averaged Gaussian x.k15 = *kfar is the line's reach: at every
+// point j with |c - j| >= reach both edges of the bin, e = (c - j) A + B - H
+// and e + A, lie beyond the erfc table's ends (|e| >= 8) on one side, so the
+// table gives them the same value and the line adds exactly zero there
+// (0 when the line adds nothing anywhere).
 __device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double nu, double E,
                                            double A, int g, const double* gamma,
                                            const double* nexp, int64_t i, const LinePos& lp,
@@ -197,8 +202,13 @@ __device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double
         r.E2 = 0.0f;
         r.H = 0.0f;
         x.y = (float)(0.5 * p.dfreq * iw);  // H, half bin width in units of the width
-        x.k15 = 0;
-        *kfar = 0;
+        // |c - j| >= K puts e = (c - j) A + B - H (and e + A) at or beyond
+        // +8 (c - j >= K) or e + A at or below -8 (j - c >= K) when
+        // K A >= 8 + |B| + A / 2; the relative margins cover the FP32
+        // rounding of A, B - H and the kernel's fma (|e| <= 8 + A there)
+        const double kk = ((8.0 * (1.0 + 1e-5) + fabs(Bf)) / Af) * (1.0 + 1e-5) + 0.5;
+        x.k15 = amp > 0.0 ? (!(kk < 1.0e8) ? 100000000 : (int)ceil(kk)) : 0;
+        *kfar = x.k15;
     }
     r.ex = ex;
     return amp > 0.0 ? ex : 0;
@@ -234,8 +244,8 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
         line_record(p, voigt, nu[i], E[i], A[i], g[i], gamma, nexp, i, lp, r, x, &nl, &kfl);
         ne += nl;
         if (r.Kg > 0.0f) em = max(em, r.ex + kExBias);
-        if (voigt && lp.ie > lp.ib) {
-            km = max(km, x.k15);
+        if (lp.ie > lp.ib) {
+            if (voigt) km = max(km, x.k15);
             kf = max(kf, kfl);
         }
         rec[d.rec_off + ii] = r;
@@ -316,8 +326,9 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.s1 = lf < rf ? lf : hi;
     it.s4 = lf < rf ? rf : hi;
     it.s2 = it.s3 = it.s4;  // [s1, s4) all region I until k_segments runs
-    it.f2 = it.s1;
-    it.f3 = it.s4;
+    const bool gauss = tp[job].profile == GX_PROF_DOPPLER || tp[job].profile == GX_PROF_GAUSSIAN;
+    it.f2 = gauss ? lo : it.s1;  // bin-Gauss: every line within reach until then
+    it.f3 = gauss ? hi : it.s4;
     const int G = max(d.groups, 1);
     it.ngroups = G;
     for (int g = 0; g < G; ++g) {
@@ -339,7 +350,10 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
 // region I, and those whose centre lies KF = kmax[2 job + 1] or more from it,
 // [s1, f2) and [f3, s4), take the one-term wing form at every point (K and KF
 // are the largest k15 and kfar of the job's lines, T,P-dependent, from
-// k_line_rec; KF >= K).  out may equal in (in place) or be a per-pass copy.
+// k_line_rec; KF >= K).  Bin-Gauss: [f2, f3) are the lines of [lo, hi) whose
+// centre lies within KF, the largest reach, of the tile; the others add
+// exactly zero to it.  Only lo, hi, s1 and s4 are read, so out may equal in
+// (in place) or be a per-pass copy.
 __device__ __forceinline__ int first_c_at_least(const LinePos* jp, int a, int b, long long v) {
     while (a < b) {
         const int m = (a + b) >> 1;
@@ -351,10 +365,19 @@ __device__ __forceinline__ int first_c_at_least(const LinePos* jp, int a, int b,
 __global__ __launch_bounds__(256) void k_segments(const WorkItem* in, WorkItem* out, int nitems,
                                                   const JobDesc* __restrict__ jd,
                                                   const LinePos* __restrict__ pos,
-                                                  const int* __restrict__ kmax) {
+                                                  const int* __restrict__ kmax, bool gauss) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nitems) return;
     WorkItem it = in[t];
+    if (gauss) {
+        const LinePos* jp = pos + jd[it.job].pos_off;
+        const int KF = kmax[2 * it.job + 1];
+        const int hi = max(it.hi, it.lo);
+        it.f2 = first_c_at_least(jp, it.lo, hi, (long long)it.w0 - KF);
+        it.f3 = first_c_at_least(jp, it.f2, hi, (long long)it.w1 + KF);
+        out[t] = it;
+        return;
+    }
     int f2 = it.s1, s2 = it.s4, s3 = it.s4, f3 = it.s4;
     if (it.s1 < it.s4) {
         const LinePos* jp = pos + jd[it.job].pos_off;
@@ -585,10 +608,10 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
 
 hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,
                     const JobDesc* jd, const LinePos* pos, const int* kmax, int profile) {
-    // only the Voigt profiles read them
-    if (nitems <= 0 || (profile != GX_PROF_VOIGT && profile != GX_PROF_VOIGT_QUAD)) return hipSuccess;
+    if (nitems <= 0) return hipSuccess;
+    const bool gauss = profile != GX_PROF_VOIGT && profile != GX_PROF_VOIGT_QUAD;
     hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 256)), dim3(256), 0, s, in, out, nitems,
-                       jd, pos, kmax);
+                       jd, pos, kmax, gauss);
     return hipGetLastError();
 }
 

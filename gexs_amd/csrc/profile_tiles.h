@@ -268,11 +268,17 @@ __device__ __forceinline__ void checked_line(TileAcc<PPL>& t, const LinePos& lp,
     } else {
 #pragma unroll
         for (int q = 0; q < PPL; ++q) {
-            const int st = slice_in_window<MASK>(w0 + GX_WAVE * q, lp);
+            const int j0 = w0 + GX_WAVE * q;
+            const int st = slice_in_window<MASK>(j0, lp);
             if (st == 0) continue;
             const float x = fabsf(fmaf(r.cf - t.jf[q], r.A, r.B));
             float v = region1(x, r.Kl, r.E2, r.H);
-            if (inner_point<PROF>(x, rx.y)) v = DEFER ? 0.0f : r.Kg * inner_value<PROF>(x, rx.y);
+            // the slice's nearest point to the centre is k15 or more away:
+            // region I at every lane (a scalar decision per slice)
+            const int kq = lp.c < j0 ? (j0 - lp.c) : (lp.c >= j0 + GX_WAVE ? (lp.c - j0 - GX_WAVE + 1) : 0);
+            if (kq < rx.k15) {
+                if (inner_point<PROF>(x, rx.y)) v = DEFER ? 0.0f : r.Kg * inner_value<PROF>(x, rx.y);
+            }
             if (st == 2)
                 t.part[q] += v;
             else
@@ -413,17 +419,19 @@ __device__ __forceinline__ float wave_shr1(float v, float lane0) {
                                                       0x138 /* wave_shr:1 */, 0xF, 0xF, false));
 }
 
-// erfc(e) from the workgroup's LDS copy of the table: piece k covers
-// e in [k / 32 - 8, (k + 1) / 32 - 8) as a0 + f (a1 + f (a2 + f a3)), f the
-// position within the piece; e outside [-8, 8) takes the end pieces
-// (erfc(8) = 1.1e-29).  32 e is exact (a power of two), and so are its floor
-// and f: the lookup adds no rounding of e.
-__device__ __forceinline__ float erfc_lds(float e) {
+// erfc(e) from the workgroup's LDS copy of the table, given v = 32 e (the
+// callers stage A and B - h times 32, a power of two, so v is exactly 32
+// times the edge they would compute unscaled): piece k covers e in
+// [k / 32 - 8, (k + 1) / 32 - 8) as a0 + f (a1 + f (a2 + f a3)), f the
+// position within the piece (exact); e outside [-8, 8) takes the end pieces
+// (erfc(8) = 1.1e-29).
+__device__ __forceinline__ float erfc_lds(float v32) {
     constexpr float half = 0.5f * (float)gx::kErfcPieces;
-    const float v = __builtin_amdgcn_fmed3f(e * gx::kErfcPerUnit, -half, half - 1.5e-5f);
-    const float fl = floorf(v);
-    const float f = v - fl;
-    const float4 c = gx_erfc_lds[(int)fl + gx::kErfcPieces / 2];
+    const float v = __builtin_amdgcn_fmed3f(v32, -half, half - 1.5e-5f);
+    int k;  // floor(v) as an integer in one instruction (v is finite after the clamp)
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(k) : "v"(v));
+    const float f = __builtin_amdgcn_fractf(v);
+    const float4 c = gx_erfc_lds[k + gx::kErfcPieces / 2];
     return fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
 }
 
@@ -454,41 +462,44 @@ __device__ __forceinline__ void gauss_staged(TileAcc<PPL>& t, GlobPos gpos, Glob
         rn = grec[li];
         xn = grecx[li];
         float* row = xs + lane * R;
-        const float Bh = rc.B - xc.y;  // the far edge at c: B - h
+        // A and the far edge at c, B - h, times the table's 32 per unit
+        const float A32 = rc.A * gx::kErfcPerUnit;
+        const float Bh32 = (rc.B - xc.y) * gx::kErfcPerUnit;
         row[0] = rc.cf;
-        row[1] = rc.A;
-        row[2] = Bh;
+        row[1] = A32;
+        row[2] = Bh32;
         row[3] = ldexpf(rc.Kg, rc.ex - t.E);
         row[4] = __int_as_float(pc.ib);
         row[5] = __int_as_float(pc.ie);
 #pragma unroll
         for (int q = 0; q < PPL; ++q)
-            row[6 + q] = erfc_lds(fmaf(rc.cf - (float)(w0 + GX_WAVE * q - 1), rc.A, Bh));
+            row[6 + q] = erfc_lds(fmaf(rc.cf - (float)(w0 + GX_WAVE * q - 1), A32, Bh32));
         const int nb = min(GX_WAVE, b - base);
         if (t.n > 0) t.fold();
+        // one line of the staged batch (row k) into the partial sums
+        auto line = [&](const float* rk) {
+            const float cf = rk[0], A32 = rk[1], Bh32 = rk[2], Kg = rk[3];
+            LinePos lp;
+            if (MASK != kMaskNone) {
+                lp.ib = __float_as_int(rk[4]);
+                lp.ie = __float_as_int(rk[5]);
+            }
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                if (PPL > 1 && MASK != kMaskNone) {
+                    LinePos ls;
+                    ls.ib = __builtin_amdgcn_readfirstlane(lp.ib);
+                    ls.ie = __builtin_amdgcn_readfirstlane(lp.ie);
+                    if (slice_in_window<MASK>(w0 + GX_WAVE * q, ls) == 0) continue;
+                }
+                const float Hb = erfc_lds(fmaf(cf - t.jf[q], A32, Bh32));
+                const float Ha = wave_shr1(Hb, rk[6 + q]);
+                t.part[q] = fmaf(Kg, in_window<MASK>(t.jj[q], lp) ? Hb - Ha : 0.0f, t.part[q]);
+            }
+        };
         for (int k0 = 0; k0 < nb; k0 += kChunk) {
             const int k1 = min(nb, k0 + kChunk);
-            for (int k = k0; k < k1; ++k) {
-                const float* rk = xs + k * R;
-                const float cf = rk[0], A = rk[1], Bh = rk[2], Kg = rk[3];
-                LinePos lp;
-                if (MASK != kMaskNone) {
-                    lp.ib = __float_as_int(rk[4]);
-                    lp.ie = __float_as_int(rk[5]);
-                }
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    if (PPL > 1 && MASK != kMaskNone) {
-                        LinePos ls;
-                        ls.ib = __builtin_amdgcn_readfirstlane(lp.ib);
-                        ls.ie = __builtin_amdgcn_readfirstlane(lp.ie);
-                        if (slice_in_window<MASK>(w0 + GX_WAVE * q, ls) == 0) continue;
-                    }
-                    const float Hb = erfc_lds(fmaf(cf - t.jf[q], A, Bh));
-                    const float Ha = wave_shr1(Hb, rk[6 + q]);
-                    t.part[q] = fmaf(Kg, in_window<MASK>(t.jj[q], lp) ? Hb - Ha : 0.0f, t.part[q]);
-                }
-            }
+            for (int k = k0; k < k1; ++k) line(xs + k * R);
             t.fold();
         }
     }
@@ -752,7 +763,9 @@ __device__ __forceinline__ int share_cut(int lo, int n, int k, int K) {
 template <int PROF, int PPL>
 __device__ __forceinline__ void sum_share(TileAcc<PPL>& t, const WorkItem& it, const Job& jb,
                                           int lane, int k, int K, float* xs) {
-    const int lo = it.lo, hi = max(it.hi, it.lo);
+    // bin-Gauss: only the lines within reach of the tile, [f2, f3)
+    const int lo = PROF == GX_PROF_DOPPLER ? it.f2 : it.lo;
+    const int hi = max(PROF == GX_PROF_DOPPLER ? it.f3 : it.hi, lo);
     const int m2 = min(max(it.s2, lo), hi), m3 = min(max(it.s3, lo), hi);
     const bool three = PROF != GX_PROF_DOPPLER && K > 1 && it.s1 < it.s4 && m3 - m2 >= K;
     const int np = three ? 3 : 1;

@@ -125,3 +125,51 @@ def test_voigt_quad_far_rule_against_the_11_point_sum(O):
     xl = np.sqrt(np.maximum(900.0 - yf * yf, 0.0)) + 10 ** rng.uniform(-6, 0, 2 * n)
     low = np.array([O.gh11(a, b) for a, b in zip(xl, yf)])
     assert np.max(np.abs(rational(xl, yf) - low) / low) > 2e-6
+
+
+def bin_gauss_reach(Af, Bf):
+    """The reach line_record (gexs_amd/csrc/kernels.hip) gives a bin-Gauss
+    line: ceil(((8 (1 + 1e-5) + |B|) / A) (1 + 1e-5) + 1/2), in FP64."""
+    return np.ceil(((8.0 * (1.0 + 1e-5) + np.abs(Bf)) / Af) * (1.0 + 1e-5) + 0.5)
+
+
+def clamped_edge_f32(c, j, A, B, h, n, per):
+    """The kernel's table argument at the far edge of point j, clamped as
+    erfc_lds clamps it: fma(cf - jf, 32 A, 32 (B - H)) in FP32."""
+    f = np.float32
+    A32 = (A * f(per)).astype(f)
+    Bh32 = ((B - h).astype(f) * f(per)).astype(f)
+    v = fma32((c.astype(f) - j.astype(f)), A32, Bh32)
+    return np.clip(v, f(-n / 2), f(n / 2) - f(1.5e-5))
+
+
+def test_bin_gauss_reach_adds_exact_zeros():
+    """Beyond its reach K a bin-Gauss line adds exactly zero: at every point j
+    with |c - j| >= K both edges of the bin (j's far edge and j - 1's) clamp
+    to the same end of the erfc table, so the kernel's difference is 0 and the
+    tiles may skip the line.  Checked in FP32 as the kernel computes the edges
+    over widths from 0.01 to 30 points per width and offsets |B| <= A / 2; the
+    bound is tight to within two points, and dropping its half-bin term lets
+    nonzero bins through."""
+    n, per = erfc_table_constants()
+    f = np.float32
+    rng = np.random.default_rng(5)
+    m = 400000
+    Af = 10 ** rng.uniform(-2.0, 1.5, m)
+    Bf = rng.uniform(-0.5, 0.5, m) * Af
+    A, B, h = Af.astype(f), Bf.astype(f), (0.5 * Af).astype(f)
+    c = rng.integers(0, 1 << 23, m)
+    K = bin_gauss_reach(Af, Bf).astype(np.int64)
+    assert np.all(K >= 1)
+
+    def same_clamp(j):
+        return clamped_edge_f32(c, j, A, B, h, n, per) == clamped_edge_f32(c, j - 1, A, B, h, n, per)
+
+    for d in range(0, 6):
+        assert np.all(same_clamp(c - K - d)), d
+        assert np.all(same_clamp(c + K + d)), d
+    # tight: two points inside the reach some bins are not zero
+    assert not np.all(same_clamp(c + K - 2)) and not np.all(same_clamp(c - K + 2))
+    # without the half bin the bound fails
+    K0 = np.ceil((8.0 + np.abs(Bf)) / Af).astype(np.int64)
+    assert not (np.all(same_clamp(c - K0)) and np.all(same_clamp(c + K0)))

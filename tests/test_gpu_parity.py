@@ -177,10 +177,20 @@ def c0_full(O):
                                              (8, 16, 1)])
 def test_doppler_full_config_vs_oracle(c0_full, ppl, split, order):
     lines, T, Q, ref, ne = c0_full
-    out, st, _ = run_manager("doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0,
+    out, st, m = run_manager("doppler", lines, T, 0.0, Q, 0.0, 10000.0, 1_000_000, 1.0,
                              ppl=ppl, split=split, order=order)
     assert st["n_eval"] == ne
     check(out, ref, 1e-4, 1e-6, f"doppler fp32c ppl={ppl} split={split} order={order}")
+    # each tile sums only the lines centred within reach of it, [f2, f3) of
+    # its window's lines [lo, hi): at 1000 K a line at 1e4 cm^-1 reaches 23
+    # points of 0.01 cm^-1, against the 100 of its cutoff (the wider the
+    # tile, the smaller the share of lines the cut removes)
+    it = m.pass_items()
+    lo, hi, w0, w1, f2, f3 = (it[:, k] for k in (0, 1, 2, 3, 10, 11))
+    assert np.all((lo <= f2) & (f2 <= f3) & (f3 <= np.maximum(hi, lo)))
+    kept, window = int(np.sum(f3 - f2)), int(np.sum(np.maximum(hi - lo, 0)))
+    log(f"  lines within reach: {kept} of {window} line-tile pairs")
+    assert kept < (0.75 if ppl <= 2 else 0.9) * window
 
 
 def test_doppler_fp64_vs_oracle(c0_full):
