@@ -256,13 +256,14 @@ struct gexs_ctx {
     DevArr item_tab; // window and job tables of the tiles kernel
     const void* last_neval = nullptr;  // N_eval of the last pass (device)
     const WorkItem* last_items = nullptr;  // work items the last FP32C profile kernel read
+    const int* last_kmax = nullptr;        // and the kmax it located their segments with
 
     // execute path: the per-line pre-pass of a pass runs on `pre` into one of
     // kPassSets buffer sets while the profile kernels of earlier passes run
     // on `stream` from the others
     hipStream_t pre = nullptr;
     struct PassBuf {
-        DevArr tab, small, rec, recx, items;
+        DevArr tab, small, rec, recx;
         hipEvent_t pre_done = nullptr, prof_done = nullptr;
         // after the set's last profile kernel: prof_done, or that execute's
         // timing event (a later record of it on `stream` marks a later point)
@@ -608,7 +609,6 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     if ((rc = pass_tables_in(c, B.tab, B.small, 1, &t)) != GEXS_OK) return rc;
     HIPCHK(c, B.rec.ensure((size_t)std::max(n, 1) * sizeof(VRec)));
     HIPCHK(c, B.recx.ensure((size_t)std::max(n, 1) * sizeof(VRecX)));
-    HIPCHK(c, B.items.ensure((size_t)std::max(c->n_items, 1) * sizeof(WorkItem)));
     HIPCHK(c, c->counter.ensure(256));
     Job jb{};
     jb.pos = c->pos.as<LinePos>();
@@ -643,19 +643,13 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, gxk::line_rec(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale, B.rec.as<VRec>(),
                             B.recx.as<VRecX>(), t.kmax, t.neval, c->cus));
-    // the items carry this pass's segment bounds (a copy per buffer set):
-    // near-centre and far lines (Voigt profiles) or the lines within reach
-    // of each tile (bin-Gauss), from this pass's kmax
-    HIPCHK(c, gxk::segments(P, c->items.as<WorkItem>(), B.items.as<WorkItem>(), c->n_items, t.jd,
-                            c->pos.as<LinePos>(), t.kmax, c->profile));
     HIPCHK(c, hipEventRecord(B.pre_done, P));
     hipStream_t s = c->stream;
     HIPCHK(c, hipStreamWaitEvent(s, B.pre_done, 0));
     if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split,
-                                 B.items.as<WorkItem>(), c->n_items, t.jobs,
-                                 t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
-                                 launch_blocks(c), true));
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
+                                 t.jobs, t.scale, t.kmax, t.xq, c->counter.as<unsigned int>(),
+                                 launch_mode(c), launch_blocks(c), true));
     if (timed) {
         HIPCHK(c, hipEventRecord(c->ev_k1, s));
         B.done = c->ev_k1;
@@ -666,7 +660,8 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     B.used = true;
     c->last.n_tiles = c->n_items;
     c->last_neval = t.neval;
-    c->last_items = B.items.as<WorkItem>();
+    c->last_items = c->items.as<WorkItem>();
+    c->last_kmax = t.kmax;
     return GEXS_OK;
 }
 
@@ -1031,6 +1026,13 @@ int gexs_pass_items(gexs_ctx* c, int64_t cap, int32_t* out, int64_t* n) {
     *n = m;
     if (m == 0 || cap == 0) return GEXS_OK;
     if (cap < m) return c->fail(GEXS_EINVAL, "pass_items: cap is smaller than the item count");
+    // the segments the profile kernel located (its waves do not store them):
+    // the same search over the same kmax, into the items' segment fields
+    if (c->last_kmax) {
+        HIPCHK(c, gxk::segments(c->stream, const_cast<WorkItem*>(c->last_items), (int)m,
+                                c->pos.as<LinePos>(), c->last_kmax, c->profile));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     std::vector<WorkItem> h((size_t)m);
     HIPCHK(c, hipMemcpy(h.data(), c->last_items, (size_t)m * sizeof(WorkItem), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < m; ++i) {
@@ -1180,12 +1182,9 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, const PassTables& t) {
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale,
                             c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval));
-    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->items.as<WorkItem>(), c->n_items, t.jd,
-                            c->pos.as<LinePos>(), t.kmax,
-                            c->profile));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
-                                 t.jobs, t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
-                                 launch_blocks(c)));
+                                 t.jobs, t.scale, t.kmax, t.xq, c->counter.as<unsigned int>(),
+                                 launch_mode(c), launch_blocks(c)));
     return GEXS_OK;
 }
 
@@ -1345,6 +1344,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     HIPCHK(c, hipEventRecord(c->ev_k1, c->stream));
     c->last.n_tiles = c->n_items;
     c->last_items = c->items.as<WorkItem>();
+    c->last_kmax = t.kmax;
     if (out) HIPCHK(c, hipMemcpyAsync(out, c->batch_out.p, np * (size_t)ntp * 8, hipMemcpyDefault, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<unsigned long long> ne((size_t)ntp, 0ull);
@@ -1529,13 +1529,12 @@ static int stream_chunk_pass(gexs_ctx* c, const std::vector<TPParams>& htp, int6
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(), gam,
                             nex, c->pos.as<LinePos>(), t.jd, ntp, n, t.tp, t.scale, c->rec.as<VRec>(),
                             c->recx.as<VRecX>(), t.kmax, t.neval));
-    HIPCHK(c, gxk::segments(s, c->items.as<WorkItem>(), c->items.as<WorkItem>(), c->n_items, t.jd,
-                            c->pos.as<LinePos>(), t.kmax, c->profile));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items, t.jobs,
-                                 t.scale, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
+                                 t.scale, t.kmax, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
                                  launch_blocks(c)));
     c->last.n_tiles = c->n_items;
     c->last_items = c->items.as<WorkItem>();
+    c->last_kmax = t.kmax;
     return GEXS_OK;
 }
 
@@ -1962,13 +1961,12 @@ int gexs_run_fixed(gexs_ctx* c, const gexs_lines_dev* L, const gexs_tp* tp, cons
                                 kmax, neval));
     }
     HIPCHK(c, gxk::tiles(s, pos, djd, 1, (int)ntiles, dtp, span, items, nullptr, nullptr));
-    HIPCHK(c, gxk::segments(s, items, items, (int)ntiles, djd, pos, kmax, c->profile));
     // launch shapes that need no host-built queue bounds or tile sort: one
     // slot per tile (grid or dispatch order) or persistent workgroups on one
     // counter; every shape of one tile width and split gives the same bits
     const int mode = (c->persistent_blocks > 0 && c->order != 2) ? 1 : (c->order == 0 ? 0 : 3);
-    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, items, (int)ntiles, djob, scale, xq,
-                                 xq, mode, launch_blocks(c), true));
+    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, items, (int)ntiles, djob, scale, kmax,
+                                 xq, xq, mode, launch_blocks(c), true));
     if (n_eval) {
         unsigned long long ne = 0;
         HIPCHK(c, hipMemcpyAsync(&ne, neval, 8, hipMemcpyDeviceToHost, s));

@@ -23,11 +23,12 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
                     unsigned long long* neval, int max_blocks = 0);
-// segment bounds of every work item (after line_rec has produced kmax per
-// job): out[i] = in[i] with f2, s2, s3, f3 set (out may equal in); the
-// bin-Gauss profiles set f2, f3 only (the lines within reach of the tile)
-hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,
-                    const JobDesc* jd, const LinePos* pos, const int* kmax, int profile);
+// the segment bounds the profile kernel locates for every work item from
+// kmax (f2, s2, s3, f3; the bin-Gauss profiles f2, f3 only, the lines within
+// reach of the tile), stored into the items: for inspection, the profile
+// kernel does not read them.  pos: the lines of every job (pos_off 0).
+hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const LinePos* pos,
+                    const int* kmax, int profile);
 // jobs job0 .. job0 + njobs - 1 of jd / tp, all with tiles of span points
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
@@ -49,18 +50,20 @@ hipError_t init_gauss_tables(hipStream_t s);
 // tiles_gauss.hip); single != 0: all items belong to jobs[0]
 hipError_t profile_tiles_voigt(hipStream_t s, int ppl, int split, const WorkItem* items, int nitems,
                                const Job* jobs, int single, const unsigned long long* scale_bits,
-                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
+                               const int* kmax, unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
 hipError_t profile_tiles_quad(hipStream_t s, int ppl, int split, const WorkItem* items, int nitems,
                               const Job* jobs, int single, const unsigned long long* scale_bits,
-                              unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
+                              const int* kmax, unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
 hipError_t profile_tiles_gauss(hipStream_t s, int ppl, int split, const WorkItem* items, int nitems,
                                const Job* jobs, int single, const unsigned long long* scale_bits,
-                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
-// single_job: all items belong to jobs[0] (read without waiting for the item)
+                               const int* kmax, unsigned int* xq, const unsigned int* xqb, int mode, int blocks);
+// single_job: all items belong to jobs[0] (read without waiting for the item).
+// kmax: the jobs' largest k15 and kfar (or reach) from line_rec, from which
+// each wave locates its item's segments.
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,
-                         unsigned int* xq, const unsigned int* xqb, int mode, int blocks,
-                         bool single_job = false);
+                         const int* kmax, unsigned int* xq, const unsigned int* xqb, int mode,
+                         int blocks, bool single_job = false);
 hipError_t line_f64(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, int n,
                     const LinePos* pos, const TPParams* tp, LineF64* out,

@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.job = job;
     it.s1 = lf < rf ? lf : hi;
     it.s4 = lf < rf ? rf : hi;
-    it.s2 = it.s3 = it.s4;  // [s1, s4) all region I until k_segments runs
+    it.s2 = it.s3 = it.s4;  // the profile kernel locates s2, s3, f2, f3 per pass
     const bool gauss = tp[job].profile == GX_PROF_DOPPLER || tp[job].profile == GX_PROF_GAUSSIAN;
     it.f2 = gauss ? lo : it.s1;  // bin-Gauss: every line within reach until then
     it.f3 = gauss ? hi : it.s4;
@@ -344,54 +344,22 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     }
 }
 
-// Near-centre and far segments of every work item: among the lines
-// [s1, s4) that cover the whole tile (set by k_tiles), those whose centre c
-// lies within K = kmax[2 job] of the tile, [s2, s3), may have points outside
-// region I, and those whose centre lies KF = kmax[2 job + 1] or more from it,
-// [s1, f2) and [f3, s4), take the one-term wing form at every point (K and KF
-// are the largest k15 and kfar of the job's lines, T,P-dependent, from
-// k_line_rec; KF >= K).  Bin-Gauss: [f2, f3) are the lines of [lo, hi) whose
-// centre lies within KF, the largest reach, of the tile; the others add
-// exactly zero to it.  Only lo, hi, s1 and s4 are read, so out may equal in
-// (in place) or be a per-pass copy.
-__device__ __forceinline__ int first_c_at_least(const LinePos* jp, int a, int b, long long v) {
-    while (a < b) {
-        const int m = (a + b) >> 1;
-        if ((long long)jp[m].c >= v) b = m; else a = m + 1;
-    }
-    return a;
-}
-
-__global__ __launch_bounds__(256) void k_segments(const WorkItem* in, WorkItem* out, int nitems,
-                                                  const JobDesc* __restrict__ jd,
+// The segments of every work item as the profile kernel locates them
+// (locate_segments, one wave per item), stored into the item: for
+// inspection only, the profile kernel locates them itself.  pos: the lines of
+// every job (pos_off 0).
+__global__ __launch_bounds__(256) void k_segments(WorkItem* items, int nitems,
                                                   const LinePos* __restrict__ pos,
                                                   const int* __restrict__ kmax, bool gauss) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & (GX_WAVE - 1);
     if (t >= nitems) return;
-    WorkItem it = in[t];
-    if (gauss) {
-        const LinePos* jp = pos + jd[it.job].pos_off;
-        const int KF = kmax[2 * it.job + 1];
-        const int hi = max(it.hi, it.lo);
-        it.f2 = first_c_at_least(jp, it.lo, hi, (long long)it.w0 - KF);
-        it.f3 = first_c_at_least(jp, it.f2, hi, (long long)it.w1 + KF);
-        out[t] = it;
-        return;
-    }
-    int f2 = it.s1, s2 = it.s4, s3 = it.s4, f3 = it.s4;
-    if (it.s1 < it.s4) {
-        const LinePos* jp = pos + jd[it.job].pos_off;
-        const int K = kmax[2 * it.job], KF = kmax[2 * it.job + 1];
-        f2 = first_c_at_least(jp, it.s1, it.s4, (long long)it.w0 - KF);
-        s2 = first_c_at_least(jp, f2, it.s4, (long long)it.w0 - K);
-        s3 = first_c_at_least(jp, s2, it.s4, (long long)it.w1 + K);
-        f3 = first_c_at_least(jp, s3, it.s4, (long long)it.w1 + KF);
-    }
-    it.f2 = f2;
-    it.s2 = s2;
-    it.s3 = s3;
-    it.f3 = f3;
-    out[t] = it;
+    WorkItem it = items[t];
+    if (gauss)
+        locate_segments<true>(it, pos, kmax, lane);
+    else
+        locate_segments<false>(it, pos, kmax, lane);
+    if (lane == 0) items[t] = it;
 }
 
 __global__ __launch_bounds__(256) void k_permute_items(const WorkItem* __restrict__ in,
@@ -606,12 +574,12 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
     return hipGetLastError();
 }
 
-hipError_t segments(hipStream_t s, const WorkItem* in, WorkItem* out, int nitems,
-                    const JobDesc* jd, const LinePos* pos, const int* kmax, int profile) {
+hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const LinePos* pos,
+                    const int* kmax, int profile) {
     if (nitems <= 0) return hipSuccess;
     const bool gauss = profile != GX_PROF_VOIGT && profile != GX_PROF_VOIGT_QUAD;
-    hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 256)), dim3(256), 0, s, in, out, nitems,
-                       jd, pos, kmax, gauss);
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 4)), dim3(256), 0, s, items, nitems, pos,
+                       kmax, gauss);
     return hipGetLastError();
 }
 
@@ -641,19 +609,19 @@ hipError_t permute_items(hipStream_t s, const WorkItem* in, const uint32_t* idx,
 
 hipError_t profile_tiles(hipStream_t s, int profile, int ppl, int split, const WorkItem* items,
                          int nitems, const Job* jobs, const unsigned long long* scale_bits,
-                         unsigned int* xq, const unsigned int* xqb, int mode, int blocks,
-                         bool single_job) {
+                         const int* kmax, unsigned int* xq, const unsigned int* xqb, int mode,
+                         int blocks, bool single_job) {
     const int single = single_job ? 1 : 0;
     switch (profile) {
         case GX_PROF_VOIGT:
-            return profile_tiles_voigt(s, ppl, split, items, nitems, jobs, single, scale_bits, xq,
+            return profile_tiles_voigt(s, ppl, split, items, nitems, jobs, single, scale_bits, kmax, xq,
                                        xqb, mode, blocks);
         case GX_PROF_VOIGT_QUAD:
-            return profile_tiles_quad(s, ppl, split, items, nitems, jobs, single, scale_bits, xq,
+            return profile_tiles_quad(s, ppl, split, items, nitems, jobs, single, scale_bits, kmax, xq,
                                       xqb, mode, blocks);
         case GX_PROF_DOPPLER:
         case GX_PROF_GAUSSIAN:
-            return profile_tiles_gauss(s, ppl, split, items, nitems, jobs, single, scale_bits, xq,
+            return profile_tiles_gauss(s, ppl, split, items, nitems, jobs, single, scale_bits, kmax, xq,
                                        xqb, mode, blocks);
         default:
             return hipErrorInvalidValue;

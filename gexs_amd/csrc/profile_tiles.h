@@ -30,6 +30,85 @@ __device__ __forceinline__ int pass_exponent(unsigned long long biased) {
     return biased == 0ull ? 0 : (int)biased - kExBias;
 }
 
+// For each n, the first index i of [a, b) with jp[i].c >= v[n] (b when there
+// is none); c is non-decreasing in i.  The whole wave searches N bounds at
+// once, 64 ways per round: lane k samples the last line of the k-th of 64
+// equal chunks (the last lane line b - 1), and the first chunk whose sample
+// reaches v holds the answer (1 round for 64 lines, 2 for 4096, 3 for 2^18).
+template <int N>
+__device__ __forceinline__ void wave_first_c(const LinePos* jp, int a0, int b0,
+                                             const long long (&v)[N], int (&out)[N], int lane) {
+    int a[N], b[N];
+    bool open = false;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+        a[n] = a0;
+        b[n] = b0;
+        out[n] = a0 < b0 ? -1 : b0;
+        open = open || a0 < b0;
+    }
+    while (open) {
+        int c[N];
+#pragma unroll
+        for (int n = 0; n < N; ++n) {  // the N samples in flight together
+            const int s = (b[n] - a[n] + GX_WAVE - 1) / GX_WAVE;
+            c[n] = out[n] < 0 ? jp[min(a[n] + (lane + 1) * s - 1, b[n] - 1)].c : 0;
+        }
+        open = false;
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+            if (out[n] >= 0) continue;
+            const int s = (b[n] - a[n] + GX_WAVE - 1) / GX_WAVE;
+            const unsigned long long m = __ballot((long long)c[n] >= v[n]);
+            if (m == 0ull) {  // b - 1 too is below v
+                out[n] = b[n];
+                continue;
+            }
+            const int k = __builtin_ctzll(m);
+            if (s == 1) {
+                out[n] = a[n] + k;
+                continue;
+            }
+            b[n] = min(a[n] + (k + 1) * s - 1, b[n] - 1) + 1;  // its sample reaches v
+            a[n] = a[n] + k * s;                               // chunk k - 1's does not
+            open = true;
+        }
+    }
+}
+
+// The pass-dependent segments of a work item (kmax[2 job] = K, the job's
+// largest k15, and kmax[2 job + 1] = KF, its largest kfar or reach, from this
+// pass's line records):
+//   Voigt profiles: among the lines [s1, s4) that cover the whole tile, f2,
+//   s2, s3 and f3 are the first with c >= w0 - KF, w0 - K, w1 + K, w1 + KF:
+//   [s2, s3) may have points outside region I, [s1, f2) and [f3, s4) take
+//   the one-term wing form at every point.
+//   bin-Gauss: [f2, f3) are the lines of [lo, hi) with c in [w0 - KF,
+//   w1 + KF), the only ones that add to the tile.
+// Every lane of the wave gets the same values.
+template <bool GAUSS>
+__device__ __forceinline__ void locate_segments(WorkItem& it, const LinePos* jp, const int* kmax,
+                                                int lane) {
+    const long long KF = kmax[2 * it.job + 1];
+    if constexpr (GAUSS) {
+        const long long v[2] = {(long long)it.w0 - KF, (long long)it.w1 + KF};
+        int r[2];
+        wave_first_c<2>(jp, it.lo, max(it.hi, it.lo), v, r, lane);
+        it.f2 = r[0];
+        it.f3 = r[1];
+    } else {
+        const long long K = kmax[2 * it.job];
+        const long long v[4] = {(long long)it.w0 - KF, (long long)it.w0 - K, (long long)it.w1 + K,
+                                (long long)it.w1 + KF};
+        int r[4];
+        wave_first_c<4>(jp, it.s1, max(it.s4, it.s1), v, r, lane);
+        it.f2 = r[0];
+        it.s2 = r[1];
+        it.s3 = r[2];
+        it.f3 = r[3];
+    }
+}
+
 // --------------------------------------------------------- profile tiles
 namespace {
 
@@ -875,6 +954,7 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
                                                          int nitems, const Job* __restrict__ jobs,
                                                          int single,
                                                          const unsigned long long* __restrict__ scale_bits,
+                                                         const int* __restrict__ kmax,
                                                          unsigned int* __restrict__ xq,
                                                          const unsigned int* __restrict__ xqb, int mode) {
     // per wave, 64 rows of wave_row() words: the staged rows of one batch of
@@ -899,13 +979,15 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
         if (SPLIT == 1) {
             const int idx = lb * kWavesPerBlock + wv;
             if (idx >= nitems) return;
-            const WorkItem it = items[idx];
+            WorkItem it = items[idx];
             const Job jb = jobs[single ? 0 : it.job];
+            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, lane);
             run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
         } else {
             if (lb >= nitems) return;
-            const WorkItem it = items[lb];
+            WorkItem it = items[lb];
             const Job jb = jobs[single ? 0 : it.job];
+            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, lane);
             run_item_split<PROF, PPL, SPLIT>(it, jb, scale_bits, lane, wv, red, xs);
         }
         return;
@@ -929,8 +1011,9 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
                 __syncthreads();  // grab is rewritten by the next pull
             }
             if (idx >= end) break;
-            const WorkItem it = items[idx];
+            WorkItem it = items[idx];
             const Job jb = jobs[single ? 0 : it.job];
+            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, lane);
             if (SPLIT == 1)
                 run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
             else
@@ -947,27 +1030,28 @@ inline int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
 template <int PROF, int PPL, int SPLIT>
 inline hipError_t launch_tiles_t(hipStream_t s, const WorkItem* items, int nitems,
                                  const Job* jobs, int single,
-                                 const unsigned long long* scale_bits, unsigned int* xq,
-                                 const unsigned int* xqb, int mode, int blocks) {
+                                 const unsigned long long* scale_bits, const int* kmax,
+                                 unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
     if (nitems <= 0) return hipSuccess;
     const int ipb = SPLIT == 1 ? kWavesPerBlock : 1;  // items per block
     const bool slots = mode == 0 || mode == 3;  // one launch slot per item
     const int nb = slots ? blocks_for(nitems, ipb)
                          : std::max(1, std::min(blocks, blocks_for(nitems, ipb)));
     hipLaunchKernelGGL((k_profile_tiles<PROF, PPL, SPLIT>), dim3(nb), dim3(block_threads<SPLIT>()), 0, s, items,
-                       nitems, jobs, single, scale_bits, xq, xqb, mode);
+                       nitems, jobs, single, scale_bits, kmax, xq, xqb, mode);
     return hipGetLastError();
 }
 
 template <int PROF, int SPLIT>
 inline hipError_t launch_ppl(hipStream_t s, int ppl, const WorkItem* items, int nitems,
                              const Job* jobs, int single, const unsigned long long* scale_bits,
-                             unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
+                             const int* kmax, unsigned int* xq, const unsigned int* xqb, int mode,
+                             int blocks) {
     switch (ppl) {
-        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
-        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
-        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
-        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
+        case 1: return launch_tiles_t<PROF, 1, SPLIT>(s, items, nitems, jobs, single, scale_bits, kmax, xq, xqb, mode, blocks);
+        case 2: return launch_tiles_t<PROF, 2, SPLIT>(s, items, nitems, jobs, single, scale_bits, kmax, xq, xqb, mode, blocks);
+        case 4: return launch_tiles_t<PROF, 4, SPLIT>(s, items, nitems, jobs, single, scale_bits, kmax, xq, xqb, mode, blocks);
+        case 8: return launch_tiles_t<PROF, 8, SPLIT>(s, items, nitems, jobs, single, scale_bits, kmax, xq, xqb, mode, blocks);
         default: return hipErrorInvalidValue;
     }
 }
@@ -975,12 +1059,12 @@ inline hipError_t launch_ppl(hipStream_t s, int ppl, const WorkItem* items, int 
 template <int PROF>
 inline hipError_t launch_prof(hipStream_t s, int ppl, int split, const WorkItem* items,
                               int nitems, const Job* jobs, int single,
-                              const unsigned long long* scale_bits,
+                              const unsigned long long* scale_bits, const int* kmax,
                               unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
-    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
-    if (split == 4) return launch_ppl<PROF, 4>(s, ppl, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
+    if (split == 1) return launch_ppl<PROF, 1>(s, ppl, items, nitems, jobs, single, scale_bits, kmax, xq, xqb, mode, blocks);
+    if (split == 4) return launch_ppl<PROF, 4>(s, ppl, items, nitems, jobs, single, scale_bits, kmax, xq, xqb, mode, blocks);
     if (split == 16)
-        return launch_ppl<PROF, 16>(s, ppl, items, nitems, jobs, single, scale_bits, xq, xqb, mode, blocks);
+        return launch_ppl<PROF, 16>(s, ppl, items, nitems, jobs, single, scale_bits, kmax, xq, xqb, mode, blocks);
     return hipErrorInvalidValue;
 }
 

@@ -481,10 +481,6 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
                 goto fail;
             k0 = k1;
         }
-        if ((e = gxk::segments(st, d_items, d_items, items_total, d_jd, (const LinePos*)base, d_kmax,
-                               ph_.profile)) !=
-            hipSuccess)
-            goto fail;
         for (int k0 = 0; k0 < nreq;) {  // one profile launch per group, slots in dispatch order
             int k1 = k0, i1 = jds[(size_t)k0].items_off;
             bool grouped = false;
@@ -496,7 +492,8 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
             const int i0 = jds[(size_t)k0].items_off;
             if ((e = gxk::profile_tiles(st, ph_.profile, batch[(size_t)k0]->ppl, batch[(size_t)k0]->split,
                                         d_items + i0,
-                                        i1 - i0, d_jobs, d_scale, d_counter, nullptr, 3, 0)) != hipSuccess)
+                                        i1 - i0, d_jobs, d_scale, d_kmax, d_counter, nullptr, 3, 0)) !=
+                hipSuccess)
                 goto fail;
             if (grouped &&
                 (e = gxk::group_sum(st, d_items + i0, i1 - i0, d_jobs, GX_WAVE * batch[(size_t)k0]->ppl)) !=

@@ -5,8 +5,8 @@ namespace gxk {
 
 hipError_t profile_tiles_quad(hipStream_t s, int ppl, int split, const WorkItem* items, int nitems,
                               const Job* jobs, int single, const unsigned long long* scale_bits,
-                              unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
-    return tile_launch::launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, single, scale_bits, xq,
+                              const int* kmax, unsigned int* xq, const unsigned int* xqb, int mode, int blocks) {
+    return tile_launch::launch_prof<GX_PROF_VOIGT_QUAD>(s, ppl, split, items, nitems, jobs, single, scale_bits, kmax, xq,
                                                         xqb, mode, blocks);
 }
 
