@@ -469,12 +469,15 @@ def valu_probe():
 
 
 def execute_bench(a, workload, profile=None, steps=None, warm=None, fp64_steps=None,
-                  tuned=None, log_candidates=False):
+                  tuned=None, log_candidates=False, alone=10):
     """ExecuteCrossSection over the resident list of `workload` (its profile,
     or `profile`): autotuned launch shape (or `tuned`, or --ppl), `warm`
     untimed then `steps` timed steps, HIP events around each profile kernel;
-    then the plain FP64 path on the same input.  Returns the measurements
-    and the manager inputs the headline also uses."""
+    then `alone` steps each followed by a synchronisation (the profile kernel
+    with nothing beside it: consecutive executes of short kernels overlap in
+    the timed steps, which lengthens each kernel's event span); then the plain
+    FP64 path on the same input.  Returns the measurements and the manager
+    inputs the headline also uses."""
     from gexs_amd import synth
     from gexs_amd.manager import HipManager
     spec = synth.WORKLOADS[workload]
@@ -494,7 +497,7 @@ def execute_bench(a, workload, profile=None, steps=None, warm=None, fp64_steps=N
     elif a.ppl:
         m.set_tuning(a.ppl, a.order, a.split)
         m.set_launch(a.persistent, False)
-    m.InitializeVectors(N, steps + warm)
+    m.InitializeVectors(N, steps + warm + alone)
     m.InitializeConstants(100.0, T, 1.0, dnu, spec["mass"], P, synth.T_REF, synth.P_SCALE_BAR,
                           cutoff=spec["cutoff"])
     m.TransferFreq(start + np.arange(N, dtype=np.float64) * dnu)
@@ -524,7 +527,13 @@ def execute_bench(a, workload, profile=None, steps=None, warm=None, fp64_steps=N
     k_ms, _, cnt = m.timing()
     assert cnt == steps, (cnt, steps)
     st = m.stats()
+    m.timing(reset=True)
+    for _ in range(alone):
+        m.ExecuteCrossSection(N, n, 0, T, P, Q)
+        m.sync()
+    k_alone_ms, _, cnt_alone = m.timing()
     m.close()
+    k_alone = k_alone_ms / cnt_alone if cnt_alone else None
     value = n_eval * steps / elapsed
     k_avg = k_ms / cnt
     flops = FLOP_PER_EVAL[prof]
@@ -557,7 +566,7 @@ def execute_bench(a, workload, profile=None, steps=None, warm=None, fp64_steps=N
         "persistent_blocks": a.persistent})
     return {"workload": workload, "profile": prof, "lst": lst, "T": T, "P": P, "Q": Q, "n": n,
             "spec": spec, "steps": steps, "warm": warm, "elapsed": elapsed, "value": value,
-            "n_eval": n_eval, "k_avg": k_avg, "flops": flops, "achieved": achieved,
+            "n_eval": n_eval, "k_avg": k_avg, "k_alone": k_alone, "flops": flops, "achieved": achieved,
             "tuning": tuning, "used": used, "tiles": st["n_tiles"], "fp64": fp64}
 
 
@@ -577,6 +586,10 @@ def roofline_of(r, prof=None, probe=None):
            "traffic": prof.get("hbm_bytes_per_launch") if prof else None,
            "kernel": f"k_profile_tiles<{r['profile'].upper()}>", "valu_probe": probe,
            "kernel_ms_avg": r["k_avg"], "flop_per_eval": r["flops"]}
+    if r.get("k_alone"):
+        # the kernel with nothing beside it (each execute synchronised)
+        out["kernel_ms_alone"] = r["k_alone"]
+        out["frac_alone"] = r["n_eval"] * r["flops"] / (r["k_alone"] * 1e-3) / PEAK_FP32
     if prof and prof.get("valu_insts_per_launch"):
         same = all(prof["launch"].get(k) == r["used"].get(k)
                    for k in ("points_per_lane", "waves_per_tile"))
@@ -587,10 +600,10 @@ def roofline_of(r, prof=None, probe=None):
     return out
 
 
-def side_workload(a, workload, profile=None):
+def side_workload(a, workload, profile=None, steps=None):
     """An extra execute-path workload for the JSON: throughput, roofline and
     the plain FP64 path beside it."""
-    r = execute_bench(a, workload, profile=profile, steps=max(5, a.steps // 2),
+    r = execute_bench(a, workload, profile=profile, steps=steps or max(5, a.steps // 2),
                       warm=max(1, a.warmup), fp64_steps=max(1, a.fp64_steps))
     sp = r["spec"]
     start, end, N = sp["grid"]
@@ -661,7 +674,8 @@ def main():
 
     doppler = voigt_quad = None
     if not a.no_profiles:
-        doppler = side_workload(a, "doppler_2e5")
+        # ~30 us passes: enough of them for a timed region of ~10 ms
+        doppler = side_workload(a, "doppler_2e5", steps=max(5, 15 * a.steps))
         voigt_quad = side_workload(a, "voigt_1e6", profile="voigt_quad")
     table = None if a.no_table else table_bench(a)
     served = None if a.no_serve else serve_bench(a)

@@ -208,10 +208,14 @@ void scratch_init(int device) {
 }
 }  // namespace
 
-// buffer sets of the execute path: the pre-pass of pass k writes set
-// k mod kPassSets once the profile kernel of pass k - kPassSets has read it,
-// so it may run during the profile kernels of the kPassSets - 1 passes before
-constexpr int kPassSets = 3;
+// buffer sets of the execute path: pass k uses set k mod kPassSets and clears
+// the counters of set k + 1 (serial mode) or k + 2 (alternating mode)
+constexpr int kPassSets = 4;
+// profile kernels at least this long (ms, the last measured execute) run
+// back to back on `stream`, their line records on `pre` (joined by events,
+// ~30 us of host time per pass); shorter passes alternate between the two
+// streams with no event per pass
+constexpr float kSerialKernelMs = 0.15f;
 
 struct gexs_ctx {
     int device = 0;
@@ -259,21 +263,26 @@ struct gexs_ctx {
     const int* last_kmax = nullptr;        // and the kmax it located their segments with
 
     // execute path: the per-line pre-pass of a pass runs on `pre` into one of
-    // kPassSets buffer sets while the profile kernels of earlier passes run
-    // on `stream` from the others
+    // execute path: the passes alternate between `stream` and `pre`, so one
+    // pass's line records overlap the profile kernel of the pass before
     hipStream_t pre = nullptr;
     struct PassBuf {
         DevArr tab, small, rec, recx;
-        hipEvent_t pre_done = nullptr, prof_done = nullptr;
-        // after the set's last profile kernel: prof_done, or that execute's
-        // timing event (a later record of it on `stream` marks a later point)
-        hipEvent_t done = nullptr;
-        bool used = false;
+        bool clean = false;  // counters zero (cleared by the pass two before)
+        hipEvent_t pre_done = nullptr, done = nullptr;  // line records / profile kernel
+        bool done_recorded = false;
     };
     PassBuf pb[kPassSets];
-    int pbuf = 0;
-    hipEvent_t geom = nullptr;  // on `stream` after positions and tiles were (re)built
-    bool geom_new = false;      // geom recorded since the pre-pass stream last waited on it
+    int64_t npass = 0;  // passes of the execute path so far
+    // work on `pre` that `stream` has not waited for, and work on `stream`
+    // (anything but the execute passes) that `pre` has not waited for
+    bool pre_pending = false, main_dirty = true;
+    hipEvent_t ev_pre = nullptr, ev_main = nullptr;
+    // profile-kernel time of the last execute whose events are complete, the
+    // timing events of the last execute, and the mode of the last pass
+    float pass_kernel_ms = 1.0f;
+    hipEvent_t prev_k0 = nullptr, prev_k1 = nullptr;
+    int pass_mode = -1;  // 1: serial, 0: alternating, -1: none yet
     int cus = 256;  // compute units of the device
     PosKey pos_key;
     PosKey items_key;
@@ -365,9 +374,30 @@ int next_events(gexs_ctx* c) {
     return GEXS_OK;
 }
 
-int set_device(gexs_ctx* c) {
+int set_device_only(gexs_ctx* c) {
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return c->hipfail(e, "hipSetDevice");
+    return GEXS_OK;
+}
+
+// the execute passes on `pre` join `stream`: what is enqueued on `stream`
+// from here on runs after them
+int join_pre(gexs_ctx* c) {
+    if (!c->pre_pending) return GEXS_OK;
+    HIPCHK(c, hipEventRecord(c->ev_pre, c->pre));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_pre, 0));
+    c->pre_pending = false;
+    return GEXS_OK;
+}
+
+// every entry point but the execute passes: the device current, the
+// execute passes on `pre` joined into `stream`, and `pre` to wait for
+// what the caller enqueues on `stream` before its next execute pass
+int set_device(gexs_ctx* c) {
+    int rc;
+    if ((rc = set_device_only(c)) != GEXS_OK) return rc;
+    if ((rc = join_pre(c)) != GEXS_OK) return rc;
+    c->main_dirty = true;
     return GEXS_OK;
 }
 
@@ -424,6 +454,9 @@ int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
     key.n = n;
     key.gen = c->gen;
     if (key == c->pos_key) return GEXS_OK;
+    int rc;
+    if ((rc = join_pre(c)) != GEXS_OK) return rc;  // passes in flight read the old positions
+    c->main_dirty = true;
     HIPCHK(c, c->pos.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(LinePos)));
     JobDesc d{};
     d.l0 = l0;
@@ -437,8 +470,6 @@ int ensure_pos(gexs_ctx* c, const TPParams& tp, int64_t l0, int64_t n) {
                             c->pos_tp.as<TPParams>(), c->pos.as<LinePos>()));
     c->pos_key = key;
     c->items_key = PosKey();  // tiles depend on positions
-    HIPCHK(c, hipEventRecord(c->geom, c->stream));
-    c->geom_new = true;
     return GEXS_OK;
 }
 
@@ -507,13 +538,16 @@ int launch_blocks(const gexs_ctx* c) {
 // job tables (host copies, uploaded when the items are rebuilt); sorted
 // heaviest first when order == 1; with order == 2 in grid order and split
 // into GX_XCDS contiguous ranges of equal line count for the per-XCD queues.
-// Rebuilds run on c->stream and record c->geom.
+// Rebuilds run on c->stream after the execute passes in flight.
 int ensure_items(gexs_ctx* c, int njobs, const TPParams* htp, const JobDesc* hjd) {
     const int span = GX_WAVE * c->ppl;
     const PosKey key = c->pos_key;
     if (key == c->items_key && c->items_span == span && c->items_order == c->order &&
         c->items_njobs == njobs)
         return GEXS_OK;
+    int rc;
+    if ((rc = join_pre(c)) != GEXS_OK) return rc;  // passes in flight read the old items
+    c->main_dirty = true;
     const int per = tiles_per_job(c);
     const int total = per * njobs;
     HIPCHK(c, c->items.ensure((size_t)std::max(total, 1) * sizeof(WorkItem)));
@@ -562,8 +596,6 @@ int ensure_items(gexs_ctx* c, int njobs, const TPParams* htp, const JobDesc* hjd
     c->items_order = c->order;
     c->items_njobs = njobs;
     c->n_items = total;
-    HIPCHK(c, hipEventRecord(c->geom, c->stream));
-    c->geom_new = true;
     return GEXS_OK;
 }
 
@@ -583,11 +615,21 @@ int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
 }
 
 // The FP32C pass of one execute over the lines and window of c->pos_key into
-// out.  The per-line pre-pass (tables and counters, k_line_rec,
-// k_segments) runs on c->pre into one of kPassSets buffer sets, after
-// the profile kernel that last used that set; the profile kernel runs on
-// c->stream after it.  The pre-pass of the next execute thus overlaps this
-// pass's profile kernel; every pass computes exactly what it would alone.
+// out: one k_line_rec_pass launch (the pass's tables as kernel arguments,
+// its line records into buffer set k mod kPassSets, the counters of a later
+// pass's set cleared) and the profile kernel.  The line records of one pass
+// overlap the profile kernel of the pass before, in one of two modes:
+//   serial (profile kernels of kSerialKernelMs or more): line records on
+//   `pre`, clearing set k + 1's counters once the profile kernel of pass
+//   k - 3 (that set's last reader, after pass k - 4, this set's) is done
+//   (they start as pass k - 1's kernel does); profile kernels back to back
+//   on `stream`, each after its pass's line records;
+//   alternating (shorter kernels, whose passes the host's event calls would
+//   limit): pass k runs on `stream` (k even) or `pre` (k odd) and clears set
+//   k + 2's counters, so every set is used by the passes of one stream, in
+//   order, with no event per pass.
+// A change of mode first joins the two streams both ways.  Every pass
+// computes exactly what it would alone.
 int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     const int n = (int)c->pos_key.n;
     const int64_t l0 = c->pos_key.l0;
@@ -603,10 +645,21 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     d.ntiles = per;
     d.q_slot = -1;
     if ((rc = ensure_items(c, 1, &tp, &d)) != GEXS_OK) return rc;
-    gexs_ctx::PassBuf& B = c->pb[c->pbuf];
-    c->pbuf = (c->pbuf + 1) % kPassSets;
-    PassTables t;
+    // the profile-kernel time of the last execute, once its events are complete
+    if (c->prev_k1 && hipEventQuery(c->prev_k1) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->prev_k0, c->prev_k1) == hipSuccess) c->pass_kernel_ms = ms;
+    }
+    const int mode = c->pass_kernel_ms >= kSerialKernelMs ? 1 : 0;
+    const bool serial = mode == 1;
+    const int64_t k = c->npass;
+    gexs_ctx::PassBuf& B = c->pb[k % kPassSets];
+    gexs_ctx::PassBuf& N = c->pb[(k + (serial ? 1 : 2)) % kPassSets];  // counters cleared here
+    PassTables t, tn;
+    const void* small_was = B.small.p;
     if ((rc = pass_tables_in(c, B.tab, B.small, 1, &t)) != GEXS_OK) return rc;
+    if (B.small.p != small_was) B.clean = false;  // new memory
+    if ((rc = pass_tables_in(c, N.tab, N.small, 1, &tn)) != GEXS_OK) return rc;
     HIPCHK(c, B.rec.ensure((size_t)std::max(n, 1) * sizeof(VRec)));
     HIPCHK(c, B.recx.ensure((size_t)std::max(n, 1) * sizeof(VRecX)));
     HIPCHK(c, c->counter.ensure(256));
@@ -619,45 +672,64 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     jb.profile = c->profile;
     jb.scale_slot = 0;
     jb.accumulate = 1;
-    hipStream_t P = c->pre;
-    if (c->geom_new) {  // positions and tiles were rebuilt on the main stream
-        HIPCHK(c, hipStreamWaitEvent(P, c->geom, 0));
-        c->geom_new = false;
+    if (mode != c->pass_mode) {  // join the streams both ways
+        HIPCHK(c, hipEventRecord(c->ev_pre, c->pre));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_pre, 0));
+        HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->pre, c->ev_main, 0));
+        c->pre_pending = false;
+        c->main_dirty = false;
+        c->pass_mode = mode;
     }
-    if (B.used) {  // buffer set free: a device-side wait only if its last kernel still runs
-        const hipError_t q = hipEventQuery(B.done);
-        if (q == hipErrorNotReady)
-            HIPCHK(c, hipStreamWaitEvent(P, B.done, 0));
-        else if (q != hipSuccess)
-            HIPCHK(c, q);
+    hipStream_t P = (serial || (k & 1)) ? c->pre : c->stream;  // line records
+    hipStream_t S = serial ? c->stream : P;                    // profile kernel
+    if (P == c->pre && c->main_dirty) {  // lines, grids or tiles changed on `stream`
+        HIPCHK(c, hipEventRecord(c->ev_main, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(P, c->ev_main, 0));
+        c->main_dirty = false;
     }
+    if (serial) {
+        // pass k - 3's profile kernel must be done; waiting for the marker
+        // just before pass k - 1's (on `stream` after pass k - 2's) also
+        // puts these line records behind that kernel's dispatch, so their
+        // blocks take the CU slots its tail leaves free instead of slots its
+        // first workgroups need (a wait only if the marker is still ahead)
+        hipEvent_t w = c->prev_k0 ? c->prev_k0 : (N.done_recorded ? N.done : nullptr);
+        if (w) {
+            const hipError_t q = hipEventQuery(w);
+            if (q == hipErrorNotReady)
+                HIPCHK(c, hipStreamWaitEvent(P, w, 0));
+            else if (q != hipSuccess)
+                HIPCHK(c, q);
+        }
+    }
+    if (!B.clean) HIPCHK(c, gxk::fill_u64(P, (unsigned long long*)B.small.p, t.zero_u64, 0ull));
     HIPCHK(c, hipEventRecord(c->ev_start, P));
-    // the pass's tables go in as kernel arguments of the set-up kernel, which
-    // also clears the pass counters: no host staging copy per pass
-    HIPCHK(c, gxk::write_tables(P, tp, d, jb, t.tp, t.jd, t.jobs, (unsigned long long*)B.small.p,
-                                (int)t.zero_u64));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    // one block per CU: the pre-pass runs beside the profile kernels of the
-    // passes before, and few blocks get CU slots as soon as they are issued
-    HIPCHK(c, gxk::line_rec(P, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
-                            gam, nex, c->pos.as<LinePos>(), t.jd, 1, n, t.tp, t.scale, B.rec.as<VRec>(),
-                            B.recx.as<VRecX>(), t.kmax, t.neval, c->cus));
-    HIPCHK(c, hipEventRecord(B.pre_done, P));
-    hipStream_t s = c->stream;
-    HIPCHK(c, hipStreamWaitEvent(s, B.pre_done, 0));
-    if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, s));
-    HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
+    // one block per CU: the line records run beside the profile kernel of the
+    // pass before, and few blocks get CU slots as soon as they are issued
+    HIPCHK(c, gxk::line_rec_pass(P, tp, d, jb, t.jobs, (unsigned long long*)N.small.p, (int)tn.zero_u64,
+                                 c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(),
+                                 c->g.as<int32_t>(), gam, nex, c->pos.as<LinePos>(), t.scale,
+                                 B.rec.as<VRec>(), B.recx.as<VRecX>(), t.kmax, t.neval, c->cus));
+    N.clean = true;
+    if (serial) {
+        HIPCHK(c, hipEventRecord(B.pre_done, P));
+        HIPCHK(c, hipStreamWaitEvent(S, B.pre_done, 0));
+    }
+    if (timed) HIPCHK(c, hipEventRecord(c->ev_k0, S));
+    HIPCHK(c, gxk::profile_tiles(S, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
                                  t.jobs, t.scale, t.kmax, t.xq, c->counter.as<unsigned int>(),
                                  launch_mode(c), launch_blocks(c), true));
-    if (timed) {
-        HIPCHK(c, hipEventRecord(c->ev_k1, s));
-        B.done = c->ev_k1;
-    } else {
-        HIPCHK(c, hipEventRecord(B.prof_done, s));
-        B.done = B.prof_done;
-    }
-    B.used = true;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev_k1, S));
+    if (serial) HIPCHK(c, hipEventRecord(B.done, S));
+    B.done_recorded = serial;
+    c->prev_k0 = timed ? c->ev_k0 : nullptr;
+    c->prev_k1 = timed ? c->ev_k1 : nullptr;
+    B.clean = false;
+    c->npass = k + 1;
+    if (S == c->pre) c->pre_pending = true;  // serial: `stream` waited for `pre`
     c->last.n_tiles = c->n_items;
     c->last_neval = t.neval;
     c->last_items = c->items.as<WorkItem>();
@@ -715,11 +787,13 @@ int gexs_create(int device, int profile, gexs_ctx** out) {
     bool ok = set_device(c) == GEXS_OK &&
               hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&c->pre, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&c->geom, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming) == hipSuccess &&
+
               next_events(c) == GEXS_OK;
     for (auto& b : c->pb)
         ok = ok && hipEventCreateWithFlags(&b.pre_done, hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&b.prof_done, hipEventDisableTiming) == hipSuccess;
+             hipEventCreateWithFlags(&b.done, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         gexs_destroy(c);
         return GEXS_EHIP;
@@ -770,11 +844,12 @@ int gexs_destroy(gexs_ctx* c) {
     if (c->h_tab) (void)hipHostFree(c->h_tab);
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->pre) (void)hipStreamSynchronize(c->pre);
+    if (c->ev_pre) (void)hipEventDestroy(c->ev_pre);
+    if (c->ev_main) (void)hipEventDestroy(c->ev_main);
     for (auto& b : c->pb) {
         if (b.pre_done) (void)hipEventDestroy(b.pre_done);
-        if (b.prof_done) (void)hipEventDestroy(b.prof_done);
+        if (b.done) (void)hipEventDestroy(b.done);
     }
-    if (c->geom) (void)hipEventDestroy(c->geom);
     if (c->pre) (void)hipStreamDestroy(c->pre);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -895,7 +970,9 @@ int gexs_execute(gexs_ctx* c, int npoints, int64_t nlines, int start_idx, double
         return c->fail(GEXS_EINVAL, "window outside the grid");
     if (!(T > 0.0) || !(Q > 0.0)) return c->fail(GEXS_EINVAL, "T and Q must be positive");
     int rc;
-    if ((rc = set_device(c)) != GEXS_OK) return rc;
+    // the FP32C passes do not join the streams (run_fp32c); the FP64 pass runs
+    // on `stream` after them
+    if ((rc = (c->k.precision == GEXS_FP64 ? set_device(c) : set_device_only(c))) != GEXS_OK) return rc;
     double* out = c->results[(size_t)c->tasks];
     c->tasks++;
     c->last = gexs_stats{};
@@ -910,7 +987,7 @@ int gexs_execute(gexs_ctx* c, int npoints, int64_t nlines, int start_idx, double
         HIPCHK(c, hipEventRecord(c->ev_start, c->stream));
         rc = run_fp64(c, tp, out, true);
     } else {
-        rc = run_fp32c(c, tp, out, true);  // records ev_start on the pre-pass stream
+        rc = run_fp32c(c, tp, out, true);
     }
     c->last_neval_pending = true;
     return rc;
@@ -935,12 +1012,13 @@ int gexs_transfer_results(gexs_ctx* c, double* out, int n, int cs_index, int acc
 
 int gexs_ready(gexs_ctx* c) {
     if (!c) return GEXS_EINVAL;
-    return hipStreamQuery(c->stream) == hipSuccess ? 1 : 0;
+    return (hipStreamQuery(c->stream) == hipSuccess && hipStreamQuery(c->pre) == hipSuccess) ? 1 : 0;
 }
 
 int gexs_sync(gexs_ctx* c) {
     if (!c) return GEXS_EINVAL;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->pre));
     return GEXS_OK;
 }
 
@@ -950,6 +1028,7 @@ int gexs_last_stats(gexs_ctx* c, gexs_stats* out) {
     if (!c || !out) return GEXS_EINVAL;
     if (c->last_neval_pending) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->pre));  // the last execute pass may have run there
         unsigned long long ne = 0;
         if (c->last_neval) HIPCHK(c, hipMemcpy(&ne, c->last_neval, 8, hipMemcpyDeviceToHost));
         c->last.n_eval = (int64_t)ne;

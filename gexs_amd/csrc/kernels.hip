@@ -214,22 +214,19 @@ __device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double
     return amp > 0.0 ? ex : 0;
 }
 
-__global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
-                                                  const double* __restrict__ E,
-                                                  const double* __restrict__ A,
-                                                  const int* __restrict__ g,
-                                                  const double* __restrict__ gamma,
-                                                  const double* __restrict__ nexp,
-                                                  const LinePos* __restrict__ pos,
-                                                  const JobDesc* __restrict__ jd,
-                                                  const TPParams* __restrict__ tp,
-                                                  unsigned long long* __restrict__ emax,
-                                                  VRec* __restrict__ rec, VRecX* __restrict__ recx,
-                                                  int* __restrict__ kmax,
-                                                  unsigned long long* __restrict__ neval) {
-    const int k = blockIdx.y;
-    const TPParams p = tp[k];
-    const JobDesc d = jd[k];
+// the records of job k's lines (the blocks of one grid row share them)
+__device__ __forceinline__ void line_rec_job(const TPParams& p, const JobDesc& d, int k,
+                                             const double* __restrict__ nu,
+                                             const double* __restrict__ E,
+                                             const double* __restrict__ A,
+                                             const int* __restrict__ g,
+                                             const double* __restrict__ gamma,
+                                             const double* __restrict__ nexp,
+                                             const LinePos* __restrict__ pos,
+                                             unsigned long long* __restrict__ emax,
+                                             VRec* __restrict__ rec, VRecX* __restrict__ recx,
+                                             int* __restrict__ kmax,
+                                             unsigned long long* __restrict__ neval) {
     const bool voigt = p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD;
     long long ne = 0;
     int km = 0, kf = 0;
@@ -265,6 +262,49 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
         if (kmax != nullptr && kf > 0) atomicMax(&kmax[2 * k + 1], kf);
         if (em > 0) atomicMax(&emax[k], (unsigned long long)em);
     }
+}
+
+__global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
+                                                  const double* __restrict__ E,
+                                                  const double* __restrict__ A,
+                                                  const int* __restrict__ g,
+                                                  const double* __restrict__ gamma,
+                                                  const double* __restrict__ nexp,
+                                                  const LinePos* __restrict__ pos,
+                                                  const JobDesc* __restrict__ jd,
+                                                  const TPParams* __restrict__ tp,
+                                                  unsigned long long* __restrict__ emax,
+                                                  VRec* __restrict__ rec, VRecX* __restrict__ recx,
+                                                  int* __restrict__ kmax,
+                                                  unsigned long long* __restrict__ neval) {
+    const int k = blockIdx.y;
+    line_rec_job(tp[k], jd[k], k, nu, E, A, g, gamma, nexp, pos, emax, rec, recx, kmax, neval);
+}
+
+// One pass of one job with its tables as kernel arguments: block 0 also
+// stores the Job the profile kernel reads (djob) and clears clear[0 ..
+// nclear), the counters of a later pass (the pass's own counters are zero
+// before it starts).
+__global__ __launch_bounds__(256) void k_line_rec_pass(TPParams p, JobDesc d, Job jb,
+                                                       Job* __restrict__ djob,
+                                                       unsigned long long* __restrict__ clear,
+                                                       int nclear, const double* __restrict__ nu,
+                                                       const double* __restrict__ E,
+                                                       const double* __restrict__ A,
+                                                       const int* __restrict__ g,
+                                                       const double* __restrict__ gamma,
+                                                       const double* __restrict__ nexp,
+                                                       const LinePos* __restrict__ pos,
+                                                       unsigned long long* __restrict__ emax,
+                                                       VRec* __restrict__ rec,
+                                                       VRecX* __restrict__ recx,
+                                                       int* __restrict__ kmax,
+                                                       unsigned long long* __restrict__ neval) {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) *djob = jb;
+        for (int i = threadIdx.x; i < nclear; i += blockDim.x) clear[i] = 0ull;
+    }
+    line_rec_job(p, d, 0, nu, E, A, g, gamma, nexp, pos, emax, rec, recx, kmax, neval);
 }
 
 // ---------------------------------------------------------------- k_tiles
@@ -571,6 +611,19 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
     int gx = std::min(blocks_for(max_n, 256), max_blocks > 0 ? max_blocks : 2048);
     hipLaunchKernelGGL(k_line_rec, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, gamma, nexp, pos,
                        jd, tp, emax, rec, recx, kmax, neval);
+    return hipGetLastError();
+}
+
+hipError_t line_rec_pass(hipStream_t s, const TPParams& tp, const JobDesc& jd, const Job& jb,
+                         Job* djob, unsigned long long* clear, int nclear, const double* nu,
+                         const double* E, const double* A, const int* g, const double* gamma,
+                         const double* nexp, const LinePos* pos, unsigned long long* emax,
+                         VRec* rec, VRecX* recx, int* kmax, unsigned long long* neval,
+                         int max_blocks) {
+    const int gx = std::max(1, std::min(blocks_for(std::max<int64_t>(jd.n, 1), 256),
+                                        max_blocks > 0 ? max_blocks : 2048));
+    hipLaunchKernelGGL(k_line_rec_pass, dim3(gx), dim3(256), 0, s, tp, jd, jb, djob, clear, nclear,
+                       nu, E, A, g, gamma, nexp, pos, emax, rec, recx, kmax, neval);
     return hipGetLastError();
 }
 

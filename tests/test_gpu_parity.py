@@ -367,6 +367,46 @@ def test_window_subset_and_chunk_accumulation(O, c1_small):
     check(out1, ref1, 1e-4, 1e-6, "window subset")
 
 
+def test_execute_passes_equal_single_executes(O, c1_small):
+    """Back-to-back executes alternate between the two streams (odd passes on
+    the second), each with its own buffer set; other calls in between join
+    the streams, and a window change rebuilds the positions while a pass may
+    still run.  Every pass must equal the same execute run alone, bit for
+    bit: eight passes at two temperatures, an item inspection (a kernel on
+    the main stream) after the third, a stats read after the fifth and a
+    window subset at the seventh."""
+    L, (nu, E, A, g, gam, nex) = c1_small
+    P, N = 1.0, 200_000
+    dnu = 10000.0 / N
+    Ts = [296.0, 700.0] * 4
+    Qs = {T: float(O.partition(L.E, L.g, T)) for T in set(Ts)}
+    wins = [(N, 0)] * 6 + [(60_000, 70_000), (N, 0)]
+    m = HipManager("voigt")
+    m.InitializeVectors(N, len(Ts))
+    m.InitializeConstants(100.0, Ts[0], Qs[Ts[0]], dnu, 18.0153, P, 296.0, P_SCALE, cutoff=25.0)
+    freq = np.arange(N) * dnu
+    m.TransferFreq(freq)
+    m.TransferVectors(nu.shape[0], E, nu, A, g, gam, nex)
+    for k, T in enumerate(Ts):
+        nwin, s0 = wins[k]
+        m.ExecuteCrossSection(nwin, nu.shape[0], s0, T, P, Qs[T])
+        if k == 2:
+            assert m.pass_items().shape[0] > 0  # a kernel on the main stream
+        if k == 4:
+            m.stats()
+    outs = []
+    for k in range(len(Ts)):
+        o = np.zeros(N)
+        m.TransferResults(freq, o, N, k)
+        outs.append(o)
+    for k, T in enumerate(Ts):
+        nwin, s0 = wins[k]
+        alone, _, _ = run_manager("voigt", (nu, E, A, g, gam, nex), T, P, Qs[T], 0.0, 10000.0, N,
+                                  25.0, start_idx=s0, nwin=nwin)
+        assert np.array_equal(outs[k], alone), (k, T)
+    log(f"  {len(Ts)} executes equal single executes bit for bit")
+
+
 def test_edge_cases(O):
     T, P, N = 296.0, 1.0, 1000
     start, end = 100.0, 110.0
