@@ -707,12 +707,15 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, hipEventRecord(c->ev_start, P));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    // one block per CU: the line records run beside the profile kernel of the
-    // pass before, and few blocks get CU slots as soon as they are issued
+    // few blocks, so the line records run beside the profile kernel of the
+    // pass before without taking its slots: one per CU beside a long kernel
+    // (they fill its tail), half that beside a short one (c0 step 33 -> 30 us
+    // against one per CU; 40 us with two per CU)
     HIPCHK(c, gxk::line_rec_pass(P, tp, d, jb, t.jobs, (unsigned long long*)N.small.p, (int)tn.zero_u64,
                                  c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(),
                                  c->g.as<int32_t>(), gam, nex, c->pos.as<LinePos>(), t.scale,
-                                 B.rec.as<VRec>(), B.recx.as<VRecX>(), t.kmax, t.neval, c->cus));
+                                 B.rec.as<VRec>(), B.recx.as<VRecX>(), t.kmax, t.neval,
+                                 serial ? c->cus : std::max(1, c->cus / 2)));
     N.clean = true;
     if (serial) {
         HIPCHK(c, hipEventRecord(B.pre_done, P));
