@@ -1112,16 +1112,16 @@ int gexs_pass_items(gexs_ctx* c, int64_t cap, int32_t* out, int64_t* n) {
     // the same search over the same kmax, into the items' segment fields
     if (c->last_kmax) {
         HIPCHK(c, gxk::segments(c->stream, const_cast<WorkItem*>(c->last_items), (int)m,
-                                c->pos.as<LinePos>(), c->last_kmax, c->profile));
+                                c->pos.as<LinePos>(), c->last_kmax, c->profile, c->items_span));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     std::vector<WorkItem> h((size_t)m);
     HIPCHK(c, hipMemcpy(h.data(), c->last_items, (size_t)m * sizeof(WorkItem), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < m; ++i) {
         const WorkItem& w = h[(size_t)i];
-        const int32_t v[12] = {w.lo, w.hi, w.w0, w.w1, w.s1, w.s2,    w.s3,
-                               w.s4, w.job, w.group, w.f2, w.f3};
-        std::memcpy(out + 12 * i, v, sizeof v);
+        const int32_t v[14] = {w.lo, w.hi,    w.w0, w.w1, w.s1, w.s2, w.s3,
+                               w.s4, w.job, w.group, w.f2, w.f3, w.e2, w.e3};
+        std::memcpy(out + 14 * i, v, sizeof v);
     }
     return GEXS_OK;
 }

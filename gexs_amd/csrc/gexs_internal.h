@@ -74,6 +74,9 @@ struct __attribute__((aligned(8))) VRecX {
 //   tile (Kl / (s - 1), no checks), [f2, s2) region I over the whole tile (no
 //   checks), [s2, s3) line centres near the tile (checked), [s3, f3) region
 //   I, [f3, s4) far wings, [s4, hi) window starts inside the tile.
+//   Of the far wings, [s1, e2) and [e3, s4) lie four half-tiles or more from
+//   the tile's centre: their sum over the tile is a polynomial in the point
+//   offset (the far-field expansion), summed per line instead of per point.
 // s1 = s4 = hi when no line covers the whole tile (every line checked).
 // bin-Gauss: [f2, f3) are the lines of [lo, hi) centred within reach of the
 // tile (the job's largest reach); s2 = s3 = s4 and the kernel sums
@@ -89,7 +92,9 @@ struct __attribute__((aligned(16))) WorkItem {
     int32_t s1, s2, s3, s4;
     int32_t group, ngroups, slot;
     int32_t f2, f3;
+    int32_t e2, e3;
 };
+static_assert(sizeof(WorkItem) == 64, "WorkItem is one 64-byte record");
 
 // per-(T,P) physics of one job, FP64
 struct TPParams {

@@ -33,12 +33,13 @@ hipError_t line_rec_pass(hipStream_t s, const TPParams& tp, const JobDesc& jd, c
                          const double* nexp, const LinePos* pos, unsigned long long* emax,
                          VRec* rec, VRecX* recx, int* kmax, unsigned long long* neval,
                          int max_blocks = 0);
-// the segment bounds the profile kernel locates for every work item from
-// kmax (f2, s2, s3, f3; the bin-Gauss profiles f2, f3 only, the lines within
+// the segment bounds the profile kernel locates for every work item of tile
+// width span from kmax (f2, s2, s3, f3, e2, e3; the bin-Gauss profiles f2,
+// f3 only, the lines within
 // reach of the tile), stored into the items: for inspection, the profile
 // kernel does not read them.  pos: the lines of every job (pos_off 0).
 hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const LinePos* pos,
-                    const int* kmax, int profile);
+                    const int* kmax, int profile, int span);
 // jobs job0 .. job0 + njobs - 1 of jd / tp, all with tiles of span points
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,

@@ -369,6 +369,8 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     const bool gauss = tp[job].profile == GX_PROF_DOPPLER || tp[job].profile == GX_PROF_GAUSSIAN;
     it.f2 = gauss ? lo : it.s1;  // bin-Gauss: every line within reach until then
     it.f3 = gauss ? hi : it.s4;
+    it.e2 = it.s1;
+    it.e3 = it.s4;
     const int G = max(d.groups, 1);
     it.ngroups = G;
     for (int g = 0; g < G; ++g) {
@@ -390,15 +392,16 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
 // every job (pos_off 0).
 __global__ __launch_bounds__(256) void k_segments(WorkItem* items, int nitems,
                                                   const LinePos* __restrict__ pos,
-                                                  const int* __restrict__ kmax, bool gauss) {
+                                                  const int* __restrict__ kmax, bool gauss,
+                                                  int half) {
     const int t = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & (GX_WAVE - 1);
     if (t >= nitems) return;
     WorkItem it = items[t];
     if (gauss)
-        locate_segments<true>(it, pos, kmax, lane);
+        locate_segments<true>(it, pos, kmax, half, lane);
     else
-        locate_segments<false>(it, pos, kmax, lane);
+        locate_segments<false>(it, pos, kmax, half, lane);
     if (lane == 0) items[t] = it;
 }
 
@@ -628,11 +631,11 @@ hipError_t line_rec_pass(hipStream_t s, const TPParams& tp, const JobDesc& jd, c
 }
 
 hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const LinePos* pos,
-                    const int* kmax, int profile) {
+                    const int* kmax, int profile, int span) {
     if (nitems <= 0) return hipSuccess;
     const bool gauss = profile != GX_PROF_VOIGT && profile != GX_PROF_VOIGT_QUAD;
     hipLaunchKernelGGL(k_segments, dim3(blocks_for(nitems, 4)), dim3(256), 0, s, items, nitems, pos,
-                       kmax, gauss);
+                       kmax, gauss, span / 2);
     return hipGetLastError();
 }
 

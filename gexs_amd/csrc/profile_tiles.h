@@ -82,13 +82,17 @@ __device__ __forceinline__ void wave_first_c(const LinePos* jp, int a0, int b0,
 //   Voigt profiles: among the lines [s1, s4) that cover the whole tile, f2,
 //   s2, s3 and f3 are the first with c >= w0 - KF, w0 - K, w1 + K, w1 + KF:
 //   [s2, s3) may have points outside region I, [s1, f2) and [f3, s4) take
-//   the one-term wing form at every point.
+//   the one-term wing form at every point.  e2 and e3 are the first with
+//   c >= w0 - max(KF, 3T + 2) and c >= max(w1 + KF, w0 + 5T + 2), T the
+//   tile's half width: [s1, e2) and [e3, s4) are far wings whose centre lies
+//   at least 4T + 2.5 points from the tile's centre w0 + T - 1/2 (the
+//   far-field expansion's lines).
 //   bin-Gauss: [f2, f3) are the lines of [lo, hi) with c in [w0 - KF,
 //   w1 + KF), the only ones that add to the tile.
 // Every lane of the wave gets the same values.
 template <bool GAUSS>
 __device__ __forceinline__ void locate_segments(WorkItem& it, const LinePos* jp, const int* kmax,
-                                                int lane) {
+                                                int half, int lane) {
     const long long KF = kmax[2 * it.job + 1];
     if constexpr (GAUSS) {
         const long long v[2] = {(long long)it.w0 - KF, (long long)it.w1 + KF};
@@ -98,14 +102,18 @@ __device__ __forceinline__ void locate_segments(WorkItem& it, const LinePos* jp,
         it.f3 = r[1];
     } else {
         const long long K = kmax[2 * it.job];
-        const long long v[4] = {(long long)it.w0 - KF, (long long)it.w0 - K, (long long)it.w1 + K,
-                                (long long)it.w1 + KF};
-        int r[4];
-        wave_first_c<4>(jp, it.s1, max(it.s4, it.s1), v, r, lane);
+        const long long T = half;
+        const long long v[6] = {(long long)it.w0 - KF, (long long)it.w0 - K, (long long)it.w1 + K,
+                                (long long)it.w1 + KF, (long long)it.w0 - max(KF, 3 * T + 2),
+                                max((long long)it.w1 + KF, (long long)it.w0 + 5 * T + 2)};
+        int r[6];
+        wave_first_c<6>(jp, it.s1, max(it.s4, it.s1), v, r, lane);
         it.f2 = r[0];
         it.s2 = r[1];
         it.s3 = r[2];
         it.f3 = r[3];
+        it.e2 = r[4];
+        it.e3 = r[5];
     }
 }
 
@@ -306,6 +314,88 @@ __device__ __forceinline__ void voigt_staged(TileAcc<PPL>& t, GlobRec grec, Cons
         }
         t.fold();
     }
+}
+
+// Far-field expansion of the one-term wing form over a tile of half width T
+// (T = 32 PPL).  A far line's value at point j, Kl / (x_j^2 + D), with
+// x_j = Xc - A T tau (Xc its x at the tile's centre t_c = w0 + T - 1/2,
+// tau = (j - t_c) / T in [-1, 1], D = E2 - 1), is the power series
+// sum_k h_k tau^k with, for den = Xc^2 + D,
+//   h_0 = Kl / den,  h_1 = a h_0,  h_k = a h_(k-1) + b h_(k-2),
+//   a = 2 Xc A T / den,  b = -(A T)^2 / den
+// (multiply the series by den - 2 Xc A T tau + (A T tau)^2 = x^2 + D and
+// compare terms).  The expansion's lines lie 4T + 2.5 points or more from
+// t_c, so A T / |Xc| < 1/4 (|B| <= A / 2); they are far lines, whose x^2
+// exceeds 2800 at every point, so |D| <= 1.5 moves the series' poles by
+// under 2.3 %; the terms beyond kFarTerms then add under 17 (0.256)^16 /
+// 0.744^2 = 1.1e-8 of h_0.  Each lane sums the coefficients of one line per
+// batch of 64 (67 VALU per batch: about one per line per wave, against 4 per
+// line per 64-point slice for the one-term form); far_add adds the lanes'
+// sums across the wave and evaluates the polynomial at each point.
+constexpr int kFarTerms = 16;
+
+template <int PPL>
+__device__ __forceinline__ void far_expand(float (&C)[kFarTerms], GlobRec grec, int a, int b,
+                                           float tcf, int E, int lane) {
+    constexpr float T = (float)(GX_WAVE * PPL / 2);
+    if (a >= b) return;
+    VRec vn = grec[min(a + lane, b - 1)];
+    for (int base = a; base < b; base += GX_WAVE) {
+        const VRec v = vn;
+        vn = grec[min(base + GX_WAVE + lane, b - 1)];
+        const float Kl = base + lane < b ? ldexpf(v.Kl, v.ex - E) : 0.0f;
+        const float Xc = fmaf(v.cf - tcf, v.A, v.B);
+        const float inv = __builtin_amdgcn_rcpf(fmaf(Xc, Xc, v.E2 - 1.0f));
+        const float u = v.A * T;
+        const float al = (2.0f * u) * (Xc * inv);
+        const float be = -(u * u) * inv;
+        float h0 = Kl * inv;
+        float h1 = al * h0;
+        C[0] += h0;
+        C[1] += h1;
+#pragma unroll
+        for (int k = 2; k < kFarTerms; ++k) {
+            const float h2 = fmaf(al, h1, be * h0);
+            C[k] += h2;
+            h0 = h1;
+            h1 = h2;
+        }
+    }
+}
+
+// the wave's coefficient sums (an xor butterfly: every lane adds the same
+// pairs, so all hold the same totals) evaluated at the lane's points and
+// added to their sums
+template <int PPL>
+__device__ __forceinline__ void far_add(TileAcc<PPL>& t, float (&C)[kFarTerms], float tcf) {
+    constexpr float invT = 1.0f / (float)(GX_WAVE * PPL / 2);
+#pragma unroll
+    for (int k = 0; k < kFarTerms; ++k) {
+#pragma unroll
+        for (int m = GX_WAVE / 2; m >= 1; m >>= 1) C[k] += __shfl_xor(C[k], m);
+    }
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        const float tau = (t.jf[q] - tcf) * invT;  // exact
+        float p = C[kFarTerms - 1];
+#pragma unroll
+        for (int k = kFarTerms - 2; k >= 0; --k) p = fmaf(p, tau, C[k]);
+        kahan_add(t.sum[q], t.cmp[q], p);
+    }
+}
+
+// the far-field expansion of the lines [a, b) and [c, d) added to the tile
+template <int PPL>
+__device__ __forceinline__ void far_lines(TileAcc<PPL>& t, GlobRec grec, int a, int b, int c, int d,
+                                          int w0, int lane) {
+    if (a >= b && c >= d) return;
+    const float tcf = (float)w0 + (float)(GX_WAVE * PPL / 2) - 0.5f;  // exact below 2^23
+    float C[kFarTerms];
+#pragma unroll
+    for (int k = 0; k < kFarTerms; ++k) C[k] = 0.0f;
+    far_expand<PPL>(C, grec, a, b, tcf, t.E, lane);
+    far_expand<PPL>(C, grec, c, d, tcf, t.E, lane);
+    far_add<PPL>(t, C, tcf);
 }
 
 // Points outside the profile kernel's region I, and their value.  Voigt:
@@ -762,6 +852,7 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             // the region-I and far segments through the staged loop (6 or 4
             // VALU per line against 14 with broadcasts), the rest batched
             const int f2 = min(max(it.f2, ca), cb), f3 = min(max(it.f3, ca), cb);
+            const int e2 = min(max(it.e2, ca), cb), e3 = min(max(it.e3, ca), cb);
             const ConstPos pos = (ConstPos)jb.pos;
             if (it.f2 > it.s1)
                 voigt_staged<1, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
@@ -769,17 +860,19 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
                 voigt_staged<1, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskHi>(t, gpos, grec, grecx, ca, s1, w0, w1, lane, incl);
-            voigt_staged<1, kMaskNone, true>(t, grec, pos, s1, f2, w0, lane, xs);
+            voigt_staged<1, kMaskNone, true>(t, grec, pos, e2, f2, w0, lane, xs);
             voigt_staged<1, kMaskNone>(t, grec, pos, f2, s2, w0, lane, xs);
             lines_batched<PROF, kMaskNone>(t, gpos, grec, grecx, s2, s3, w0, w1, lane, incl);
             voigt_staged<1, kMaskNone>(t, grec, pos, s3, f3, w0, lane, xs);
-            voigt_staged<1, kMaskNone, true>(t, grec, pos, f3, s4, w0, lane, xs);
+            voigt_staged<1, kMaskNone, true>(t, grec, pos, f3, e3, w0, lane, xs);
             if (it.f3 < it.s4)
                 voigt_staged<1, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
             else if (it.s3 < it.s4)
                 voigt_staged<1, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskLo>(t, gpos, grec, grecx, s4, cb, w0, w1, lane, incl);
+            t.fold();
+            far_lines<1>(t, grec, s1, e2, e3, s4, w0, lane);
         } else {
             lines_batched<PROF, kMaskBoth>(t, gpos, grec, grecx, ca, cb, w0, w1, lane, incl);
         }
@@ -794,6 +887,7 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             // are all of [lo, s1); when the last one is far above (s3 < s4,
             // f3 < s4) so are all of [s4, hi)
             const int f2 = min(max(it.f2, ca), cb), f3 = min(max(it.f3, ca), cb);
+            const int e2 = min(max(it.e2, ca), cb), e3 = min(max(it.e3, ca), cb);
             const GlobRec grec = (GlobRec)jb.rec;
             if (it.f2 > it.s1)
                 voigt_staged<PPL, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
@@ -801,18 +895,20 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
                 voigt_staged<PPL, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
-            voigt_staged<PPL, kMaskNone, true>(t, grec, pos, s1, f2, w0, lane, xs);
+            voigt_staged<PPL, kMaskNone, true>(t, grec, pos, e2, f2, w0, lane, xs);
             voigt_staged<PPL, kMaskNone>(t, grec, pos, f2, s2, w0, lane, xs);
             near_staged<PROF, PPL>(t, (GlobPos)jb.pos, grec, (GlobRecX)jb.recx, s2, s3, w0, w1, lane,
                                    xs);
             voigt_staged<PPL, kMaskNone>(t, grec, pos, s3, f3, w0, lane, xs);
-            voigt_staged<PPL, kMaskNone, true>(t, grec, pos, f3, s4, w0, lane, xs);
+            voigt_staged<PPL, kMaskNone, true>(t, grec, pos, f3, e3, w0, lane, xs);
             if (it.f3 < it.s4)
                 voigt_staged<PPL, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
             else if (it.s3 < it.s4)
                 voigt_staged<PPL, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
+            t.fold();
+            far_lines<PPL>(t, grec, s1, e2, e3, s4, w0, lane);
         } else {
             lines_checked<PROF, PPL, kMaskBoth>(t, pos, rec, recx, ca, cb, w0, w1);
         }
@@ -981,13 +1077,13 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             if (idx >= nitems) return;
             WorkItem it = items[idx];
             const Job jb = jobs[single ? 0 : it.job];
-            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, lane);
+            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, GX_WAVE * PPL / 2, lane);
             run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
         } else {
             if (lb >= nitems) return;
             WorkItem it = items[lb];
             const Job jb = jobs[single ? 0 : it.job];
-            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, lane);
+            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, GX_WAVE * PPL / 2, lane);
             run_item_split<PROF, PPL, SPLIT>(it, jb, scale_bits, lane, wv, red, xs);
         }
         return;
@@ -1013,7 +1109,7 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             if (idx >= end) break;
             WorkItem it = items[idx];
             const Job jb = jobs[single ? 0 : it.job];
-            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, lane);
+            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, GX_WAVE * PPL / 2, lane);
             if (SPLIT == 1)
                 run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
             else

@@ -153,12 +153,12 @@ class HipManager:
         return nu, ib, ie
 
     def pass_items(self):
-        """Work items of the last FP32C pass as an int32 array (n, 12): lo, hi,
-        w0, w1, s1, s2, s3, s4, job, group, f2, f3 (line range, grid points,
-        segments)."""
+        """Work items of the last FP32C pass as an int32 array (n, 14): lo, hi,
+        w0, w1, s1, s2, s3, s4, job, group, f2, f3, e2, e3 (line range, grid
+        points, segments)."""
         n = C.c_int64()
         self._chk(self._L.gexs_pass_items(self._h, 0, None, C.byref(n)))
-        out = np.empty((n.value, 12), np.int32)
+        out = np.empty((n.value, 14), np.int32)
         if n.value:
             self._chk(self._L.gexs_pass_items(self._h, n.value, _ptr(out), C.byref(n)))
         return out

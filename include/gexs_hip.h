@@ -143,16 +143,19 @@ int gexs_line_windows(gexs_ctx* ctx, int64_t n, double* nu_sorted, int32_t* ib, 
  * lines with start - half_width <= nu <= end + half_width,
  * Profiles/BaseProfile.cpp:152-157). */
 int gexs_pass_lines(gexs_ctx* ctx, int64_t* first, int64_t* count);
-/* The work items (tiles) the last FP32C profile kernel read, 12 int32 each:
- * {lo, hi, w0, w1, s1, s2, s3, s4, job, group, f2, f3} -- the tile's line
- * range [lo, hi) among the pass's lines, its grid points [w0, w1) and the
- * segment bounds: [lo, s1) windows ending inside the tile, [s1, s4) windows
- * covering it, of which [s2, s3) are centred near it (Humlicek regions II-IV
- * possible) and [s1, f2), [f3, s4) so far from it that the one-term wing
- * form applies, [s4, hi) windows starting inside it.  *n receives the item
- * count (0 after an FP64 pass); out needs cap >= *n rows (cap 0: count only).
- * Waits for the ctx stream.  No counterpart in the reference: it lets a
- * caller see which line shares a launch shape gave each wave. */
+/* The work items (tiles) the last FP32C profile kernel read, 14 int32 each:
+ * {lo, hi, w0, w1, s1, s2, s3, s4, job, group, f2, f3, e2, e3} -- the tile's
+ * line range [lo, hi) among the pass's lines, its grid points [w0, w1) and
+ * the segment bounds the kernel located: [lo, s1) windows ending inside the
+ * tile, [s1, s4) windows covering it, of which [s2, s3) are centred near it
+ * (Humlicek regions II-IV possible), [s1, f2) and [f3, s4) so far from it
+ * that the one-term wing form applies, and of those [s1, e2) and [e3, s4)
+ * summed by the far-field expansion; [s4, hi) windows starting inside it.
+ * Bin-averaged Gaussian: [f2, f3) are the lines within reach of the tile.
+ * *n receives the item count (0 after an FP64 pass); out needs cap >= *n
+ * rows (cap 0: count only).  Waits for the ctx streams.  No counterpart in
+ * the reference: it lets a caller see which line shares a launch shape gave
+ * each wave. */
 int gexs_pass_items(gexs_ctx* ctx, int64_t cap, int32_t* out, int64_t* n);
 
 /* Tuning of the FP32C profile kernel: points per lane (1, 2, 4 or 8; a

@@ -173,3 +173,72 @@ def test_bin_gauss_reach_adds_exact_zeros():
     # without the half bin the bound fails
     K0 = np.ceil((8.0 + np.abs(Bf)) / Af).astype(np.int64)
     assert not (np.all(same_clamp(c - K0)) and np.all(same_clamp(c + K0)))
+
+
+def far_expansion_f32(Kl, A, B, E2, d, T, nterms=16, flip=False):
+    """The profile kernel's far-field expansion in FP32 (far_expand /
+    far_add in gexs_amd/csrc/profile_tiles.h): the line's coefficients of
+    Kl / ((Xc - A T tau)^2 + E2 - 1) by the two-term recurrence, Xc its x at
+    the tile's centre, d = c - t_c (a half-integer)."""
+    f = np.float32
+    Kl, A, B, E2 = (np.asarray(v, f) for v in (Kl, A, B, E2))
+    Xc = fma32(np.asarray(d, f), A, B)
+    inv = (f(1) / fma32(Xc, Xc, (E2 - f(1)).astype(f))).astype(f)
+    u = (A * f(T)).astype(f)
+    al = ((f(2) * u).astype(f) * (Xc * inv).astype(f)).astype(f)
+    be = (-(u * u).astype(f) * inv).astype(f)
+    if flip:
+        be = -be
+    h = [(Kl * inv).astype(f)]
+    h.append((al * h[0]).astype(f))
+    for k in range(2, nterms):
+        h.append(fma32(al, h[-1], (be * h[-2]).astype(f)))
+    return np.stack(h, axis=1)
+
+
+def horner_f32(C, tau):
+    f = np.float32
+    p = np.broadcast_to(C[..., -1:], C.shape[:-1] + (tau.shape[-1],)).astype(f)
+    for k in range(C.shape[-1] - 2, -1, -1):
+        p = fma32(p, tau.astype(f), np.broadcast_to(C[..., k:k + 1], p.shape))
+    return p
+
+
+def test_far_field_expansion_matches_the_one_term_form():
+    """The far-field expansion the Voigt tiles use for far lines 4T + 2.5
+    points or more from a tile's centre (T its half width): evaluated in
+    FP32 as the kernel does, its value at every point of the tile is within
+    5e-7 of the one-term form Kl / (x^2 + E2 - 1) computed exactly, over
+    widths, offsets, Lorentz ratios y from 1e-2 to 1e2 and distances up to
+    5000 points, at every tile width (the kernel's direct FP32 evaluation of
+    the same form reaches 4.6e-7 on these lines: the FP32 rounding of the
+    record's A, B and Kl); with the recurrence's second coefficient negated,
+    or 10 terms instead of 16 (truncation up to 11 (0.256)^10 / 0.744^2 =
+    2.4e-5), it is not."""
+    rng = np.random.default_rng(9)
+    for T in (32, 64, 128, 256):
+        m = 20000
+        A = 10 ** rng.uniform(-2.0, 1.0, m)
+        B = rng.uniform(-0.5, 0.5, m) * A
+        y = 10 ** rng.uniform(-2.0, 2.0, m)
+        E2 = y * y - 0.5
+        side = rng.choice([-1.0, 1.0], m)
+        d = side * (4 * T + 2.5 + np.floor(rng.uniform(0, 1, m) ** 3 * 5000))
+        tau = (np.arange(2 * T) - T + 0.5) / T
+        x = (d[:, None] - T * tau[None, :]) * A[:, None] + B[:, None]
+        s = x * x + E2[:, None]
+        far = np.all(s * s >= (2 * y[:, None] ** 2 + 1) * 8388608.0, axis=1)
+        sel = far
+        assert np.sum(sel) > m // 4
+        Kl = rng.uniform(0.5, 1.0, m)
+        exact = Kl[sel, None] / (x[sel] ** 2 + (E2[sel, None] - 1.0))
+        C = far_expansion_f32(Kl[sel], A[sel], B[sel], E2[sel], d[sel], T)
+        got = horner_f32(C, np.broadcast_to(tau, (np.sum(sel), 2 * T))).astype(np.float64)
+        err = np.max(np.abs(got - exact) / exact)
+        assert err < 5e-7, (T, err)
+        bad = horner_f32(far_expansion_f32(Kl[sel], A[sel], B[sel], E2[sel], d[sel], T, flip=True),
+                         np.broadcast_to(tau, (np.sum(sel), 2 * T))).astype(np.float64)
+        assert np.max(np.abs(bad - exact) / exact) > 1e-3
+        few = horner_f32(far_expansion_f32(Kl[sel], A[sel], B[sel], E2[sel], d[sel], T, nterms=10),
+                         np.broadcast_to(tau, (np.sum(sel), 2 * T))).astype(np.float64)
+        assert np.max(np.abs(few - exact) / exact) > 5e-6

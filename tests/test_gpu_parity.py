@@ -407,6 +407,33 @@ def test_execute_passes_equal_single_executes(O, c1_small):
     log(f"  {len(Ts)} executes equal single executes bit for bit")
 
 
+@pytest.mark.parametrize("ppl,split,order", [(4, 4, 3), (1, 1, 1), (2, 16, 1)])
+def test_far_wings_pointwise(O, ppl, split, order):
+    """Points whose whole value comes from far wings: 20,000 lines fill
+    1000-1400 cm^-1 and the points at 1405-1420 cm^-1 (within the 25 cm^-1
+    cutoff of the band's upper lines, 500 or more points from any of them)
+    are sums of the far-field expansion alone at 64- and 256-point tiles.
+    There the GPU must equal the FP64 oracle point by point, within 2e-6 of
+    each point's own value, not only of the spectrum's peak."""
+    rng = np.random.default_rng(31)
+    n = 20000
+    nu = np.sort(rng.uniform(1000.0, 1400.0, n))
+    lines = (nu, rng.uniform(0.0, 5000.0, n), 10 ** rng.uniform(-3.0, 1.0, n),
+             rng.integers(1, 10, n).astype(np.int32), rng.uniform(0.02, 0.10, n),
+             rng.uniform(0.3, 0.8, n))
+    T, P, Q, N = 296.0, 1.0, 1000.0, 100_000
+    out, st, _ = run_manager("voigt", lines, T, P, Q, 1000.0, 2000.0, N, 25.0, ppl=ppl,
+                             split=split, order=order)
+    ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 1000.0, 2000.0, N, 25.0)
+    assert st["n_eval"] == ne
+    check(out, ref, 1e-4, 1e-6, f"far wings ppl={ppl} split={split} order={order}")
+    gap = slice(40_500, 42_000)  # 1405-1420 cm^-1
+    assert np.all(ref[gap] > 0)
+    rel = np.max(np.abs(out[gap] - ref[gap]) / ref[gap])
+    log(f"  far wings ppl={ppl}: pointwise relative error at 1405-1420 cm^-1 {rel:.3e}")
+    assert rel < 2e-6
+
+
 def test_edge_cases(O):
     T, P, N = 296.0, 1.0, 1000
     start, end = 100.0, 110.0
