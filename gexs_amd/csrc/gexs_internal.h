@@ -54,9 +54,11 @@ struct __attribute__((aligned(16))) LinePos {
 //              its numerator Kl (x^2 + b) equals Kl (s + 1).)
 // Voigt-Quad:  cf, A, B, Kg as Voigt.
 // bin-Gauss:   cf, A, B as Voigt with 1/w = gG, Kg 2^ex = S * 0.5/dfreq.
+// (cf, A, B, Kl, E2, ex first: the far-field expansion reads those 24 bytes)
 struct __attribute__((aligned(32))) VRec {
-    float cf, A, B, Kl, E2, H, Kg;
+    float cf, A, B, Kl, E2;
     int32_t ex;
+    float H, Kg;
 };
 
 // per line, per (T,P): what the checked loop needs beyond VRec.

@@ -339,10 +339,19 @@ __device__ __forceinline__ void far_expand(float (&C)[kFarTerms], GlobRec grec, 
                                            float tcf, int E, int lane) {
     constexpr float T = (float)(GX_WAVE * PPL / 2);
     if (a >= b) return;
-    VRec vn = grec[min(a + lane, b - 1)];
+    // the record's first 24 bytes (cf, A, B, Kl | E2, ex): 16 + 8 byte loads
+    struct Far {
+        float cf, A, B, Kl, E2;
+        int ex;
+    };
+    auto load = [&](int i) {
+        const VRec* r = &grec[i];
+        return Far{r->cf, r->A, r->B, r->Kl, r->E2, r->ex};
+    };
+    Far vn = load(min(a + lane, b - 1));
     for (int base = a; base < b; base += GX_WAVE) {
-        const VRec v = vn;
-        vn = grec[min(base + GX_WAVE + lane, b - 1)];
+        const Far v = vn;
+        vn = load(min(base + GX_WAVE + lane, b - 1));
         const float Kl = base + lane < b ? ldexpf(v.Kl, v.ex - E) : 0.0f;
         const float Xc = fmaf(v.cf - tcf, v.A, v.B);
         const float inv = __builtin_amdgcn_rcpf(fmaf(Xc, Xc, v.E2 - 1.0f));
@@ -1024,6 +1033,38 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 }  // namespace
 
+// The item's segments: one wave per item searches them itself; the waves of
+// a split item take them from the first wave's search (segs, in LDS).  The
+// item's own __syncthreads before its partial sums keep segs unchanged until
+// every wave has read it.
+template <int PROF, int PPL, int SPLIT>
+__device__ __forceinline__ void item_segments(WorkItem& it, const LinePos* jp, const int* kmax,
+                                              int lane, int wv, int* segs) {
+    constexpr bool gauss = PROF == GX_PROF_DOPPLER;
+    if constexpr (SPLIT == 1) {
+        locate_segments<gauss>(it, jp, kmax, GX_WAVE * PPL / 2, lane);
+    } else {
+        if (wv == 0) {
+            locate_segments<gauss>(it, jp, kmax, GX_WAVE * PPL / 2, lane);
+            if (lane == 0) {
+                segs[0] = it.f2;
+                segs[1] = it.s2;
+                segs[2] = it.s3;
+                segs[3] = it.f3;
+                segs[4] = it.e2;
+                segs[5] = it.e3;
+            }
+        }
+        __syncthreads();
+        it.f2 = __builtin_amdgcn_readfirstlane(segs[0]);
+        it.s2 = __builtin_amdgcn_readfirstlane(segs[1]);
+        it.s3 = __builtin_amdgcn_readfirstlane(segs[2]);
+        it.f3 = __builtin_amdgcn_readfirstlane(segs[3]);
+        it.e2 = __builtin_amdgcn_readfirstlane(segs[4]);
+        it.e3 = __builtin_amdgcn_readfirstlane(segs[5]);
+    }
+}
+
 // single != 0: every item belongs to jobs[0], which the waves then load
 // without waiting for their item (no dependent load).  The choice is an index
 // select: a select between a loaded Job and one passed by value puts the Job's
@@ -1060,6 +1101,7 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
     constexpr int kRedWords = SPLIT > 1 ? 2 * SPLIT * GX_WAVE * PPL : 0;
     __shared__ __attribute__((aligned(16))) float smem[kXsWords > kRedWords ? kXsWords : kRedWords];
     __shared__ int grab;
+    __shared__ int segs[6];
     float* xs_all = smem;
     double* red = reinterpret_cast<double*>(smem);
     const int lane = threadIdx.x & (GX_WAVE - 1);
@@ -1077,13 +1119,13 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             if (idx >= nitems) return;
             WorkItem it = items[idx];
             const Job jb = jobs[single ? 0 : it.job];
-            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, GX_WAVE * PPL / 2, lane);
+            item_segments<PROF, PPL, 1>(it, jb.pos, kmax, lane, wv, segs);
             run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
         } else {
             if (lb >= nitems) return;
             WorkItem it = items[lb];
             const Job jb = jobs[single ? 0 : it.job];
-            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, GX_WAVE * PPL / 2, lane);
+            item_segments<PROF, PPL, SPLIT>(it, jb.pos, kmax, lane, wv, segs);
             run_item_split<PROF, PPL, SPLIT>(it, jb, scale_bits, lane, wv, red, xs);
         }
         return;
@@ -1109,7 +1151,7 @@ __global__ __launch_bounds__(block_threads<SPLIT>()) __attribute__((amdgpu_waves
             if (idx >= end) break;
             WorkItem it = items[idx];
             const Job jb = jobs[single ? 0 : it.job];
-            locate_segments<PROF == GX_PROF_DOPPLER>(it, jb.pos, kmax, GX_WAVE * PPL / 2, lane);
+            item_segments<PROF, PPL, SPLIT>(it, jb.pos, kmax, lane, wv, segs);
             if (SPLIT == 1)
                 run_item<PROF, PPL>(it, jb, scale_bits, lane, xs);
             else
