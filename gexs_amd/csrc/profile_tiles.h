@@ -407,6 +407,104 @@ __device__ __forceinline__ void far_lines(TileAcc<PPL>& t, GlobRec grec, int a, 
     far_add<PPL>(t, C, tcf);
 }
 
+// Window-edge lines by the far-field expansion.  HI: the lines [a, b) whose
+// windows end inside the tile (ie in (w0, w1], non-decreasing in nu order):
+// point j takes the lines with ie > j, a suffix of [a, b).  LO: the lines
+// whose windows start inside it (ib in [w0, w1), non-decreasing): point j
+// takes those with ib <= j, a prefix.  The window bounds go to the wave's LDS
+// block; each point finds its first line (suffix) or its end (prefix) by
+// binary search there.  The lines' coefficients (kEdgeTerms terms, lane k one
+// line per batch) are scanned across the lanes, batch after batch with the
+// sum so far carried, in the order suffix or prefix sums need, and each point
+// takes the scanned sum of its line when that line is in the batch.  The
+// lines must lie 14T + 1 points or more from the tile's centre (ratio
+// A T / |Xc| <= 0.072, so 8 terms leave under 9 (0.074)^8 / 0.926^2 = 1e-8);
+// the caller checks nothing else: false (nothing added) when the nearest
+// line is closer or the lines outnumber the LDS block, and the caller sums
+// them directly.
+constexpr int kEdgeTerms = 8;
+
+template <int PPL, bool HI>
+__device__ __forceinline__ bool edge_expand(TileAcc<PPL>& t, GlobPos gpos, GlobRec grec, int a,
+                                            int b, int w0, int lane, float* xs, int cap) {
+    constexpr int T = GX_WAVE * PPL / 2;
+    if (a >= b) return true;
+    const int n = b - a;
+    if (n > cap) return false;
+    const float tcf = (float)w0 + (float)T - 0.5f;
+    const int cnear = __builtin_amdgcn_readfirstlane(gpos[HI ? b - 1 : a].c);
+    const float dist = HI ? tcf - (float)cnear : (float)cnear - tcf;
+    if (dist < (float)(14 * T + 1)) return false;
+    int* E = reinterpret_cast<int*>(xs);
+    for (int i = lane; i < n; i += GX_WAVE) E[i] = HI ? gpos[a + i].ie : gpos[a + i].ib;
+    // m[q]: the number of lines with bound <= j, i.e. the first with bound
+    // > j (E is non-decreasing); a fixed number of halvings (n's bit length)
+    int m[PPL];
+    int steps = 0;
+    while ((1 << steps) <= n) ++steps;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        const int j = t.jj[q];
+        int lo = 0, len = n;  // the answer lies in [lo, lo + len]
+        for (int st = 0; st < steps; ++st) {
+            const int half = len >> 1;
+            const bool go = len > 0 && E[min(lo + half, n - 1)] <= j;
+            lo = go ? lo + half + 1 : lo;
+            len = len > 0 ? (go ? len - half - 1 : half) : 0;
+        }
+        m[q] = lo;
+    }
+    float carry[kEdgeTerms];
+#pragma unroll
+    for (int k = 0; k < kEdgeTerms; ++k) carry[k] = 0.0f;
+    for (int done = 0; done < n; done += GX_WAVE) {
+        const int nb = min(GX_WAVE, n - done);
+        // lane k's line: HI from the top down, LO from the bottom up
+        const int li = HI ? n - 1 - done - lane : done + lane;
+        const bool act = lane < nb;
+        const VRec* r = &grec[a + min(max(li, 0), n - 1)];
+        const float cf = r->cf, A = r->A, B = r->B, Klm = r->Kl, E2 = r->E2;
+        const int ex = r->ex;
+        const float Kl = act ? ldexpf(Klm, ex - t.E) : 0.0f;
+        const float Xc = fmaf(cf - tcf, A, B);
+        const float inv = __builtin_amdgcn_rcpf(fmaf(Xc, Xc, E2 - 1.0f));
+        const float u = A * (float)T;
+        const float al = (2.0f * u) * (Xc * inv);
+        const float be = -(u * u) * inv;
+        float c[kEdgeTerms];
+        c[0] = Kl * inv;
+        c[1] = al * c[0];
+#pragma unroll
+        for (int k = 2; k < kEdgeTerms; ++k) c[k] = fmaf(al, c[k - 1], be * c[k - 2]);
+        // inclusive prefix sums over the lanes, plus the lines before
+#pragma unroll
+        for (int k = 0; k < kEdgeTerms; ++k) {
+#pragma unroll
+            for (int d = 1; d < GX_WAVE; d <<= 1) {
+                const float o = __shfl_up(c[k], d);
+                c[k] += lane >= d ? o : 0.0f;
+            }
+            c[k] += carry[k];
+        }
+        // each point: its sum when its line is in this batch
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) {
+            // HI: suffix from line m (lane n - 1 - done - m); LO: prefix to line m - 1
+            const int want = HI ? n - 1 - done - m[q] : m[q] - 1 - done;
+            const bool here = want >= 0 && want < nb;
+            float p = __shfl(c[kEdgeTerms - 1], want & (GX_WAVE - 1));
+            const float tau = (t.jf[q] - tcf) * (1.0f / (float)T);
+#pragma unroll
+            for (int k = kEdgeTerms - 2; k >= 0; --k) p = fmaf(p, tau, __shfl(c[k], want & (GX_WAVE - 1)));
+            if (here) kahan_add(t.sum[q], t.cmp[q], p);
+        }
+#pragma unroll
+        for (int k = 0; k < kEdgeTerms; ++k)
+            carry[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[k]), nb - 1));
+    }
+    return true;
+}
+
 // Points outside the profile kernel's region I, and their value.  Voigt:
 // Humlicek's regions II-IV (|x| + y < 15).  Voigt-Quad: x^2 + y^2 < kQuadR2,
 // the Zaghloul-Ali core and the Gauss-Hermite wings (gx::quad_inner).  Beyond,
@@ -863,9 +961,11 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             const int f2 = min(max(it.f2, ca), cb), f3 = min(max(it.f3, ca), cb);
             const int e2 = min(max(it.e2, ca), cb), e3 = min(max(it.e3, ca), cb);
             const ConstPos pos = (ConstPos)jb.pos;
-            if (it.f2 > it.s1)
-                voigt_staged<1, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
-            else if (it.s2 > it.s1)
+            constexpr int cap = GX_WAVE * wave_row<PROF, 1>();
+            if (it.f2 > it.s1) {
+                if (!(it.e2 > it.s1 && edge_expand<1, true>(t, gpos, grec, ca, s1, w0, lane, xs, cap)))
+                    voigt_staged<1, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
+            } else if (it.s2 > it.s1)
                 voigt_staged<1, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskHi>(t, gpos, grec, grecx, ca, s1, w0, w1, lane, incl);
@@ -874,9 +974,10 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             lines_batched<PROF, kMaskNone>(t, gpos, grec, grecx, s2, s3, w0, w1, lane, incl);
             voigt_staged<1, kMaskNone>(t, grec, pos, s3, f3, w0, lane, xs);
             voigt_staged<1, kMaskNone, true>(t, grec, pos, f3, e3, w0, lane, xs);
-            if (it.f3 < it.s4)
-                voigt_staged<1, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
-            else if (it.s3 < it.s4)
+            if (it.f3 < it.s4) {
+                if (!(it.e3 < it.s4 && edge_expand<1, false>(t, gpos, grec, s4, cb, w0, lane, xs, cap)))
+                    voigt_staged<1, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
+            } else if (it.s3 < it.s4)
                 voigt_staged<1, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_batched<PROF, kMaskLo>(t, gpos, grec, grecx, s4, cb, w0, w1, lane, incl);
@@ -898,9 +999,12 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
             const int f2 = min(max(it.f2, ca), cb), f3 = min(max(it.f3, ca), cb);
             const int e2 = min(max(it.e2, ca), cb), e3 = min(max(it.e3, ca), cb);
             const GlobRec grec = (GlobRec)jb.rec;
-            if (it.f2 > it.s1)
-                voigt_staged<PPL, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
-            else if (it.s2 > it.s1)
+            constexpr int cap = GX_WAVE * wave_row<PROF, PPL>();
+            const GlobPos gpos = (GlobPos)jb.pos;
+            if (it.f2 > it.s1) {
+                if (!(it.e2 > it.s1 && edge_expand<PPL, true>(t, gpos, grec, ca, s1, w0, lane, xs, cap)))
+                    voigt_staged<PPL, kMaskHi, true>(t, grec, pos, ca, s1, w0, lane, xs);
+            } else if (it.s2 > it.s1)
                 voigt_staged<PPL, kMaskHi>(t, grec, pos, ca, s1, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskHi>(t, pos, rec, recx, ca, s1, w0, w1);
@@ -910,9 +1014,10 @@ __device__ __forceinline__ void sum_lines(TileAcc<PPL>& t, const WorkItem& it, c
                                    xs);
             voigt_staged<PPL, kMaskNone>(t, grec, pos, s3, f3, w0, lane, xs);
             voigt_staged<PPL, kMaskNone, true>(t, grec, pos, f3, e3, w0, lane, xs);
-            if (it.f3 < it.s4)
-                voigt_staged<PPL, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
-            else if (it.s3 < it.s4)
+            if (it.f3 < it.s4) {
+                if (!(it.e3 < it.s4 && edge_expand<PPL, false>(t, gpos, grec, s4, cb, w0, lane, xs, cap)))
+                    voigt_staged<PPL, kMaskLo, true>(t, grec, pos, s4, cb, w0, lane, xs);
+            } else if (it.s3 < it.s4)
                 voigt_staged<PPL, kMaskLo>(t, grec, pos, s4, cb, w0, lane, xs);
             else
                 lines_checked<PROF, PPL, kMaskLo>(t, pos, rec, recx, s4, cb, w0, w1);
