@@ -410,14 +410,18 @@ def test_execute_passes_equal_single_executes(O, c1_small):
 @pytest.mark.parametrize("ppl,split,order", [(4, 4, 3), (1, 1, 1), (2, 16, 1)])
 def test_far_wings_pointwise(O, ppl, split, order):
     """Points whose whole value comes from far wings: 20,000 lines fill
-    1000-1400 cm^-1 and the points at 1405-1420 cm^-1 (within the 25 cm^-1
-    cutoff of the band's upper lines, 500 or more points from any of them)
-    are sums of the far-field expansion alone at 64- and 256-point tiles.
+    1000-1400 cm^-1 and 1600-2000 cm^-1.  The points at 1405-1420 cm^-1 are
+    within the 25 cm^-1 cutoff of the lower band's upper lines only, 500 or
+    more points from any of them (windows that cover their tiles, and windows
+    that end inside them); those at 1580-1595 cm^-1 likewise of the upper
+    band's lower lines (windows that start inside their tiles).  At 64- and
+    256-point tiles those sums come from the far-field expansion alone.
     There the GPU must equal the FP64 oracle point by point, within 2e-6 of
     each point's own value, not only of the spectrum's peak."""
     rng = np.random.default_rng(31)
     n = 20000
-    nu = np.sort(rng.uniform(1000.0, 1400.0, n))
+    nu = np.sort(np.concatenate([rng.uniform(1000.0, 1400.0, n // 2),
+                                 rng.uniform(1600.0, 2000.0, n // 2)]))
     lines = (nu, rng.uniform(0.0, 5000.0, n), 10 ** rng.uniform(-3.0, 1.0, n),
              rng.integers(1, 10, n).astype(np.int32), rng.uniform(0.02, 0.10, n),
              rng.uniform(0.3, 0.8, n))
@@ -427,11 +431,11 @@ def test_far_wings_pointwise(O, ppl, split, order):
     ref, ne, _ = run_oracle(O, "voigt", lines, T, P, Q, 1000.0, 2000.0, N, 25.0)
     assert st["n_eval"] == ne
     check(out, ref, 1e-4, 1e-6, f"far wings ppl={ppl} split={split} order={order}")
-    gap = slice(40_500, 42_000)  # 1405-1420 cm^-1
-    assert np.all(ref[gap] > 0)
-    rel = np.max(np.abs(out[gap] - ref[gap]) / ref[gap])
-    log(f"  far wings ppl={ppl}: pointwise relative error at 1405-1420 cm^-1 {rel:.3e}")
-    assert rel < 2e-6
+    for gap, what in ((slice(40_500, 42_000), "1405-1420"), (slice(58_000, 59_500), "1580-1595")):
+        assert np.all(ref[gap] > 0)
+        rel = np.max(np.abs(out[gap] - ref[gap]) / ref[gap])
+        log(f"  far wings ppl={ppl}: pointwise relative error at {what} cm^-1 {rel:.3e}")
+        assert rel < 2e-6
 
 
 def test_edge_cases(O):
