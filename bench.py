@@ -585,7 +585,11 @@ def roofline_of(r, prof=None, probe=None):
            "frac_unpacked": r["achieved"] / (PEAK_FP32 / 2),
            "traffic": prof.get("hbm_bytes_per_launch") if prof else None,
            "kernel": f"k_profile_tiles<{r['profile'].upper()}>", "valu_probe": probe,
-           "kernel_ms_avg": r["k_avg"], "flop_per_eval": r["flops"]}
+           "kernel_ms_avg": r["k_avg"], "flop_per_eval": r["flops"],
+           "flop_note": "achieved counts the reference's flops per evaluation (12 Voigt, 16 "
+                        "bin-Gauss) for every line-point pair in the windows; the kernel sums far "
+                        "wings as a polynomial per tile and skips exact zeros, so frac can pass 1; "
+                        "valu_issue_frac is the hardware's own measure"}
     if r.get("k_alone"):
         # the kernel with nothing beside it (each execute synchronised)
         out["kernel_ms_alone"] = r["k_alone"]
