@@ -262,12 +262,12 @@ struct gexs_ctx {
     const WorkItem* last_items = nullptr;  // work items the last FP32C profile kernel read
     const int* last_kmax = nullptr;        // and the kmax it located their segments with
 
-    // execute path: the per-line pre-pass of a pass runs on `pre` into one of
-    // execute path: the passes alternate between `stream` and `pre`, so one
-    // pass's line records overlap the profile kernel of the pass before
+    // execute path: a pass's line records run on `pre` or on `stream` (see
+    // run_fp32c), so that they overlap the profile kernel of the pass before
     hipStream_t pre = nullptr;
     struct PassBuf {
         DevArr tab, small, rec, recx;
+        DevArr red;  // the line records' workgroup sums (gxk::kRecRedBytes)
         bool clean = false;  // counters zero (cleared by the pass two before)
         hipEvent_t pre_done = nullptr, done = nullptr;  // line records / profile kernel
         bool done_recorded = false;
@@ -615,9 +615,10 @@ int select_range(gexs_ctx* c, double lo, double hi, int64_t* l0, int64_t* l1) {
 }
 
 // The FP32C pass of one execute over the lines and window of c->pos_key into
-// out: one k_line_rec_pass launch (the pass's tables as kernel arguments,
-// its line records into buffer set k mod kPassSets, the counters of a later
-// pass's set cleared) and the profile kernel.  The line records of one pass
+// out: the line records (k_line_rec_pass with the pass's tables as kernel
+// arguments, its line records into buffer set k mod kPassSets, the counters
+// of a later pass's set cleared; then k_rec_reduce adds its workgroups' sums
+// to the pass's counters) and the profile kernel.  The line records of one pass
 // overlap the profile kernel of the pass before, in one of two modes:
 //   serial (profile kernels of kSerialKernelMs or more): line records on
 //   `pre`, clearing set k + 1's counters once the profile kernel of pass
@@ -662,6 +663,7 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     if ((rc = pass_tables_in(c, N.tab, N.small, 1, &tn)) != GEXS_OK) return rc;
     HIPCHK(c, B.rec.ensure((size_t)std::max(n, 1) * sizeof(VRec)));
     HIPCHK(c, B.recx.ensure((size_t)std::max(n, 1) * sizeof(VRecX)));
+    HIPCHK(c, B.red.ensure(gxk::kRecRedBytes));
     HIPCHK(c, c->counter.ensure(256));
     Job jb{};
     jb.pos = c->pos.as<LinePos>();
@@ -707,15 +709,14 @@ int run_fp32c(gexs_ctx* c, const TPParams& tp, double* out, bool timed) {
     HIPCHK(c, hipEventRecord(c->ev_start, P));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
-    // few blocks, so the line records run beside the profile kernel of the
-    // pass before without taking its slots: one per CU beside a long kernel
-    // (they fill its tail), half that beside a short one (c0 step 33 -> 30 us
-    // against one per CU; 40 us with two per CU)
+    // workgroups: four per CU beside a long kernel, one per CU beside a short
+    // one (sweeps of 1-8 per CU: 1e6-line Voigt steps 0.352-0.363 ms, 2e5-line
+    // Doppler steps 26-31 us; the best shapes are these, by 1-5 %)
     HIPCHK(c, gxk::line_rec_pass(P, tp, d, jb, t.jobs, (unsigned long long*)N.small.p, (int)tn.zero_u64,
                                  c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(),
                                  c->g.as<int32_t>(), gam, nex, c->pos.as<LinePos>(), t.scale,
-                                 B.rec.as<VRec>(), B.recx.as<VRecX>(), t.kmax, t.neval,
-                                 serial ? c->cus : std::max(1, c->cus / 2)));
+                                 B.rec.as<VRec>(), B.recx.as<VRecX>(), t.kmax, t.neval, B.red.p,
+                                 serial ? 4 * c->cus : c->cus));
     N.clean = true;
     if (serial) {
         HIPCHK(c, hipEventRecord(B.pre_done, P));

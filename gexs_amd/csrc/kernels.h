@@ -23,16 +23,23 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
                     unsigned long long* neval, int max_blocks = 0);
+// Scratch of one line_rec_pass: the sums of each of its workgroups, which a
+// one-workgroup kernel adds to the counters.  Passes that may run at the
+// same time need scratch of their own.
+constexpr int kRecMaxBlocks = 2048;
+constexpr size_t kRecPartBytes = 32;
+constexpr size_t kRecRedBytes = (size_t)kRecMaxBlocks * kRecPartBytes;
 // line_rec of one job whose tables are kernel arguments (tp, jd, and jb,
 // stored to djob for the profile kernel), its counters (emax, kmax, neval)
-// zero before; the launch also zeroes clear[0 .. nclear), a later pass's
-// counters (at least one block runs, also for jd.n = 0)
+// zero before, added to with four atomics per pass (through red); the
+// launch also zeroes clear[0 .. nclear), a later pass's counters (at least
+// one block runs, also for jd.n = 0; at most kRecMaxBlocks).  Two kernels.
 hipError_t line_rec_pass(hipStream_t s, const TPParams& tp, const JobDesc& jd, const Job& jb,
                          Job* djob, unsigned long long* clear, int nclear, const double* nu,
                          const double* E, const double* A, const int* g, const double* gamma,
                          const double* nexp, const LinePos* pos, unsigned long long* emax,
                          VRec* rec, VRecX* recx, int* kmax, unsigned long long* neval,
-                         int max_blocks = 0);
+                         void* red, int max_blocks = 0);
 // the segment bounds the profile kernel locates for every work item of tile
 // width span from kmax (f2, s2, s3, f3, e2, e3; the bin-Gauss profiles f2,
 // f3 only, the lines within

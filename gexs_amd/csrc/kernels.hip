@@ -214,23 +214,29 @@ __device__ __forceinline__ int line_record(const TPParams& p, bool voigt, double
     return amp > 0.0 ? ex : 0;
 }
 
+// the sums of one workgroup's lines, valid in thread 0
+struct RecSums {
+    long long ne;  // N_eval
+    int km;        // largest x.k15
+    int kf;        // largest kfar
+    int em;        // largest biased exponent of a line reaching the grid (0: none)
+};
+
 // the records of job k's lines (the blocks of one grid row share them)
-__device__ __forceinline__ void line_rec_job(const TPParams& p, const JobDesc& d, int k,
-                                             const double* __restrict__ nu,
-                                             const double* __restrict__ E,
-                                             const double* __restrict__ A,
-                                             const int* __restrict__ g,
-                                             const double* __restrict__ gamma,
-                                             const double* __restrict__ nexp,
-                                             const LinePos* __restrict__ pos,
-                                             unsigned long long* __restrict__ emax,
-                                             VRec* __restrict__ rec, VRecX* __restrict__ recx,
-                                             int* __restrict__ kmax,
-                                             unsigned long long* __restrict__ neval) {
+__device__ __forceinline__ RecSums line_rec_block(const TPParams& p, const JobDesc& d,
+                                                  const double* __restrict__ nu,
+                                                  const double* __restrict__ E,
+                                                  const double* __restrict__ A,
+                                                  const int* __restrict__ g,
+                                                  const double* __restrict__ gamma,
+                                                  const double* __restrict__ nexp,
+                                                  const LinePos* __restrict__ pos,
+                                                  VRec* __restrict__ rec,
+                                                  VRecX* __restrict__ recx) {
     const bool voigt = p.profile == GX_PROF_VOIGT || p.profile == GX_PROF_VOIGT_QUAD;
     long long ne = 0;
     int km = 0, kf = 0;
-    int em = 0;  // largest biased exponent of a line reaching the grid (0: none)
+    int em = 0;
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < d.n; ii += gridDim.x * blockDim.x) {
         const int64_t i = d.l0 + ii;
         const LinePos lp = pos[d.pos_off + ii];
@@ -252,17 +258,28 @@ __device__ __forceinline__ void line_rec_job(const TPParams& p, const JobDesc& d
     __shared__ int sh_km[4];
     __shared__ int sh_kf[4];
     __shared__ int sh_em[4];
-    ne = block_sum_ll(ne, sh_ne);
-    km = block_max_i(km, sh_km);
-    kf = block_max_i(kf, sh_kf);
-    em = block_max_i(em, sh_em);
-    if (threadIdx.x == 0) {
-        if (neval != nullptr && ne != 0) atomicAdd(&neval[k], (unsigned long long)ne);
-        if (kmax != nullptr && km > 0) atomicMax(&kmax[2 * k], km);
-        if (kmax != nullptr && kf > 0) atomicMax(&kmax[2 * k + 1], kf);
-        if (em > 0) atomicMax(&emax[k], (unsigned long long)em);
-    }
+    return RecSums{block_sum_ll(ne, sh_ne), block_max_i(km, sh_km), block_max_i(kf, sh_kf),
+                   block_max_i(em, sh_em)};
 }
+
+// adds one workgroup's (or launch's) sums to job k's counters (thread 0)
+__device__ __forceinline__ void rec_publish(const RecSums& s, int k, unsigned long long* emax,
+                                            int* kmax, unsigned long long* neval) {
+    if (threadIdx.x != 0) return;
+    if (neval != nullptr && s.ne != 0) atomicAdd(&neval[k], (unsigned long long)s.ne);
+    if (kmax != nullptr && s.km > 0) atomicMax(&kmax[2 * k], s.km);
+    if (kmax != nullptr && s.kf > 0) atomicMax(&kmax[2 * k + 1], s.kf);
+    if (s.em > 0) atomicMax(&emax[k], (unsigned long long)s.em);
+}
+
+using gxk::kRecMaxBlocks;
+using gxk::kRecPartBytes;
+
+struct alignas(32) RecPart {
+    long long ne;
+    int km, kf, em;
+};
+static_assert(sizeof(RecPart) == kRecPartBytes, "RecPart layout");
 
 __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
                                                   const double* __restrict__ E,
@@ -278,7 +295,8 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
                                                   int* __restrict__ kmax,
                                                   unsigned long long* __restrict__ neval) {
     const int k = blockIdx.y;
-    line_rec_job(tp[k], jd[k], k, nu, E, A, g, gamma, nexp, pos, emax, rec, recx, kmax, neval);
+    rec_publish(line_rec_block(tp[k], jd[k], nu, E, A, g, gamma, nexp, pos, rec, recx), k, emax,
+                kmax, neval);
 }
 
 // One pass of one job with its tables as kernel arguments: block 0 also
@@ -295,16 +313,42 @@ __global__ __launch_bounds__(256) void k_line_rec_pass(TPParams p, JobDesc d, Jo
                                                        const double* __restrict__ gamma,
                                                        const double* __restrict__ nexp,
                                                        const LinePos* __restrict__ pos,
-                                                       unsigned long long* __restrict__ emax,
                                                        VRec* __restrict__ rec,
                                                        VRecX* __restrict__ recx,
-                                                       int* __restrict__ kmax,
-                                                       unsigned long long* __restrict__ neval) {
+                                                       RecPart* __restrict__ parts) {
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) *djob = jb;
         for (int i = threadIdx.x; i < nclear; i += blockDim.x) clear[i] = 0ull;
     }
-    line_rec_job(p, d, 0, nu, E, A, g, gamma, nexp, pos, emax, rec, recx, kmax, neval);
+    const RecSums t = line_rec_block(p, d, nu, E, A, g, gamma, nexp, pos, rec, recx);
+    if (threadIdx.x == 0) parts[blockIdx.x] = RecPart{t.ne, t.km, t.kf, t.em};
+}
+
+// Adds the workgroup sums of one k_line_rec_pass launch (nb of them) to job
+// 0's counters: one workgroup, four atomics.  The launch's own workgroups do
+// not combine their sums: device-scope atomics on one address serialise, and
+// an agent-scope release writes back the XCD's L2, so either, once per
+// workgroup, made a 2048-workgroup launch several times longer than its loads
+// and stores.
+__global__ __launch_bounds__(256) void k_rec_reduce(const RecPart* __restrict__ parts, int nb,
+                                                    unsigned long long* __restrict__ emax,
+                                                    int* __restrict__ kmax,
+                                                    unsigned long long* __restrict__ neval) {
+    RecSums v{0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        const RecPart q = parts[i];
+        v.ne += q.ne;
+        v.km = max(v.km, q.km);
+        v.kf = max(v.kf, q.kf);
+        v.em = max(v.em, q.em);
+    }
+    __shared__ long long sh_ne[4];
+    __shared__ int sh_km[4];
+    __shared__ int sh_kf[4];
+    __shared__ int sh_em[4];
+    const RecSums t{block_sum_ll(v.ne, sh_ne), block_max_i(v.km, sh_km), block_max_i(v.kf, sh_kf),
+                    block_max_i(v.em, sh_em)};
+    rec_publish(t, 0, emax, kmax, neval);
 }
 
 // ---------------------------------------------------------------- k_tiles
@@ -622,11 +666,14 @@ hipError_t line_rec_pass(hipStream_t s, const TPParams& tp, const JobDesc& jd, c
                          const double* E, const double* A, const int* g, const double* gamma,
                          const double* nexp, const LinePos* pos, unsigned long long* emax,
                          VRec* rec, VRecX* recx, int* kmax, unsigned long long* neval,
-                         int max_blocks) {
-    const int gx = std::max(1, std::min(blocks_for(std::max<int64_t>(jd.n, 1), 256),
-                                        max_blocks > 0 ? max_blocks : 2048));
+                         void* red, int max_blocks) {
+    if (red == nullptr) return hipErrorInvalidValue;
+    const int cap = max_blocks > 0 ? std::min(max_blocks, kRecMaxBlocks) : kRecMaxBlocks;
+    const int gx = std::max(1, std::min(blocks_for(std::max<int64_t>(jd.n, 1), 256), cap));
     hipLaunchKernelGGL(k_line_rec_pass, dim3(gx), dim3(256), 0, s, tp, jd, jb, djob, clear, nclear,
-                       nu, E, A, g, gamma, nexp, pos, emax, rec, recx, kmax, neval);
+                       nu, E, A, g, gamma, nexp, pos, rec, recx, (RecPart*)red);
+    hipLaunchKernelGGL(k_rec_reduce, dim3(1), dim3(256), 0, s, (const RecPart*)red, gx, emax, kmax,
+                       neval);
     return hipGetLastError();
 }
 
