@@ -255,6 +255,7 @@ struct gexs_ctx {
     DevArr pos, rec, recx, lf64, items, items_tmp, cost_key, item_idx;
     DevArr tab;      // per-pass tables [TPParams | JobDesc | Job] x njobs (one upload)
     DevArr small;    // per-pass counters [xq | scale | neval | kmax] (one fill)
+    DevArr rec_parts;  // workgroup sums of a multi-job line-record launch (njobs x gxk::kRecRedBytes)
     DevArr neval;    // N_eval of the plain FP64 pass
     DevArr counter;  // bounds of the per-XCD tile queues
     DevArr item_tab; // window and job tables of the tiles kernel
@@ -1262,9 +1263,10 @@ static int enqueue_batch(gexs_ctx* c, int njobs, int nq, const PassTables& t) {
     }
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    HIPCHK(c, c->rec_parts.ensure((size_t)njobs * gxk::kRecRedBytes));
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(),
                             gam, nex, c->pos.as<LinePos>(), t.jd, njobs, n, t.tp, t.scale,
-                            c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval));
+                            c->rec.as<VRec>(), c->recx.as<VRecX>(), t.kmax, t.neval, 0, c->rec_parts.p));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items,
                                  t.jobs, t.scale, t.kmax, t.xq, c->counter.as<unsigned int>(),
                                  launch_mode(c), launch_blocks(c)));
@@ -1350,6 +1352,7 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
     double* hT = (double*)((char*)c->h_tab + t.bytes);
     HIPCHK(c, c->rec.ensure((size_t)std::max(n, 1) * ntp * sizeof(VRec)));
     HIPCHK(c, c->recx.ensure((size_t)std::max(n, 1) * ntp * sizeof(VRecX)));
+    HIPCHK(c, c->rec_parts.ensure((size_t)ntp * gxk::kRecRedBytes));  // before any capture
     HIPCHK(c, c->counter.ensure(256));
     HIPCHK(c, c->q_T.ensure((size_t)ntp * 8));
     HIPCHK(c, c->q_part.ensure((size_t)ntp * 256 * 8));
@@ -1609,9 +1612,10 @@ static int stream_chunk_pass(gexs_ctx* c, const std::vector<TPParams>& htp, int6
     HIPCHK(c, gxk::fill_u64(s, (unsigned long long*)t.kmax, ntp, 0ull));
     const double* gam = c->has_gamma ? c->gamma.as<double>() : nullptr;
     const double* nex = c->has_gamma ? c->nexp.as<double>() : nullptr;
+    HIPCHK(c, c->rec_parts.ensure((size_t)ntp * gxk::kRecRedBytes));
     HIPCHK(c, gxk::line_rec(s, c->nu.as<double>(), c->E.as<double>(), c->A.as<double>(), c->g.as<int32_t>(), gam,
                             nex, c->pos.as<LinePos>(), t.jd, ntp, n, t.tp, t.scale, c->rec.as<VRec>(),
-                            c->recx.as<VRecX>(), t.kmax, t.neval));
+                            c->recx.as<VRecX>(), t.kmax, t.neval, 0, c->rec_parts.p));
     HIPCHK(c, gxk::profile_tiles(s, c->profile, c->ppl, c->split, c->items.as<WorkItem>(), c->n_items, t.jobs,
                                  t.scale, t.kmax, t.xq, c->counter.as<unsigned int>(), launch_mode(c),
                                  launch_blocks(c)));
@@ -1724,6 +1728,7 @@ int gexs_stream_xsec(gexs_ctx* c, const gexs_states* st, const gexs_transitions*
     } else {
         HIPCHK(c, c->rec.ensure(cb * (size_t)ntp * sizeof(VRec)));
         HIPCHK(c, c->recx.ensure(cb * (size_t)ntp * sizeof(VRecX)));
+        HIPCHK(c, c->rec_parts.ensure((size_t)ntp * gxk::kRecRedBytes));
         HIPCHK(c, c->counter.ensure(256));
         PassTables t;
         if ((rc = pass_tables(c, ntp, &t)) != GEXS_OK) return rc;

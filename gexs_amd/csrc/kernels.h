@@ -17,15 +17,19 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
 // kmax[2k + 1] its largest kfar (Voigt) or reach (bin-Gauss), each zero
 // before.
 // max_blocks > 0 caps the blocks per job (a pass that runs beside another
-// kernel gets its few blocks dispatched at once); 0: up to 2048.
+// kernel gets its few blocks dispatched at once); 0: up to kRecMaxBlocks.
+// parts (njobs * kRecRedBytes, or nullptr): each workgroup stores its sums
+// there and a second kernel, one workgroup per job, adds them to the
+// counters (four atomics per job); nullptr: four atomics per workgroup,
+// which serialise on the job's counters (tens of ns each).
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
-                    unsigned long long* neval, int max_blocks = 0);
-// Scratch of one line_rec_pass: the sums of each of its workgroups, which a
-// one-workgroup kernel adds to the counters.  Passes that may run at the
-// same time need scratch of their own.
+                    unsigned long long* neval, int max_blocks = 0, void* parts = nullptr);
+// Scratch of one job of a line-record launch: the sums of each of its
+// workgroups, which a kernel of one workgroup per job adds to the counters.
+// Launches that may run at the same time need scratch of their own.
 constexpr int kRecMaxBlocks = 2048;
 constexpr size_t kRecPartBytes = 32;
 constexpr size_t kRecRedBytes = (size_t)kRecMaxBlocks * kRecPartBytes;

@@ -293,10 +293,14 @@ __global__ __launch_bounds__(256) void k_line_rec(const double* __restrict__ nu,
                                                   unsigned long long* __restrict__ emax,
                                                   VRec* __restrict__ rec, VRecX* __restrict__ recx,
                                                   int* __restrict__ kmax,
-                                                  unsigned long long* __restrict__ neval) {
+                                                  unsigned long long* __restrict__ neval,
+                                                  RecPart* __restrict__ parts) {
     const int k = blockIdx.y;
-    rec_publish(line_rec_block(tp[k], jd[k], nu, E, A, g, gamma, nexp, pos, rec, recx), k, emax,
-                kmax, neval);
+    const RecSums t = line_rec_block(tp[k], jd[k], nu, E, A, g, gamma, nexp, pos, rec, recx);
+    if (parts == nullptr)
+        rec_publish(t, k, emax, kmax, neval);
+    else if (threadIdx.x == 0)
+        parts[(size_t)k * kRecMaxBlocks + blockIdx.x] = RecPart{t.ne, t.km, t.kf, t.em};
 }
 
 // One pass of one job with its tables as kernel arguments: block 0 also
@@ -324,16 +328,18 @@ __global__ __launch_bounds__(256) void k_line_rec_pass(TPParams p, JobDesc d, Jo
     if (threadIdx.x == 0) parts[blockIdx.x] = RecPart{t.ne, t.km, t.kf, t.em};
 }
 
-// Adds the workgroup sums of one k_line_rec_pass launch (nb of them) to job
-// 0's counters: one workgroup, four atomics.  The launch's own workgroups do
-// not combine their sums: device-scope atomics on one address serialise, and
-// an agent-scope release writes back the XCD's L2, so either, once per
-// workgroup, made a 2048-workgroup launch several times longer than its loads
-// and stores.
+// Adds the workgroup sums of a line-record launch (nb per job, job k's at
+// parts + k kRecMaxBlocks) to job k's counters: one workgroup per job, four
+// atomics.  The launch's own workgroups do not combine their sums:
+// device-scope atomics on one address serialise, and an agent-scope release
+// writes back the XCD's L2, so either, once per workgroup, made a
+// 2048-workgroup launch several times longer than its loads and stores.
 __global__ __launch_bounds__(256) void k_rec_reduce(const RecPart* __restrict__ parts, int nb,
                                                     unsigned long long* __restrict__ emax,
                                                     int* __restrict__ kmax,
                                                     unsigned long long* __restrict__ neval) {
+    const int k = blockIdx.x;
+    parts += (size_t)k * kRecMaxBlocks;
     RecSums v{0, 0, 0, 0};
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
         const RecPart q = parts[i];
@@ -348,7 +354,7 @@ __global__ __launch_bounds__(256) void k_rec_reduce(const RecPart* __restrict__ 
     __shared__ int sh_em[4];
     const RecSums t{block_sum_ll(v.ne, sh_ne), block_max_i(v.km, sh_km), block_max_i(v.kf, sh_kf),
                     block_max_i(v.em, sh_em)};
-    rec_publish(t, 0, emax, kmax, neval);
+    rec_publish(t, k, emax, kmax, neval);
 }
 
 // ---------------------------------------------------------------- k_tiles
@@ -653,11 +659,15 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
-                    unsigned long long* neval, int max_blocks) {
+                    unsigned long long* neval, int max_blocks, void* parts) {
     if (njobs <= 0 || max_n <= 0) return hipSuccess;
-    int gx = std::min(blocks_for(max_n, 256), max_blocks > 0 ? max_blocks : 2048);
+    const int cap = max_blocks > 0 ? std::min(max_blocks, kRecMaxBlocks) : kRecMaxBlocks;
+    const int gx = std::min(blocks_for(max_n, 256), cap);
     hipLaunchKernelGGL(k_line_rec, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, gamma, nexp, pos,
-                       jd, tp, emax, rec, recx, kmax, neval);
+                       jd, tp, emax, rec, recx, kmax, neval, (RecPart*)parts);
+    if (parts != nullptr)
+        hipLaunchKernelGGL(k_rec_reduce, dim3(njobs), dim3(256), 0, s, (const RecPart*)parts, gx, emax,
+                           kmax, neval);
     return hipGetLastError();
 }
 

@@ -410,10 +410,12 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
     const size_t t_b = align_up(Ts.size() * 8);
     const size_t tab_b = tp_b + jd_b + jobs_b + t_b;
     // small arrays: scale, neval, kmax (kMaxBatch u64 each; cleared by one
-    // fill), counter, Q, partials
+    // fill), counter, Q, partition partials, line-record workgroup sums
     const size_t sm_scale = 0, sm_neval = kMaxBatch * 8, sm_kmax = 2 * kMaxBatch * 8,
                  sm_counter = 3 * kMaxBatch * 8, sm_q = sm_counter + 256,
-                 sm_part = sm_q + kMaxBatch * 8, sm_b = sm_part + (size_t)kMaxBatch * kQBlocks * 8;
+                 sm_part = sm_q + kMaxBatch * 8,
+                 sm_rp = align_up(sm_part + (size_t)kMaxBatch * kQBlocks * 8),
+                 sm_b = sm_rp + (size_t)kMaxBatch * gxk::kRecRedBytes;
     hipError_t e = hipSuccess;
     size_t dummy_cap = 0;
     if (!s.small && (e = grow(&s.small, &dummy_cap, sm_b)) != hipSuccess) goto fail;
@@ -468,8 +470,8 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         if ((e = gxk::line_pos(st, v_.nu, d_jd, nreq, max_n, d_tp, (LinePos*)base)) != hipSuccess)
             goto fail;
         if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, (const LinePos*)base, d_jd,
-                               nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval)) !=
-            hipSuccess)
+                               nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval, 0,
+                               sm + sm_rp)) != hipSuccess)
             goto fail;
         for (int k0 = 0; k0 < nreq;) {  // tiles per tile-width group
             int k1 = k0, mt = 0;
