@@ -22,11 +22,15 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
 // there and a second kernel, one workgroup per job, adds them to the
 // counters (four atomics per job); nullptr: four atomics per workgroup,
 // which serialise on the job's counters (tens of ns each).
+// pos_out (or nullptr): the launch also computes every line's window, as
+// line_pos does, stores it there (at the jobs' pos_off) and uses it instead
+// of reading pos: one launch and one dependency fewer.
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
-                    unsigned long long* neval, int max_blocks = 0, void* parts = nullptr);
+                    unsigned long long* neval, int max_blocks = 0, void* parts = nullptr,
+                    LinePos* pos_out = nullptr);
 // Scratch of one job of a line-record launch: the sums of each of its
 // workgroups, which a kernel of one workgroup per job adds to the counters.
 // Launches that may run at the same time need scratch of their own.

@@ -467,11 +467,10 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         if ((e = gxk::partition(st, v_.sE, v_.sg, v_.n_states, d_T, nreq, d_part, kQBlocks, d_Q, d_jd, nreq,
                                 d_tp, d_scale, 3 * kMaxBatch)) != hipSuccess)
             goto fail;
-        if ((e = gxk::line_pos(st, v_.nu, d_jd, nreq, max_n, d_tp, (LinePos*)base)) != hipSuccess)
-            goto fail;
-        if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, (const LinePos*)base, d_jd,
+        // windows and records of every request's lines in one launch
+        if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, nullptr, d_jd,
                                nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval, 0,
-                               sm + sm_rp)) != hipSuccess)
+                               sm + sm_rp, (LinePos*)base)) != hipSuccess)
             goto fail;
         for (int k0 = 0; k0 < nreq;) {  // tiles per tile-width group
             int k1 = k0, mt = 0;
