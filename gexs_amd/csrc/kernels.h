@@ -30,7 +30,9 @@ hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const doub
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
                     unsigned long long* neval, int max_blocks = 0, void* parts = nullptr,
-                    LinePos* pos_out = nullptr);
+                    LinePos* pos_out = nullptr, bool reduce = true);
+// the workgroups per job of a line_rec launch over max_n lines
+int rec_blocks(int64_t max_n, int max_blocks = 0);
 // Scratch of one job of a line-record launch: the sums of each of its
 // workgroups, which a kernel of one workgroup per job adds to the counters.
 // Launches that may run at the same time need scratch of their own.
@@ -56,9 +58,19 @@ hipError_t line_rec_pass(hipStream_t s, const TPParams& tp, const JobDesc& jd, c
 hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const LinePos* pos,
                     const int* kmax, int profile, int span);
 // jobs job0 .. job0 + njobs - 1 of jd / tp, all with tiles of span points
+// red (or nullptr): a line_rec launch's workgroup sums not yet reduced
+// (line_rec with reduce = false); one more workgroup per job adds them to
+// jobs job0 .. job0 + njobs - 1's counters, in place of line_rec's second kernel
+struct RecReduce {
+    const void* parts;  // line_rec's parts
+    int nb;             // its workgroups per job (rec_blocks)
+    unsigned long long* emax;
+    int* kmax;
+    unsigned long long* neval;
+};
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
-                 uint32_t* item_idx, int job0 = 0);
+                 uint32_t* item_idx, int job0 = 0, const RecReduce* red = nullptr);
 // items of tiles cut into line groups (WorkItem::ngroups > 1): every point of
 // such a tile = the sum of its groups' rows of Job::part (span = 64 * PPL)
 hipError_t group_sum(hipStream_t s, const WorkItem* items, int nitems, const Job* jobs, int span);

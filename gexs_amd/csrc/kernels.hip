@@ -342,11 +342,10 @@ __global__ __launch_bounds__(256) void k_line_rec_pass(TPParams p, JobDesc d, Jo
 // device-scope atomics on one address serialise, and an agent-scope release
 // writes back the XCD's L2, so either, once per workgroup, made a
 // 2048-workgroup launch several times longer than its loads and stores.
-__global__ __launch_bounds__(256) void k_rec_reduce(const RecPart* __restrict__ parts, int nb,
-                                                    unsigned long long* __restrict__ emax,
-                                                    int* __restrict__ kmax,
-                                                    unsigned long long* __restrict__ neval) {
-    const int k = blockIdx.x;
+__device__ __forceinline__ void rec_reduce_job(const RecPart* __restrict__ parts, int nb, int k,
+                                               unsigned long long* __restrict__ emax,
+                                               int* __restrict__ kmax,
+                                               unsigned long long* __restrict__ neval) {
     parts += (size_t)k * kRecMaxBlocks;
     RecSums v{0, 0, 0, 0};
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
@@ -365,6 +364,13 @@ __global__ __launch_bounds__(256) void k_rec_reduce(const RecPart* __restrict__ 
     rec_publish(t, k, emax, kmax, neval);
 }
 
+__global__ __launch_bounds__(256) void k_rec_reduce(const RecPart* __restrict__ parts, int nb,
+                                                    unsigned long long* __restrict__ emax,
+                                                    int* __restrict__ kmax,
+                                                    unsigned long long* __restrict__ neval) {
+    rec_reduce_job(parts, nb, blockIdx.x, emax, kmax, neval);
+}
+
 // ---------------------------------------------------------------- k_tiles
 // One work item per tile: lines whose window meets [w0, w1) form the
 // contiguous range [lo, hi) because ib and ie are non-decreasing in nu.
@@ -374,8 +380,12 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
                                                const TPParams* __restrict__ tp, int span,
                                                int job0, WorkItem* __restrict__ items,
                                                uint32_t* __restrict__ cost_key,
-                                               uint32_t* __restrict__ item_idx) {
+                                               uint32_t* __restrict__ item_idx, gxk::RecReduce red) {
     const int job = job0 + blockIdx.y;
+    if (red.parts != nullptr && blockIdx.x == gridDim.x - 1) {  // the job's line-record sums
+        rec_reduce_job((const RecPart*)red.parts, red.nb, job, red.emax, red.kmax, red.neval);
+        return;
+    }
     const JobDesc d = jd[job];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.ntiles) return;
@@ -663,17 +673,22 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
     return hipGetLastError();
 }
 
+int rec_blocks(int64_t max_n, int max_blocks) {
+    const int cap = max_blocks > 0 ? std::min(max_blocks, kRecMaxBlocks) : kRecMaxBlocks;
+    return std::min(blocks_for(std::max<int64_t>(max_n, 1), 256), cap);
+}
+
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
                     unsigned long long* emax, VRec* rec, VRecX* recx, int* kmax,
-                    unsigned long long* neval, int max_blocks, void* parts, LinePos* pos_out) {
+                    unsigned long long* neval, int max_blocks, void* parts, LinePos* pos_out,
+                    bool reduce) {
     if (njobs <= 0 || max_n <= 0) return hipSuccess;
-    const int cap = max_blocks > 0 ? std::min(max_blocks, kRecMaxBlocks) : kRecMaxBlocks;
-    const int gx = std::min(blocks_for(max_n, 256), cap);
+    const int gx = rec_blocks(max_n, max_blocks);
     hipLaunchKernelGGL(k_line_rec, dim3(gx, njobs), dim3(256), 0, s, nu, E, A, g, gamma, nexp, pos,
                        jd, tp, emax, rec, recx, kmax, neval, (RecPart*)parts, pos_out);
-    if (parts != nullptr)
+    if (parts != nullptr && reduce)
         hipLaunchKernelGGL(k_rec_reduce, dim3(njobs), dim3(256), 0, s, (const RecPart*)parts, gx, emax,
                            kmax, neval);
     return hipGetLastError();
@@ -706,10 +721,12 @@ hipError_t segments(hipStream_t s, WorkItem* items, int nitems, const LinePos* p
 
 hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs, int max_tiles,
                  const TPParams* tp, int span, WorkItem* items, uint32_t* cost_key,
-                 uint32_t* item_idx, int job0) {
-    if (njobs <= 0 || max_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(max_tiles, 256), njobs), dim3(256), 0, s, pos, jd,
-                       tp, span, job0, items, cost_key, item_idx);
+                 uint32_t* item_idx, int job0, const RecReduce* red) {
+    if (njobs <= 0 || (max_tiles <= 0 && red == nullptr)) return hipSuccess;
+    const RecReduce r = red ? *red : RecReduce{nullptr, 0, nullptr, nullptr, nullptr};
+    const int gx = blocks_for(std::max(max_tiles, 0), 256) + (red ? 1 : 0);
+    hipLaunchKernelGGL(k_tiles, dim3(gx, njobs), dim3(256), 0, s, pos, jd, tp, span, job0, items,
+                       cost_key, item_idx, r);
     return hipGetLastError();
 }
 

@@ -467,17 +467,20 @@ int Server::launch(Slot& s, std::vector<Req*>& batch, std::string* err) {
         if ((e = gxk::partition(st, v_.sE, v_.sg, v_.n_states, d_T, nreq, d_part, kQBlocks, d_Q, d_jd, nreq,
                                 d_tp, d_scale, 3 * kMaxBatch)) != hipSuccess)
             goto fail;
-        // windows and records of every request's lines in one launch
+        // windows and records of every request's lines in one launch; its
+        // workgroup sums are added to the counters by the tile launches below
         if ((e = gxk::line_rec(st, v_.nu, v_.E, v_.A, v_.g, v_.gamma, v_.nexp, nullptr, d_jd,
                                nreq, max_n, d_tp, d_scale, (VRec*)base, (VRecX*)base, d_kmax, d_neval, 0,
-                               sm + sm_rp, (LinePos*)base)) != hipSuccess)
+                               sm + sm_rp, (LinePos*)base, false)) != hipSuccess)
             goto fail;
+        const gxk::RecReduce red{sm + sm_rp, gxk::rec_blocks(max_n), d_scale, d_kmax, d_neval};
+        const gxk::RecReduce* redp = max_n > 0 ? &red : nullptr;  // no launch, no sums
         for (int k0 = 0; k0 < nreq;) {  // tiles per tile-width group
             int k1 = k0, mt = 0;
             while (k1 < nreq && shape(batch[(size_t)k1]) == shape(batch[(size_t)k0]))
                 mt = std::max(mt, batch[(size_t)k1++]->ntiles);
             if ((e = gxk::tiles(st, (const LinePos*)base, d_jd, k1 - k0, mt, d_tp,
-                                GX_WAVE * batch[(size_t)k0]->ppl, d_items, nullptr, nullptr, k0)) !=
+                                GX_WAVE * batch[(size_t)k0]->ppl, d_items, nullptr, nullptr, k0, redp)) !=
                 hipSuccess)
                 goto fail;
             k0 = k1;
