@@ -104,6 +104,7 @@ def test_voigt_fp32c_vs_oracle(O, c1_small, c1_small_ref, ppl, split, order):
                              split=split, order=order)
     assert st["n_eval"] == ne
     check(out, ref, 1e-4, 1e-6, f"voigt fp32c ppl={ppl} split={split} order={order}")
+    check_item_ranges(m, lines[0].shape[0])
     if split > 1:
         # the waves of a tile take three-piece shares (before, inside and after
         # the near-centre lines [s2, s3)) where a tile has >= split near lines;
@@ -111,6 +112,26 @@ def test_voigt_fp32c_vs_oracle(O, c1_small, c1_small_ref, ppl, split, order):
         # covered on the 0.1 cm^-1 grid of test_voigt_coarse_grids)
         three = three_piece_tiles(m, st, ppl, split)
         assert three > 0 or (ppl, split) == (1, 16)
+
+
+def check_item_ranges(m, n):
+    """Every work item's line range [lo, hi) and whole-tile lines [s1, s4)
+    equal numpy's bisection of the pass's windows (ib and ie are
+    non-decreasing in nu): lo = first ie > w0, hi = first ib >= w1, and the
+    lines covering [w0, w1) run from the first ie >= w1 to the first ib > w0,
+    within [lo, hi) (s1 = s4 = hi when none does)."""
+    _, ib, ie = m.line_windows(n)
+    it = m.pass_items()
+    lo, hi, w0, w1, s1, s4 = (it[:, k].astype(np.int64) for k in (0, 1, 2, 3, 4, 7))
+    elo = np.searchsorted(ie, w0, side="right")
+    ehi = np.searchsorted(ib, w1, side="left")
+    lf = np.clip(np.searchsorted(ie, w1, side="left"), elo, ehi)
+    rf = np.clip(np.searchsorted(ib, w0, side="right"), elo, ehi)
+    cover = lf < rf
+    assert np.array_equal(lo, elo) and np.array_equal(hi, ehi)
+    assert np.array_equal(s1, np.where(cover, lf, ehi))
+    assert np.array_equal(s4, np.where(cover, rf, ehi))
+    assert np.any(cover) and np.any(hi > s4)
 
 
 def three_piece_tiles(m, st, ppl, split):
