@@ -25,6 +25,8 @@ hipError_t line_pos(hipStream_t s, const double* nu, const JobDesc* jd, int njob
 // pos_out (or nullptr): the launch also computes every line's window, as
 // line_pos does, stores it there (at the jobs' pos_off) and uses it instead
 // of reading pos: one launch and one dependency fewer.
+// reduce = false (with parts): the second kernel is left out, and the
+// caller passes the sums to the tiles launch (RecReduce) instead.
 hipError_t line_rec(hipStream_t s, const double* nu, const double* E, const double* A,
                     const int* g, const double* gamma, const double* nexp, const LinePos* pos,
                     const JobDesc* jd, int njobs, int64_t max_n, const TPParams* tp,
