@@ -1402,7 +1402,8 @@ int gexs_run_batch(gexs_ctx* c, const gexs_tp* tp, int ntp, const gexs_grid* gri
                       grid->start, grid->end, cutoff, c->rec.p, c->recx.p, c->batch_out.p,
                       c->items.p, c->pos.p, c->tab.p, c->small.p, c->counter.p, c->h_tab,
                       c->st_E.p, c->st_g.p, c->nu.p, c->q_T.p, c->q_part.p);
-        const std::string k2 = std::string(key) + "|" + std::to_string((uintptr_t)c->q_Q.p);
+        const std::string k2 = std::string(key) + "|" + std::to_string((uintptr_t)c->q_Q.p) + "|" +
+                               std::to_string((uintptr_t)c->rec_parts.p);
         if (!c->graph_exec || c->graph_key != k2) {
             if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
             if (c->graph) (void)hipGraphDestroy(c->graph);

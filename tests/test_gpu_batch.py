@@ -123,6 +123,12 @@ def test_graph_follows_tuning_and_buffer_changes(batch_mgr):
     m.run_batch(table(12), *GRID)  # grows the per-pass tables
     m.set_launch(1024, True)
     assert np.array_equal(m.run_batch(tps, *GRID), eager[1])
+    # one (T,P) more grows the line records' workgroup-sum scratch by a whole
+    # job's share, while the 256-byte-rounded tables may keep their size
+    m.set_launch(1024, False)
+    m.run_batch(table(13), *GRID)
+    m.set_launch(1024, True)
+    assert np.array_equal(m.run_batch(tps, *GRID), eager[1])
     m.set_tuning(4, 1, 1)
     m.set_launch(0, False)
 
