@@ -387,7 +387,8 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
         return;
     }
     const JobDesc d = jd[job];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (GX_WAVE - 1);
+    const int t = blockIdx.x * (blockDim.x / GX_WAVE) + (threadIdx.x >> 6);  // one wave per tile
     if (t >= d.ntiles) return;
     const int win_lo = tp[job].win_lo, win_hi = tp[job].win_hi;
     const LinePos* jp = pos + d.pos_off;
@@ -398,26 +399,33 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     // window covers the whole tile are [lf, rf) within [lo, hi): lf = first
     // with ie >= w1, rf = first with ib > w0.  ib and ie are non-decreasing
     // in nu, and every line before lo has ib <= max(ie, 0) <= w0 < w1, so each
-    // bound is also the first over all [0, n), clamped to hi for lf and rf:
-    // four independent bisections, stepped together so that their dependent
-    // loads overlap (one chain after another took ~24 us on a 5e5-line job)
+    // bound is also the first over all [0, n), clamped to hi for lf and rf.
+    // The wave searches 64-way: each round the lanes test 64 evenly spaced
+    // lines of a bound's range [a, b] (the bound is b when no line in [a, b)
+    // qualifies) and the ballot keeps the stretch before the first that
+    // does, so 5e5 lines take 4-5 rounds; the four bounds share the rounds
+    // (one thread per tile, bisecting, took 14-30 us per launch).
     int a[4] = {0, 0, 0, 0}, b[4] = {n, n, n, n};
     while (a[0] < b[0] || a[1] < b[1] || a[2] < b[2] || a[3] < b[3]) {
+        int st[4], i[4];
         LinePos q[4];
-        int m[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            m[k] = min((a[k] + b[k]) >> 1, n - 1);  // n >= 1 here; a finished chain reloads
-            q[k] = jp[m[k]];
+            st[k] = max((b[k] - a[k] + GX_WAVE - 1) / GX_WAVE, 1);
+            i[k] = a[k] + lane * st[k];
+            q[k] = jp[min(i[k], max(b[k] - 1, 0))];  // n >= 1 here
         }
         const bool up[4] = {q[0].ie > w0, q[1].ib >= w1, q[2].ie >= w1, q[3].ib > w0};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (a[k] < b[k]) {
-                if (up[k])
-                    b[k] = m[k];
-                else
-                    a[k] = m[k] + 1;
+            if (a[k] >= b[k]) continue;  // wave-uniform
+            const unsigned long long m = __ballot(i[k] < b[k] && up[k]);
+            if (m == 0ull) {
+                a[k] += ((b[k] - 1 - a[k]) / st[k]) * st[k] + 1;  // past the last sample
+            } else {
+                const int f = __builtin_ctzll(m);
+                b[k] = a[k] + f * st[k];
+                if (f > 0) a[k] += (f - 1) * st[k] + 1;
             }
         }
     }
@@ -441,6 +449,7 @@ __global__ __launch_bounds__(256) void k_tiles(const LinePos* __restrict__ pos,
     it.e3 = it.s4;
     const int G = max(d.groups, 1);
     it.ngroups = G;
+    if (lane != 0) return;
     for (int g = 0; g < G; ++g) {
         const int slot = d.items_off + t * G + g;
         it.group = g;
@@ -724,7 +733,7 @@ hipError_t tiles(hipStream_t s, const LinePos* pos, const JobDesc* jd, int njobs
                  uint32_t* item_idx, int job0, const RecReduce* red) {
     if (njobs <= 0 || (max_tiles <= 0 && red == nullptr)) return hipSuccess;
     const RecReduce r = red ? *red : RecReduce{nullptr, 0, nullptr, nullptr, nullptr};
-    const int gx = blocks_for(std::max(max_tiles, 0), 256) + (red ? 1 : 0);
+    const int gx = blocks_for(std::max(max_tiles, 0), 256 / GX_WAVE) + (red ? 1 : 0);  // a wave per tile
     hipLaunchKernelGGL(k_tiles, dim3(gx, njobs), dim3(256), 0, s, pos, jd, tp, span, job0, items,
                        cost_key, item_idx, r);
     return hipGetLastError();
